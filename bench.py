@@ -1,0 +1,175 @@
+"""PF-FLIP step benchmark on one MI355X.
+
+Workload: z-up dam break, 128^3 MAC grid, 27 particles per cell on a 3^3
+seeded lattice with +-0.25 sub-cell jitter (56.6M particles), liquid
+x < 0.4, z < 0.6, 1000:1 density contrast, FP32 particles, MG-PCG to a 1e-6
+relative (inf-norm) residual, droplet stage off.  A step is one full
+``Simulator.step()``: CFL dt, binning, P2G, eq. 7 / beta / gravity / RHS,
+MG-PCG, projection, extrapolation, G2P + RK3 advection.
+
+Prints one JSON line: particle-steps/s over the timed steps, the P2G roofline
+(algorithmic bytes / measured average kernel time vs 8 TB/s), the FP32 vs
+BF16 MG-PCG time to 1e-6 on the last step's captured system, per-stage
+milliseconds, and the CPU baseline (the FP64 numpy restatement of the
+reference step on a bounded sample, one thread).
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def p2g_bytes(n_particles, n_cells):
+    """Algorithmic bytes of one P2G launch: x, v (12 B each) and a 1-byte
+    phase per particle; face mass x3, momentum x3 and cell mass (4 B each)."""
+    return 25 * n_particles + 28 * n_cells
+
+
+def cpu_baseline(seconds_budget=30.0):
+    """FP64 numpy restatement of the reference step (oracle/ref_np.py), one
+    thread, on a bounded sample: the same dam break at 32^3 x 27 ppc."""
+    from oracle import ref_np as O
+    n = 32
+    cfg = O.OracleConfig(res=(n, n, n), dx=1.0 / n, ppc=27, gravity=(0.0, 0.0, -9.81),
+                         pcg_tol=1e-6, pcg_max_iter=400)
+    sx, sy, sz = cfg.domain_size
+    pos, vel, pt = O.seed_particles(cfg, lambda p: (p[:, 0] < 0.4 * sx) & (p[:, 2] < 0.6 * sz),
+                                    jitter=0.5)
+    sim = O.OracleSim(cfg)
+    sim.set_particles(pos, vel, pt)
+    steps, t0 = 0, time.perf_counter()
+    while steps < 3:
+        sim.step()
+        steps += 1
+        if time.perf_counter() - t0 > seconds_budget:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": len(pos) * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/ref_np.py FP64 step, dam break 32^3 x 27 ppc ({len(pos)} particles), "
+                      f"{steps} steps in {dt:.1f} s"}
+
+
+def solve_compare(pf, torch, sim, cfg, tol):
+    """Time FP32 and BF16 MG-PCG on the same captured system (beta, rhs)."""
+    out = {}
+    beta = [b.clone() for b in sim.beta]
+    rhs = sim.rhs.clone()
+    for prec in ("fp32", "bf16"):
+        s = pf.MGPCG(sim.res, cfg.device, omega=cfg.mg_omega, pre=cfg.mg_pre_sweeps,
+                     post=cfg.mg_post_sweeps, coarsest=cfg.mg_coarsest_sweeps,
+                     min_dim=cfg.mg_min_dim, precision=prec)
+        p = torch.zeros_like(rhs)
+        s.build_hierarchy(*beta)
+        s.solve(rhs, p, tol=tol, max_iter=400)          # warm-up
+        reps, times = 3, []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            it, rel = s.solve(rhs, p, tol=tol, max_iter=400)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        # true residual of the returned pressure
+        Ap = torch.zeros_like(rhs)
+        L0 = s.levels[0]
+        pf.apply_A(p, L0.bx, L0.by, L0.bz, L0.diag, Ap)
+        b0 = rhs - rhs.double().mean().float()
+        true_rel = float((b0 - Ap).abs().max() / b0.abs().max())
+        out[prec] = {"ms": 1e3 * min(times), "iters": it, "rel": rel, "true_rel": true_rel}
+        del s
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--res", type=int, default=128)
+    ap.add_argument("--ppc", type=int, default=27)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-solve-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
+                                   "of it (the path would shard by slab decomposition, see "
+                                   "DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    pf = importlib.import_module("implementation-of-adaptive-phase-field-flip_amd")
+
+    n = args.res
+    cfg = pf.Config(res=(n, n, n), dx=1.0 / n, ppc=args.ppc, gravity=(0.0, 0.0, -9.81),
+                    droplets=False, pcg_tol=1e-6, pcg_max_iter=400,
+                    solver_precision=args.precision)
+    t_seed = time.perf_counter()
+    pos, vel, pt = pf.scenes.dam_break(cfg, frac=(0.4, 1.0, 0.6), jitter=0.5)
+    print(f"[bench] seeded {len(pos)} particles in {time.perf_counter() - t_seed:.1f} s",
+          file=sys.stderr, flush=True)
+    sim = pf.Simulator(cfg)
+    sim.set_particles(pos, vel, pt)
+    n_p, n_c = len(pos), n ** 3
+    del pos, vel, pt
+    for w in range(args.warmup):
+        rec = sim.step()
+        print(f"[bench] warmup {w}: {rec}", file=sys.stderr, flush=True)
+
+    sim.stage_ms = {}
+    recs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        recs.append(sim.step())
+        print(f"[bench] step {k}: pcg_iter={recs[-1]['pcg_iter']} dt={recs[-1]['dt']:.4g} "
+              f"wall={recs[-1]['wall_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stages = {k: v / args.steps for k, v in sim.stage_ms.items()}
+    sim.stage_ms = None
+
+    p2g_ms = stages["p2g"]
+    achieved = p2g_bytes(n_p, n_c) / (p2g_ms * 1e-3) / 1e9
+    result = {
+        "metric": "PF-FLIP step particle-steps/s at 128^3/27ppc; MG-PCG ms to 1e-6, mixed vs FP32",
+        "value": n_p * args.steps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered lattice)",
+        "config": {"workload": f"dam break z-up {n}^3 x {args.ppc} ppc, 1000:1, MG-PCG tol 1e-6",
+                   "grid": n, "ppc": args.ppc, "particles": n_p, "pcg_tol": 1e-6,
+                   "solver_precision": args.precision, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "p2g (k_p2g + output zeroing)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": p2g_bytes(n_p, n_c), "avg_ms": p2g_ms},
+        "pcg_iters": [r["pcg_iter"] for r in recs],
+        "stages_ms": stages,
+    }
+    if not args.skip_solve_compare:
+        result["mgpcg_1e-6"] = solve_compare(pf, torch, sim, cfg, 1e-6)
+    if not args.skip_cpu:
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,22 @@
+"""MI355X-native PF-FLIP time step (drop-in for the reference ``pfflip`` package).
+
+Exports ``Config``, ``Simulator``, ``MGPCG``, ``apply_A`` and ``scenes`` with
+the reference's names and semantics (pfflip/__init__.py:7-10,
+pfflip/poisson.py:78,240).  Compute runs in hand-written gfx950 HIP kernels
+(``csrc/``, built into ``libpfflip_hip.so`` and bound with ctypes); there is
+no CPU fallback.
+"""
+
+from . import scenes
+from .config import Config
+from .poisson import MGPCG, apply_A
+from .sim import Simulator
+
+
+def build(verbose: bool = False) -> str:
+    """Compile the HIP library for gfx950 (hipcc; no GPU needed)."""
+    from .build import build as _build
+    return _build(verbose=verbose)
+
+
+__all__ = ["Config", "Simulator", "MGPCG", "apply_A", "scenes", "build"]

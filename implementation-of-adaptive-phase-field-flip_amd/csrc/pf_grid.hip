@@ -1,0 +1,457 @@
+// Grid-side kernels: fused eq. (7) phase field / beta / walls / gravity / RHS,
+// pressure projection + divergence, liquid-velocity extrapolation.
+#include "pf_common.hpp"
+
+namespace pf {
+
+int fill_u32(void* dst, int64_t n, uint32_t v, hipStream_t s);
+
+// ============================================================== phase / beta / forces
+struct PhaseK {
+    Dims d;
+    int flags;
+    const float* mass[3];
+    const float* mom[3];
+    float* vel[3];
+    float* vel_star[3];
+    int* valid[3];
+    const float* cmass;
+    const float* cwsum;
+    float* phi_cell;
+    int* is_liq;
+    float* phi_face[3];
+    float* beta[3];
+    const int* solid;
+    float* rhs;
+    float mass_eps, rho_min, alpha_phi, rho_0l, rho_l, rho_g;
+    float g[3];
+    float gdt[3];  // g_a * dt, formed once so every pass adds the same value
+    float dt, dx, rho_t0, k_div;
+};
+
+// eq. (7), kernels.py:234-239
+__device__ inline float phase_field(float rho, const PhaseK& a) {
+    if (rho < a.rho_min) return 0.0f;
+    return fminf(sqrtf(fmaxf(rho - a.rho_min, 0.0f) / (a.alpha_phi * a.rho_0l)), 1.0f);
+}
+
+__device__ inline int cell_tag(const PhaseK& a, int c) {
+    return phase_field(a.cmass[c], a) >= 0.5f ? 1 : 0;
+}
+
+// Coordinates of the low / high cell of face (i,j,k) of axis A.
+template <int A>
+__device__ inline void face_cells(const Dims& d, int i, int j, int k, bool& lo_out, bool& hi_out,
+                                  int& clo, int& chi) {
+    const int ia = A == 0 ? i : (A == 1 ? j : k);
+    const int na = A == 0 ? d.nx : (A == 1 ? d.ny : d.nz);
+    lo_out = ia == 0;
+    hi_out = ia == na;
+    clo = lo_out ? -1 : cell_index(d, i - (A == 0), j - (A == 1), k - (A == 2));
+    chi = hi_out ? -1 : cell_index(d, i, j, k);
+}
+
+// walls (kernels.py:314-326) and solid-touching faces (kernels.py:601-627)
+template <int A>
+__device__ inline bool face_blocked(const PhaseK& a, bool lo_out, bool hi_out, int clo, int chi) {
+    if (lo_out || hi_out) return true;
+    if (a.solid && (a.solid[clo] == 1 || a.solid[chi] == 1)) return true;
+    return false;
+}
+
+// Face velocity after every velocity stage selected by flags, without
+// writing anything (used for the high faces a cell's RHS needs).
+template <int A>
+__device__ inline float final_face_vel(const PhaseK& a, int i, int j, int k) {
+    const int f = face_index<A>(a.d, i, j, k);
+    float v;
+    if (a.flags & PF_PH_VEL_FROM_MOM) {
+        const float m = a.mass[A][f];
+        v = m > a.mass_eps ? a.mom[A][f] / m : 0.0f;
+    } else {
+        v = a.vel[A][f];
+    }
+    if (a.flags & PF_PH_ZERO) {
+        bool lo, hi;
+        int clo, chi;
+        face_cells<A>(a.d, i, j, k, lo, hi, clo, chi);
+        if (face_blocked<A>(a, lo, hi, clo, chi)) return 0.0f;
+    }
+    if ((a.flags & PF_PH_GRAVITY) && a.g[A] != 0.0f) v += a.gdt[A];
+    return v;
+}
+
+// Process face (i,j,k) of axis A: all selected stages, with writes.
+// Returns the final velocity (for the owning cell's RHS).
+template <int A>
+__device__ inline float process_face(const PhaseK& a, int i, int j, int k) {
+    const int f = face_index<A>(a.d, i, j, k);
+    const int fl = a.flags;
+    bool lo, hi;
+    int clo, chi;
+    face_cells<A>(a.d, i, j, k, lo, hi, clo, chi);
+    const bool blocked = (fl & PF_PH_ZERO) ? face_blocked<A>(a, lo, hi, clo, chi) : false;
+
+    float v;
+    if (fl & PF_PH_VEL_FROM_MOM) {
+        const float m = a.mass[A][f];
+        const bool ok = m > a.mass_eps;
+        v = ok ? a.mom[A][f] / m : 0.0f;
+        if (a.valid[A]) a.valid[A][f] = ok ? 1 : 0;
+    } else {
+        v = a.vel[A][f];
+    }
+    bool write_beta = false;
+    float beta = 0.0f;
+    if (fl & PF_PH_PHASE) {
+        // face_beta, kernels.py:278-311
+        int ta = lo ? 0 : cell_tag(a, clo);
+        int tb = hi ? 0 : cell_tag(a, chi);
+        if (lo) ta = tb;
+        if (hi) tb = ta;
+        float ph;
+        if (ta == 1 && tb == 1)
+            ph = 1.0f;
+        else if (ta == 0 && tb == 0)
+            ph = 0.0f;
+        else
+            ph = phase_field(a.mass[A][f], a);
+        a.phi_face[A][f] = ph;
+        const float rho = (1.0f - ph) * a.rho_g + ph * a.rho_l;
+        beta = 1.0f / rho;
+        write_beta = true;
+    }
+    if (blocked) {
+        v = 0.0f;
+        beta = 0.0f;
+        write_beta = true;
+    }
+    if (fl & PF_PH_STAR) a.vel_star[A][f] = v;
+    if (fl & PF_PH_GRAVITY) {
+        if (a.g[A] != 0.0f) v += a.gdt[A];
+        if (blocked) v = 0.0f;
+    }
+    if (fl & (PF_PH_VEL_FROM_MOM | PF_PH_ZERO | PF_PH_GRAVITY)) a.vel[A][f] = v;
+    if (write_beta) a.beta[A][f] = beta;
+    return v;
+}
+
+// One thread per point of the (nx+1)(ny+1)(nz+1) lattice: cell (i,j,k) and
+// its three low faces; the last layer of each axis only has faces.
+__global__ __launch_bounds__(256) void k_phase(PhaseK a) {
+    const Dims d = a.d;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ez = d.nz + 1, ey = d.ny + 1;
+    if (t >= (int64_t)(d.nx + 1) * ey * ez) return;
+    const int k = (int)(t % ez), j = (int)((t / ez) % ey), i = (int)(t / ((int64_t)ez * ey));
+    const bool ci = i < d.nx, cj = j < d.ny, ck = k < d.nz;
+    float ulo = 0.f, vlo = 0.f, wlo = 0.f;
+    if (cj && ck) ulo = process_face<0>(a, i, j, k);
+    if (ci && ck) vlo = process_face<1>(a, i, j, k);
+    if (ci && cj) wlo = process_face<2>(a, i, j, k);
+    if (!(ci && cj && ck)) return;
+    const int c = cell_index(d, i, j, k);
+    if (a.flags & PF_PH_PHASE) {
+        // cell_phase, kernels.py:242-255
+        const float ph = phase_field(a.cmass[c], a);
+        a.phi_cell[c] = ph;
+        a.is_liq[c] = ph >= 0.5f ? 1 : 0;
+    }
+    if (a.flags & PF_PH_RHS) {
+        // build_rhs, kernels.py:338-363
+        const float uhi = final_face_vel<0>(a, i + 1, j, k);
+        const float vhi = final_face_vel<1>(a, i, j + 1, k);
+        const float whi = final_face_vel<2>(a, i, j, k + 1);
+        const float du = (uhi - ulo) + (vhi - vlo) + (whi - wlo);
+        float r = -(a.dx / a.dt) * du;
+        if (a.k_div > 0.0f) {
+            const float excess = (a.cwsum[c] - a.rho_t0) / a.rho_t0;
+            r -= a.dx * a.dx * a.k_div * fmaxf(excess, 0.0f) / a.dt;
+        }
+        a.rhs[c] = r;
+    }
+}
+
+// ============================================================== projection
+struct ProjK {
+    Dims d;
+    const float* p;
+    const float* beta[3];
+    float* vel[3];
+    const float* vel_star[3];
+    float* dvel[3];
+    float dx, dt;
+};
+
+template <int A>
+__device__ inline void project_face(const ProjK& a, int i, int j, int k) {
+    const Dims d = a.d;
+    const int f = face_index<A>(d, i, j, k);
+    float v = a.vel[A][f];
+    const int ia = A == 0 ? i : (A == 1 ? j : k);
+    const int na = A == 0 ? d.nx : (A == 1 ? d.ny : d.nz);
+    if (a.dt != 0.0f) {
+        const float b = a.beta[A][f];
+        // kernels.py:376-387: skip beta == 0 and the two boundary planes
+        if (b != 0.0f && ia > 0 && ia < na) {
+            const float ph = a.p[cell_index(d, i, j, k)];
+            const float pl = a.p[cell_index(d, i - (A == 0), j - (A == 1), k - (A == 2))];
+            v -= a.dt * b * (ph - pl) / a.dx;
+            a.vel[A][f] = v;
+        }
+    }
+    if (a.dvel[A]) a.dvel[A][f] = v - a.vel_star[A][f];
+}
+
+__global__ __launch_bounds__(256) void k_project(ProjK a) {
+    const Dims d = a.d;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ez = d.nz + 1, ey = d.ny + 1;
+    if (t >= (int64_t)(d.nx + 1) * ey * ez) return;
+    const int k = (int)(t % ez), j = (int)((t / ez) % ey), i = (int)(t / ((int64_t)ez * ey));
+    const bool ci = i < d.nx, cj = j < d.ny, ck = k < d.nz;
+    if (cj && ck) project_face<0>(a, i, j, k);
+    if (ci && ck) project_face<1>(a, i, j, k);
+    if (ci && cj) project_face<2>(a, i, j, k);
+}
+
+// max |div u / dx| (kernels.py:390-400); out must start at 0
+__global__ __launch_bounds__(256) void k_max_div(Dims d, const float* __restrict__ u,
+                                                 const float* __restrict__ v,
+                                                 const float* __restrict__ w, float dx, float* out) {
+    __shared__ float sh[4];
+    float m = 0.0f;
+    const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(c % d.nz), j = (int)((c / d.nz) % d.ny), i = (int)(c / ((int64_t)d.nz * d.ny));
+        const float dv = (u[face_index<0>(d, i + 1, j, k)] - u[face_index<0>(d, i, j, k)]) +
+                         (v[face_index<1>(d, i, j + 1, k)] - v[face_index<1>(d, i, j, k)]) +
+                         (w[face_index<2>(d, i, j, k + 1)] - w[face_index<2>(d, i, j, k)]);
+        m = fmaxf(m, fabsf(dv / dx));
+    }
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) atomic_max_nonneg(out, m);
+}
+
+// ============================================================== extrapolation
+struct ExtrapK {
+    Dims d;
+    const float* vin[3];
+    const uint8_t* kin[3];
+    float* vout[3];
+    uint8_t* kout[3];
+    const int* is_liq;
+};
+
+// decode a flat index over the three face arrays
+__device__ inline bool decode_face(const Dims& d, int64_t t, int& A, int& i, int& j, int& k, int& f) {
+    const int64_t f0 = face_count(d, 0), f1 = face_count(d, 1), f2 = face_count(d, 2);
+    if (t < f0) {
+        A = 0;
+    } else if (t < f0 + f1) {
+        A = 1;
+        t -= f0;
+    } else if (t < f0 + f1 + f2) {
+        A = 2;
+        t -= f0 + f1;
+    } else {
+        return false;
+    }
+    const int sy = d.ny + (A == 1), sz = d.nz + (A == 2);
+    f = (int)t;
+    k = (int)(t % sz);
+    j = (int)((t / sz) % sy);
+    i = (int)(t / ((int64_t)sz * sy));
+    return true;
+}
+
+// seed_liquid_velocity, kernels.py:406-425
+__global__ __launch_bounds__(256) void k_extrap_seed(ExtrapK a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int A, i, j, k, f;
+    if (!decode_face(a.d, t, A, i, j, k, f)) return;
+    const Dims d = a.d;
+    int liq = 0;
+    if (i < d.nx && j < d.ny && k < d.nz) liq = max(liq, a.is_liq[cell_index(d, i, j, k)]);
+    const int im = i - (A == 0), jm = j - (A == 1), km = k - (A == 2);
+    if (im >= 0 && jm >= 0 && km >= 0) liq = max(liq, a.is_liq[cell_index(d, im, jm, km)]);
+    a.kout[A][f] = (uint8_t)liq;
+    a.vout[A][f] = liq == 1 ? a.vin[A][f] : 0.0f;
+}
+
+// extrapolate_sweep, kernels.py:428-462
+__global__ __launch_bounds__(256) void k_extrap_sweep(ExtrapK a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int A, i, j, k, f;
+    if (!decode_face(a.d, t, A, i, j, k, f)) return;
+    const float* vin = a.vin[A];
+    const uint8_t* kin = a.kin[A];
+    if (kin[f] == 1) {
+        a.vout[A][f] = vin[f];
+        a.kout[A][f] = 1;
+        return;
+    }
+    const Dims d = a.d;
+    const int sx = d.nx + (A == 0), sy = d.ny + (A == 1), sz = d.nz + (A == 2);
+    float acc = 0.0f;
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int ax = q / 2, s = (q % 2 == 0) ? -1 : 1;
+        const int ii = i + (ax == 0 ? s : 0), jj = j + (ax == 1 ? s : 0), kk = k + (ax == 2 ? s : 0);
+        if (ii < 0 || jj < 0 || kk < 0 || ii >= sx || jj >= sy || kk >= sz) continue;
+        const int g = (ii * sy + jj) * sz + kk;
+        if (kin[g] == 1) {
+            acc += vin[g];
+            cnt += 1;
+        }
+    }
+    if (cnt > 0) {
+        a.vout[A][f] = acc / (float)cnt;
+        a.kout[A][f] = 1;
+    } else {
+        a.vout[A][f] = vin[f];
+        a.kout[A][f] = 0;
+    }
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+PF_API int pf_phase_beta_forces(const pf_phase_args* args, pf_stream_t stream) {
+    PF_REQUIRE(args, "pf_phase_beta_forces: null args");
+    const pf_phase_args& in = *args;
+    PF_REQUIRE(in.nx > 0 && in.ny > 0 && in.nz > 0, "pf_phase_beta_forces: bad grid");
+    PF_REQUIRE((int64_t)(in.nx + 1) * (in.ny + 1) * (in.nz + 1) < (int64_t)1 << 31,
+               "pf_phase_beta_forces: grid too large");
+    const int fl = in.flags;
+    PF_REQUIRE(fl != 0 && (fl & ~63) == 0, "pf_phase_beta_forces: bad flags %d", fl);
+    for (int q = 0; q < 3; ++q) {
+        if (fl & PF_PH_VEL_FROM_MOM)
+            PF_REQUIRE(in.mass[q] && in.mom[q], "pf_phase_beta_forces: need mass/mom");
+        PF_REQUIRE(in.vel[q], "pf_phase_beta_forces: need vel");
+        if (fl & PF_PH_PHASE)
+            PF_REQUIRE(in.mass[q] && in.phi_face[q] && in.beta[q], "pf_phase_beta_forces: phase needs mass, phi_face, beta");
+        if (fl & (PF_PH_ZERO | PF_PH_GRAVITY)) PF_REQUIRE(in.beta[q], "pf_phase_beta_forces: need beta");
+        if (fl & PF_PH_STAR) PF_REQUIRE(in.vel_star[q], "pf_phase_beta_forces: need vel_star");
+    }
+    if (fl & PF_PH_PHASE)
+        PF_REQUIRE(in.cell_mass && in.phi_cell && in.is_liquid, "pf_phase_beta_forces: phase needs cell arrays");
+    if (fl & PF_PH_RHS) {
+        PF_REQUIRE(in.rhs, "pf_phase_beta_forces: need rhs");
+        PF_REQUIRE(in.dt != 0.0f, "pf_phase_beta_forces: rhs needs dt != 0");
+        PF_REQUIRE(!(in.k_div > 0.0f) || in.cell_wsum, "pf_phase_beta_forces: k_div needs cell_wsum");
+    }
+    PhaseK a;
+    a.d = Dims{in.nx, in.ny, in.nz};
+    a.flags = fl;
+    for (int q = 0; q < 3; ++q) {
+        a.mass[q] = in.mass[q];
+        a.mom[q] = in.mom[q];
+        a.vel[q] = in.vel[q];
+        a.vel_star[q] = in.vel_star[q];
+        a.valid[q] = in.valid[q];
+        a.phi_face[q] = in.phi_face[q];
+        a.beta[q] = in.beta[q];
+        a.g[q] = in.gravity[q];
+        a.gdt[q] = in.gravity[q] * in.dt;
+    }
+    a.cmass = in.cell_mass;
+    a.cwsum = in.cell_wsum;
+    a.phi_cell = in.phi_cell;
+    a.is_liq = in.is_liquid;
+    a.solid = in.solid;
+    a.rhs = in.rhs;
+    a.mass_eps = in.mass_eps;
+    a.rho_min = in.rho_min;
+    a.alpha_phi = in.alpha_phi;
+    a.rho_0l = in.rho_0l;
+    a.rho_l = in.rho_l;
+    a.rho_g = in.rho_g;
+    a.dt = in.dt;
+    a.dx = in.dx;
+    a.rho_t0 = in.rho_tilde_0;
+    a.k_div = in.k_div;
+    const int64_t n = (int64_t)(in.nx + 1) * (in.ny + 1) * (in.nz + 1);
+    k_phase<<<cdiv(n, 256), 256, 0, S(stream)>>>(a);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_project(const float* p, float* const beta[3], float* const vel[3],
+                      float* const vel_star[3], float* const dvel[3], int nx, int ny, int nz,
+                      float dx, float dt, float* max_div, pf_stream_t stream) {
+    PF_REQUIRE(nx > 0 && ny > 0 && nz > 0, "pf_project: bad grid");
+    PF_REQUIRE(vel && vel[0] && vel[1] && vel[2], "pf_project: need vel");
+    PF_REQUIRE(dt == 0.0f || (p && beta && beta[0] && beta[1] && beta[2]), "pf_project: need p, beta");
+    PF_REQUIRE(dx > 0.f, "pf_project: dx must be positive");
+    hipStream_t s = S(stream);
+    const Dims d{nx, ny, nz};
+    ProjK a;
+    a.d = d;
+    a.p = p;
+    bool want_dvel = dvel && vel_star;
+    for (int q = 0; q < 3; ++q) {
+        a.beta[q] = beta ? beta[q] : nullptr;
+        a.vel[q] = vel[q];
+        a.vel_star[q] = want_dvel ? vel_star[q] : nullptr;
+        a.dvel[q] = want_dvel ? dvel[q] : nullptr;
+        if (want_dvel) PF_REQUIRE(vel_star[q] && dvel[q], "pf_project: partial dvel/vel_star");
+    }
+    a.dx = dx;
+    a.dt = dt;
+    const int64_t n = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+    if (dt != 0.0f || want_dvel) {
+        k_project<<<cdiv(n, 256), 256, 0, s>>>(a);
+        PF_CHECK_LAUNCH();
+    }
+    if (max_div) {
+        int rc = fill_u32(max_div, 1, 0u, s);
+        if (rc) return rc;
+        const int64_t nb = cdiv((int64_t)nx * ny * nz, 256 * 4);
+        k_max_div<<<(unsigned)(nb < 4096 ? nb : 4096), 256, 0, s>>>(d, vel[0], vel[1], vel[2], dx,
+                                                                     max_div);
+        PF_CHECK_LAUNCH();
+    }
+    return PF_OK;
+}
+
+PF_API int pf_extrapolate(float* const vel[3], const int32_t* is_liquid, float* const vel_liq[3],
+                          float* const tmp[3], uint8_t* const known[3], uint8_t* const known2[3],
+                          int nx, int ny, int nz, int sweeps, pf_stream_t stream) {
+    PF_REQUIRE(nx > 0 && ny > 0 && nz > 0 && sweeps >= 0, "pf_extrapolate: bad sizes");
+    PF_REQUIRE(vel && is_liquid && vel_liq && tmp && known && known2, "pf_extrapolate: null args");
+    hipStream_t s = S(stream);
+    const Dims d{nx, ny, nz};
+    const int64_t n = face_count(d, 0) + face_count(d, 1) + face_count(d, 2);
+    // ping-pong so that the last sweep lands in vel_liq
+    float* const* bufs[2] = {vel_liq, tmp};
+    uint8_t* const* kb[2] = {known, known2};
+    int cur = (sweeps % 2 == 0) ? 0 : 1;
+    ExtrapK a;
+    a.d = d;
+    a.is_liq = is_liquid;
+    for (int q = 0; q < 3; ++q) {
+        a.vin[q] = vel[q];
+        a.kin[q] = nullptr;
+        a.vout[q] = bufs[cur][q];
+        a.kout[q] = kb[cur][q];
+    }
+    k_extrap_seed<<<cdiv(n, 256), 256, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    for (int sw = 0; sw < sweeps; ++sw) {
+        const int nxt = 1 - cur;
+        for (int q = 0; q < 3; ++q) {
+            a.vin[q] = bufs[cur][q];
+            a.kin[q] = kb[cur][q];
+            a.vout[q] = bufs[nxt][q];
+            a.kout[q] = kb[nxt][q];
+        }
+        k_extrap_sweep<<<cdiv(n, 256), 256, 0, s>>>(a);
+        PF_CHECK_LAUNCH();
+        cur = nxt;
+    }
+    return PF_OK;
+}

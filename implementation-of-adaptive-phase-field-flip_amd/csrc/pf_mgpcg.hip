@@ -1,0 +1,1002 @@
+// Multigrid-preconditioned flexible CG for A p = b with
+// (A p)_i = sum_f beta_f (p_i - p_nbr) on the MAC grid.
+//
+// Hierarchy: unsmoothed-aggregation Galerkin (coarse face = sum of 4 fine
+// faces), red-black weighted Gauss-Seidel (omega = 6/7), V-cycle with
+// pre (0,1) / post (1,0) colour orders.  Fine levels run one kernel per
+// half-sweep out of HBM; every level from `small_from` down is solved inside
+// one workgroup with all its data in LDS.  The FPCG scalars live on the
+// device; dot products are deterministic FP64 block reductions, and the
+// host only polls a "done" flag one iteration behind the GPU.
+#include <atomic>
+
+#include "pf_common.hpp"
+
+namespace pf {
+
+int fill_u32(void* dst, int64_t n, uint32_t v, hipStream_t s);
+
+constexpr int MAXB = 1024;  // blocks of a reduction launch
+constexpr int RT = 256;     // threads of element-wise / reduction kernels
+
+struct CGState {
+    double rz, alpha, beta, mu, b_norm, rel, qAq, mu_x;
+    int it, done, pad0, pad1;
+    unsigned ticket[8];
+    double part[6 * MAXB];
+    float fpart[MAXB];
+};
+
+static unsigned red_blocks(int64_t n) {
+    const int64_t nb = cdiv(n, (int64_t)RT * 4);
+    return (unsigned)(nb < 1 ? 1 : (nb > MAXB ? MAXB : nb));
+}
+
+// ================================================================ stencil
+template <typename FT, typename PT>
+__device__ inline void stencil_s(const Dims& d, const FT* __restrict__ bx, const FT* __restrict__ by,
+                                 const FT* __restrict__ bz, const PT* p, int i, int j, int k,
+                                 float& diag, float& s, float& pc) {
+    const float bxl = to_f32(bx[face_index<0>(d, i, j, k)]);
+    const float bxh = to_f32(bx[face_index<0>(d, i + 1, j, k)]);
+    const float byl = to_f32(by[face_index<1>(d, i, j, k)]);
+    const float byh = to_f32(by[face_index<1>(d, i, j + 1, k)]);
+    const float bzl = to_f32(bz[face_index<2>(d, i, j, k)]);
+    const float bzh = to_f32(bz[face_index<2>(d, i, j, k + 1)]);
+    // compute_diag order (poisson.py:49-51)
+    diag = bxl + bxh + byl + byh + bzl + bzh;
+    pc = to_f32(p[cell_index(d, i, j, k)]);
+    // _apply_row order, neighbours clamped at the walls (poisson.py:69-75)
+    s = bxl * to_f32(p[cell_index(d, max(i - 1, 0), j, k)]);
+    s += bxh * to_f32(p[cell_index(d, min(i + 1, d.nx - 1), j, k)]);
+    s += byl * to_f32(p[cell_index(d, i, max(j - 1, 0), k)]);
+    s += byh * to_f32(p[cell_index(d, i, min(j + 1, d.ny - 1), k)]);
+    s += bzl * to_f32(p[cell_index(d, i, j, max(k - 1, 0))]);
+    s += bzh * to_f32(p[cell_index(d, i, j, min(k + 1, d.nz - 1))]);
+}
+
+// A p in the difference form sum_f beta_f (p_i - p_f).  It equals
+// diag*p_i - sum_f beta_f p_f (the reference's form, poisson.py:75), but in
+// FP32 the subtractions of nearby values are exact, so the product carries
+// no cancellation error: the reference's form limits the recursively
+// updated CG residual to ~3e-6 (FP32) / ~4e-5 (16-bit preconditioner) at a
+// 1000:1 coefficient jump, this form to ~1e-9 / ~5e-8.
+template <typename FT, typename PT>
+__device__ inline void stencil(const Dims& d, const FT* __restrict__ bx, const FT* __restrict__ by,
+                               const FT* __restrict__ bz, const PT* p, int i, int j, int k,
+                               float& diag, float& Ap, float& pc) {
+    const float bxl = to_f32(bx[face_index<0>(d, i, j, k)]);
+    const float bxh = to_f32(bx[face_index<0>(d, i + 1, j, k)]);
+    const float byl = to_f32(by[face_index<1>(d, i, j, k)]);
+    const float byh = to_f32(by[face_index<1>(d, i, j + 1, k)]);
+    const float bzl = to_f32(bz[face_index<2>(d, i, j, k)]);
+    const float bzh = to_f32(bz[face_index<2>(d, i, j, k + 1)]);
+    diag = bxl + bxh + byl + byh + bzl + bzh;
+    pc = to_f32(p[cell_index(d, i, j, k)]);
+    float s = bxl * (pc - to_f32(p[cell_index(d, max(i - 1, 0), j, k)]));
+    s += bxh * (pc - to_f32(p[cell_index(d, min(i + 1, d.nx - 1), j, k)]));
+    s += byl * (pc - to_f32(p[cell_index(d, i, max(j - 1, 0), k)]));
+    s += byh * (pc - to_f32(p[cell_index(d, i, min(j + 1, d.ny - 1), k)]));
+    s += bzl * (pc - to_f32(p[cell_index(d, i, j, max(k - 1, 0))]));
+    s += bzh * (pc - to_f32(p[cell_index(d, i, j, min(k + 1, d.nz - 1))]));
+    Ap = s;
+}
+
+// ================================================================ hierarchy build
+// Coarse face = sum of the 4 fine faces it covers (poisson.py:130-163).
+__global__ __launch_bounds__(RT) void k_galerkin(Dims df, Dims dc, const float* __restrict__ fx,
+                                                 const float* __restrict__ fy,
+                                                 const float* __restrict__ fz, float* __restrict__ cx,
+                                                 float* __restrict__ cy, float* __restrict__ cz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ez = dc.nz + 1, ey = dc.ny + 1;
+    if (t >= (int64_t)(dc.nx + 1) * ey * ez) return;
+    const int K = (int)(t % ez), J = (int)((t / ez) % ey), I = (int)(t / ((int64_t)ez * ey));
+    const int i0 = 2 * I, j0 = 2 * J, k0 = 2 * K;
+    if (J < dc.ny && K < dc.nz) {
+        float acc = 0.f;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const int jj = j0 + a, kk = k0 + b;
+                if (i0 <= df.nx && jj < df.ny && kk < df.nz) acc += fx[face_index<0>(df, i0, jj, kk)];
+            }
+        cx[face_index<0>(dc, I, J, K)] = acc;
+    }
+    if (I < dc.nx && K < dc.nz) {
+        float acc = 0.f;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const int ii = i0 + a, kk = k0 + b;
+                if (ii < df.nx && j0 <= df.ny && kk < df.nz) acc += fy[face_index<1>(df, ii, j0, kk)];
+            }
+        cy[face_index<1>(dc, I, J, K)] = acc;
+    }
+    if (I < dc.nx && J < dc.ny) {
+        float acc = 0.f;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const int ii = i0 + a, jj = j0 + b;
+                if (ii < df.nx && jj < df.ny && k0 <= df.nz) acc += fz[face_index<2>(df, ii, jj, k0)];
+            }
+        cz[face_index<2>(dc, I, J, K)] = acc;
+    }
+}
+
+// diag (poisson.py:40-51) and, for levels run out of HBM in reduced
+// precision, the 16-bit copies of the face coefficients.
+template <typename ST>
+__global__ __launch_bounds__(RT) void k_diag_copy(Dims d, const float* __restrict__ bx,
+                                                  const float* __restrict__ by,
+                                                  const float* __restrict__ bz, float* diag, ST* hx,
+                                                  ST* hy, ST* hz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ez = d.nz + 1, ey = d.ny + 1;
+    if (t >= (int64_t)(d.nx + 1) * ey * ez) return;
+    const int k = (int)(t % ez), j = (int)((t / ez) % ey), i = (int)(t / ((int64_t)ez * ey));
+    const bool ci = i < d.nx, cj = j < d.ny, ck = k < d.nz;
+    if (hx) {
+        if (cj && ck) hx[face_index<0>(d, i, j, k)] = from_f32<ST>(bx[face_index<0>(d, i, j, k)]);
+        if (ci && ck) hy[face_index<1>(d, i, j, k)] = from_f32<ST>(by[face_index<1>(d, i, j, k)]);
+        if (ci && cj) hz[face_index<2>(d, i, j, k)] = from_f32<ST>(bz[face_index<2>(d, i, j, k)]);
+    }
+    if (diag && ci && cj && ck)
+        diag[cell_index(d, i, j, k)] = bx[face_index<0>(d, i, j, k)] + bx[face_index<0>(d, i + 1, j, k)] +
+                                       by[face_index<1>(d, i, j, k)] + by[face_index<1>(d, i, j + 1, k)] +
+                                       bz[face_index<2>(d, i, j, k)] + bz[face_index<2>(d, i, j, k + 1)];
+}
+
+// ================================================================ HBM smoother / transfers
+// Red-black half-sweep (poisson.py:105-126).  One thread per cell of the
+// active colour.  ZINIT: p is known to be zero (a freshly cleared level),
+// so res = b and every cell of the other colour is written as 0.
+template <typename ST, bool ZINIT>
+__global__ __launch_bounds__(RT) void k_rbgs(Dims d, const ST* __restrict__ bx,
+                                             const ST* __restrict__ by, const ST* __restrict__ bz,
+                                             ST* p, const float* __restrict__ b, int colour,
+                                             float omega, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int nkp = (d.nz + 1) >> 1;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)d.nx * d.ny * nkp) return;
+    const int kp = (int)(t % nkp);
+    const int row = (int)(t / nkp);
+    const int j = row % d.ny, i = row / d.ny;
+    if (ZINIT) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int k = 2 * kp + q;
+            if (k >= d.nz) break;
+            float v = 0.0f;
+            if (((i + j + k) & 1) == colour) {
+                const float dg = to_f32(bx[face_index<0>(d, i, j, k)]) + to_f32(bx[face_index<0>(d, i + 1, j, k)]) +
+                                 to_f32(by[face_index<1>(d, i, j, k)]) + to_f32(by[face_index<1>(d, i, j + 1, k)]) +
+                                 to_f32(bz[face_index<2>(d, i, j, k)]) + to_f32(bz[face_index<2>(d, i, j, k + 1)]);
+                if (dg > 0.0f) v = omega * b[cell_index(d, i, j, k)] / dg;
+            }
+            p[cell_index(d, i, j, k)] = from_f32<ST>(v);
+        }
+    } else {
+        const int k = 2 * kp + ((i + j + colour) & 1);
+        if (k >= d.nz) return;
+        float dg, Ap, pc;
+        stencil(d, bx, by, bz, p, i, j, k, dg, Ap, pc);
+        if (dg <= 0.0f) return;
+        const int c = cell_index(d, i, j, k);
+        const float res = b[c] - Ap;
+        p[c] = from_f32<ST>(pc + omega * res / dg);
+    }
+}
+
+// residual (poisson.py:91-102) fused with R = P^T (poisson.py:166-175)
+template <typename ST>
+__global__ __launch_bounds__(RT) void k_resid_restrict(Dims d, Dims dc, const ST* __restrict__ bx,
+                                                       const ST* __restrict__ by,
+                                                       const ST* __restrict__ bz,
+                                                       const ST* __restrict__ p,
+                                                       const float* __restrict__ b,
+                                                       float* __restrict__ bc,
+                                                       const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)dc.nx * dc.ny * dc.nz) return;
+    const int K = (int)(t % dc.nz), J = (int)((t / dc.nz) % dc.ny), I = (int)(t / ((int64_t)dc.nz * dc.ny));
+    float acc = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int i = 2 * I + a, j = 2 * J + bb, k = 2 * K + c;
+                float dg, Ap, pc;
+                stencil(d, bx, by, bz, p, i, j, k, dg, Ap, pc);
+                acc += b[cell_index(d, i, j, k)] - Ap;
+            }
+    bc[t] = acc;
+}
+
+// P: piecewise-constant injection added onto the fine level (poisson.py:178-182)
+template <typename ST, typename CT>
+__global__ __launch_bounds__(RT) void k_prolong(Dims d, Dims dc, ST* p, const CT* __restrict__ pc,
+                                                const int* __restrict__ done) {
+    if (done && *done) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)d.nx * d.ny * d.nz) return;
+    const int k = (int)(t % d.nz), j = (int)((t / d.nz) % d.ny), i = (int)(t / ((int64_t)d.nz * d.ny));
+    const float v = to_f32(p[t]) + to_f32(pc[cell_index(dc, i / 2, j / 2, k / 2)]);
+    p[t] = from_f32<ST>(v);
+}
+
+// ================================================================ reductions
+// sum of a level vector (FP64), combined by the last block into *out_mean
+template <typename T>
+__global__ __launch_bounds__(RT) void k_mean(const T* __restrict__ x, int64_t n, CGState* st,
+                                             int slot, double* out_mean, int reset_solve) {
+    __shared__ double sh[16];
+    double v[1] = {0.0};
+    for (int64_t c = (int64_t)blockIdx.x * RT + threadIdx.x; c < n; c += (int64_t)gridDim.x * RT)
+        v[0] += (double)to_f32(x[c]);
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) st->part[blockIdx.x] = v[0];
+    if (take_last_ticket(&st->ticket[slot])) {
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (unsigned q = 0; q < gridDim.x; ++q) s += st->part[q];
+            *out_mean = s / (double)n;
+            if (reset_solve) {
+                st->done = 0;
+                st->it = 0;
+                st->rel = 1.0;
+            }
+        }
+    }
+}
+
+// y = x + shift (FP32 shift of the FP64 mean, as _remove_mean's shift_kernel)
+template <typename TI, typename TO>
+__global__ __launch_bounds__(RT) void k_shift(const TI* x, TO* y, int64_t n, const double* mean) {
+    const float c = (float)(-*mean);
+    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
+        y[i] = from_f32<TO>(to_f32(x[i]) + c);
+}
+
+template <typename T>
+__global__ __launch_bounds__(RT) void k_zero(T* x, int64_t n, const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT)
+        x[i] = from_f32<T>(0.0f);
+}
+
+// ================================================================ small levels in LDS
+struct SmallArgs {
+    int s, L, pre, post, coarsest;
+    float omega;
+    Dims d[PF_MAX_LEVELS];
+    const float* bx[PF_MAX_LEVELS];
+    const float* by[PF_MAX_LEVELS];
+    const float* bz[PF_MAX_LEVELS];
+    int off[PF_MAX_LEVELS][5];  // LDS float offsets of bx, by, bz, p, b
+    const float* b_top;
+    void* p_top;
+    const int* done;
+};
+
+struct LdsLevel {
+    Dims d;
+    float* bx;
+    float* by;
+    float* bz;
+    float* p;
+    float* b;
+};
+
+__device__ inline LdsLevel lds_level(const SmallArgs& a, float* lds, int l) {
+    LdsLevel v;
+    v.d = a.d[l];
+    v.bx = lds + a.off[l][0];
+    v.by = lds + a.off[l][1];
+    v.bz = lds + a.off[l][2];
+    v.p = lds + a.off[l][3];
+    v.b = lds + a.off[l][4];
+    return v;
+}
+
+__device__ inline void lds_half_sweep(const LdsLevel& v, int colour, float omega) {
+    const Dims& d = v.d;
+    const int nkp = (d.nz + 1) >> 1;
+    const int n = d.nx * d.ny * nkp;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int kp = t % nkp, row = t / nkp;
+        const int j = row % d.ny, i = row / d.ny;
+        const int k = 2 * kp + ((i + j + colour) & 1);
+        if (k >= d.nz) continue;
+        float dg, Ap, pc;
+        stencil(d, v.bx, v.by, v.bz, v.p, i, j, k, dg, Ap, pc);
+        if (dg <= 0.0f) continue;
+        const int c = cell_index(d, i, j, k);
+        v.p[c] = pc + omega * (v.b[c] - Ap) / dg;
+    }
+    __syncthreads();
+}
+
+__device__ inline void lds_smooth(const LdsLevel& v, int sweeps, bool reverse, float omega) {
+    for (int s = 0; s < sweeps; ++s) {
+        lds_half_sweep(v, reverse ? 1 : 0, omega);
+        lds_half_sweep(v, reverse ? 0 : 1, omega);
+    }
+}
+
+template <typename OT>
+__global__ __launch_bounds__(1024) void k_small_vcycle(SmallArgs a) {
+    extern __shared__ float lds[];
+    if (a.done && *a.done) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int l = a.s; l < a.L; ++l) {
+        const LdsLevel v = lds_level(a, lds, l);
+        const Dims d = v.d;
+        const int nfx = (d.nx + 1) * d.ny * d.nz, nfy = d.nx * (d.ny + 1) * d.nz, nfz = d.nx * d.ny * (d.nz + 1);
+        for (int e = tid; e < nfx; e += nt) v.bx[e] = a.bx[l][e];
+        for (int e = tid; e < nfy; e += nt) v.by[e] = a.by[l][e];
+        for (int e = tid; e < nfz; e += nt) v.bz[e] = a.bz[l][e];
+    }
+    {
+        const LdsLevel v = lds_level(a, lds, a.s);
+        const int n = v.d.nx * v.d.ny * v.d.nz;
+        for (int e = tid; e < n; e += nt) v.b[e] = a.b_top[e];
+    }
+    __syncthreads();
+    // descend
+    for (int l = a.s; l < a.L - 1; ++l) {
+        const LdsLevel v = lds_level(a, lds, l);
+        const LdsLevel c = lds_level(a, lds, l + 1);
+        const int n = v.d.nx * v.d.ny * v.d.nz;
+        for (int e = tid; e < n; e += nt) v.p[e] = 0.0f;
+        __syncthreads();
+        lds_smooth(v, a.pre, false, a.omega);
+        const int nc = c.d.nx * c.d.ny * c.d.nz;
+        for (int t = tid; t < nc; t += nt) {
+            const int K = t % c.d.nz, J = (t / c.d.nz) % c.d.ny, I = t / (c.d.nz * c.d.ny);
+            float acc = 0.0f;
+            for (int x = 0; x < 2; ++x)
+                for (int y = 0; y < 2; ++y)
+                    for (int z = 0; z < 2; ++z) {
+                        const int i = 2 * I + x, j = 2 * J + y, k = 2 * K + z;
+                        float dg, Ap, pc;
+                        stencil(v.d, v.bx, v.by, v.bz, v.p, i, j, k, dg, Ap, pc);
+                        acc += v.b[cell_index(v.d, i, j, k)] - Ap;
+                    }
+            c.b[t] = acc;
+        }
+        __syncthreads();
+    }
+    // coarsest: sweeps, then remove the mean (poisson.py:325-328)
+    {
+        const LdsLevel v = lds_level(a, lds, a.L - 1);
+        const int n = v.d.nx * v.d.ny * v.d.nz;
+        for (int e = tid; e < n; e += nt) v.p[e] = 0.0f;
+        __syncthreads();
+        lds_smooth(v, a.coarsest, false, a.omega);
+        __shared__ double sh[16];
+        double s[1] = {0.0};
+        for (int e = tid; e < n; e += nt) s[0] += (double)v.p[e];
+        block_sum<1>(s, sh);
+        __shared__ float shift;
+        if (tid == 0) shift = (float)(-(s[0] / (double)n));
+        __syncthreads();
+        for (int e = tid; e < n; e += nt) v.p[e] = v.p[e] + shift;
+        __syncthreads();
+    }
+    // ascend
+    for (int l = a.L - 2; l >= a.s; --l) {
+        const LdsLevel v = lds_level(a, lds, l);
+        const LdsLevel c = lds_level(a, lds, l + 1);
+        const int n = v.d.nx * v.d.ny * v.d.nz;
+        for (int t = tid; t < n; t += nt) {
+            const int k = t % v.d.nz, j = (t / v.d.nz) % v.d.ny, i = t / (v.d.nz * v.d.ny);
+            v.p[t] = v.p[t] + c.p[cell_index(c.d, i / 2, j / 2, k / 2)];
+        }
+        __syncthreads();
+        lds_smooth(v, a.post, true, a.omega);
+    }
+    {
+        const LdsLevel v = lds_level(a, lds, a.s);
+        const int n = v.d.nx * v.d.ny * v.d.nz;
+        OT* out = static_cast<OT*>(a.p_top);
+        for (int e = tid; e < n; e += nt) out[e] = from_f32<OT>(v.p[e]);
+    }
+}
+
+// ================================================================ FPCG kernels
+template <typename ZT>
+struct CGK {
+    Dims d;
+    int64_t n;
+    const float* bx;
+    const float* by;
+    const float* bz;
+    const float* b;
+    float* rhs;
+    float* x;
+    float* r;
+    float* q;
+    float* Ap;
+    const ZT* z;
+    CGState* st;
+    double tol;
+};
+
+// rhs = b - mean(b); r = rhs; x = 0; b_norm = max|rhs| (poisson.py:363-372)
+template <typename ZT>
+__global__ __launch_bounds__(RT) void k_cg_init(CGK<ZT> a) {
+    __shared__ float sh[4];
+    const float c = (float)(-a.st->mu);
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
+        const float v = a.b[i] + c;
+        a.rhs[i] = v;
+        a.r[i] = v;
+        a.x[i] = 0.0f;
+        m = fmaxf(m, fabsf(v));
+    }
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) a.st->fpart[blockIdx.x] = m;
+    if (take_last_ticket(&a.st->ticket[1])) {
+        if (threadIdx.x == 0) {
+            float mm = 0.0f;
+            for (unsigned q = 0; q < gridDim.x; ++q) mm = fmaxf(mm, a.st->fpart[q]);
+            a.st->b_norm = (double)mm;
+            if (mm == 0.0f) {
+                a.st->done = 2;
+                a.st->rel = 0.0;
+            }
+        }
+    }
+}
+
+// after the first V-cycle: mu = mean(z), rz = <r, z - mu>
+template <typename ZT>
+__global__ __launch_bounds__(RT) void k_cg_first(CGK<ZT> a) {
+    __shared__ double sh[3 * 16];
+    if (a.st->done) return;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
+        const double zz = (double)to_f32(a.z[i]), rr = (double)a.r[i];
+        v[0] += zz;
+        v[1] += rr;
+        v[2] += rr * zz;
+    }
+    block_sum<3>(v, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) a.st->part[q * MAXB + blockIdx.x] = v[q];
+    if (take_last_ticket(&a.st->ticket[2])) {
+        if (threadIdx.x == 0) {
+            double s[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < 3; ++q)
+                for (unsigned bb = 0; bb < gridDim.x; ++bb) s[q] += a.st->part[q * MAXB + bb];
+            const double mu = s[0] / (double)a.n;
+            a.st->mu = mu;
+            a.st->rz = s[2] - mu * s[1];
+        }
+    }
+}
+
+// q = (z - mu) + beta q  (first call: beta = 0)
+template <typename ZT>
+__global__ __launch_bounds__(RT) void k_cg_q(CGK<ZT> a, int first) {
+    if (a.st->done) return;
+    const float c = (float)(-a.st->mu);
+    const float be = first ? 0.0f : (float)a.st->beta;
+    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
+        const float zz = to_f32(a.z[i]) + c;
+        a.q[i] = first ? zz : zz + be * a.q[i];
+    }
+}
+
+// Ap = A q; qAq; alpha = rz / qAq (poisson.py:385-389)
+template <typename ZT>
+__global__ __launch_bounds__(RT) void k_cg_apply(CGK<ZT> a) {
+    __shared__ double sh[16];
+    if (a.st->done) return;
+    double v[1] = {0.0};
+    const Dims d = a.d;
+    for (int64_t c = (int64_t)blockIdx.x * RT + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * RT) {
+        const int k = (int)(c % d.nz), j = (int)((c / d.nz) % d.ny), i = (int)(c / ((int64_t)d.nz * d.ny));
+        float dg, Ap, pc;
+        stencil(d, a.bx, a.by, a.bz, a.q, i, j, k, dg, Ap, pc);
+        a.Ap[c] = Ap;
+        v[0] += (double)pc * (double)Ap;
+    }
+    block_sum<1>(v, sh);
+    if (threadIdx.x == 0) a.st->part[blockIdx.x] = v[0];
+    if (take_last_ticket(&a.st->ticket[3])) {
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (unsigned bb = 0; bb < gridDim.x; ++bb) s += a.st->part[bb];
+            a.st->qAq = s;
+            a.st->it += 1;
+            if (s <= 0.0)
+                a.st->done = 1;
+            else
+                a.st->alpha = a.st->rz / s;
+        }
+    }
+}
+
+// x += alpha q; r -= alpha Ap; rel = max|r| / b_norm (poisson.py:390-398)
+template <typename ZT>
+__global__ __launch_bounds__(RT) void k_cg_update(CGK<ZT> a) {
+    __shared__ float sh[4];
+    if (a.st->done) return;
+    const float al = (float)a.st->alpha, nal = (float)(-a.st->alpha);
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
+        a.x[i] = a.x[i] + al * a.q[i];
+        const float rn = a.r[i] + nal * a.Ap[i];
+        a.r[i] = rn;
+        m = fmaxf(m, fabsf(rn));
+    }
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) a.st->fpart[blockIdx.x] = m;
+    if (take_last_ticket(&a.st->ticket[4])) {
+        if (threadIdx.x == 0) {
+            float mm = 0.0f;
+            for (unsigned bb = 0; bb < gridDim.x; ++bb) mm = fmaxf(mm, a.st->fpart[bb]);
+            const double rel = (double)mm / a.st->b_norm;
+            a.st->rel = rel;
+            if (rel < a.tol) a.st->done = 1;
+        }
+    }
+}
+
+// Polak-Ribiere with the flexible numerator (poisson.py:400-406):
+// z' = z - mean(z); rz_new = <r, z'>; num = rz_new - <r_prev, z'>
+// = -alpha <Ap, z'> because r_prev = r + alpha Ap.
+template <typename ZT>
+__global__ __launch_bounds__(RT) void k_cg_beta(CGK<ZT> a) {
+    __shared__ double sh[5 * 16];
+    if (a.st->done) return;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
+        const double zz = (double)to_f32(a.z[i]), rr = (double)a.r[i], ap = (double)a.Ap[i];
+        v[0] += zz;
+        v[1] += rr;
+        v[2] += ap;
+        v[3] += rr * zz;
+        v[4] += ap * zz;
+    }
+    block_sum<5>(v, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 5; ++q) a.st->part[q * MAXB + blockIdx.x] = v[q];
+    if (take_last_ticket(&a.st->ticket[5])) {
+        if (threadIdx.x == 0) {
+            double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int q = 0; q < 5; ++q)
+                for (unsigned bb = 0; bb < gridDim.x; ++bb) s[q] += a.st->part[q * MAXB + bb];
+            const double mu = s[0] / (double)a.n;
+            const double rz_new = s[3] - mu * s[1];
+            const double apz = s[4] - mu * s[2];
+            const double num = -a.st->alpha * apz;
+            const double rz = a.st->rz;
+            a.st->beta = (rz != 0.0) ? fmax(num / rz, 0.0) : 0.0;
+            a.st->mu = mu;
+            a.st->rz = rz_new;
+        }
+    }
+}
+
+__global__ __launch_bounds__(RT) void k_apply_A(Dims d, const float* __restrict__ p,
+                                                const float* __restrict__ bx,
+                                                const float* __restrict__ by,
+                                                const float* __restrict__ bz,
+                                                const float* __restrict__ diag, float* out) {
+    const int64_t c = (int64_t)blockIdx.x * RT + threadIdx.x;
+    if (c >= (int64_t)d.nx * d.ny * d.nz) return;
+    const int k = (int)(c % d.nz), j = (int)((c / d.nz) % d.ny), i = (int)(c / ((int64_t)d.nz * d.ny));
+    float dg, sn, pc;
+    stencil_s(d, bx, by, bz, p, i, j, k, dg, sn, pc);
+    out[c] = (diag ? diag[c] : dg) * pc - sn;
+}
+
+// ================================================================ host side
+static Dims ldims(const pf_mg_level& l) { return Dims{l.nx, l.ny, l.nz}; }
+static int64_t ncell(const pf_mg_level& l) { return (int64_t)l.nx * l.ny * l.nz; }
+
+static int64_t small_lds_floats(const pf_mg& mg, int s, int (*off)[5]) {
+    int64_t o = 0;
+    for (int l = s; l < mg.nlev; ++l) {
+        const Dims d = ldims(mg.lev[l]);
+        const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
+        if (off) {
+            off[l][0] = (int)o;
+            o += face_count(d, 0);
+            off[l][1] = (int)o;
+            o += face_count(d, 1);
+            off[l][2] = (int)o;
+            o += face_count(d, 2);
+            off[l][3] = (int)o;
+            o += nc;
+            off[l][4] = (int)o;
+            o += nc;
+        } else {
+            o += face_count(d, 0) + face_count(d, 1) + face_count(d, 2) + 2 * nc;
+        }
+    }
+    return o;
+}
+
+constexpr int64_t kSmallMaxCells = 8192;
+constexpr int64_t kSmallMaxBytes = 150 * 1024;
+
+template <typename ST>
+struct Lv {
+    static const ST* fx(const pf_mg_level& l) { return reinterpret_cast<const ST*>(l.bx_h); }
+    static const ST* fy(const pf_mg_level& l) { return reinterpret_cast<const ST*>(l.by_h); }
+    static const ST* fz(const pf_mg_level& l) { return reinterpret_cast<const ST*>(l.bz_h); }
+    static ST* p(const pf_mg_level& l) { return reinterpret_cast<ST*>(l.p_h); }
+};
+template <>
+struct Lv<float> {
+    static const float* fx(const pf_mg_level& l) { return l.bx; }
+    static const float* fy(const pf_mg_level& l) { return l.by; }
+    static const float* fz(const pf_mg_level& l) { return l.bz; }
+    static float* p(const pf_mg_level& l) { return l.p; }
+};
+
+// The single-workgroup kernel needs more than the default 64 KB of dynamic
+// LDS; raise the limit once per device (gfx950 has 160 KB per workgroup).
+static int ensure_small_lds() {
+    static std::atomic<uint64_t> done_mask{0};
+    int dev = 0;
+    PF_CHECK_HIP(hipGetDevice(&dev));
+    const uint64_t bit = 1ull << (dev & 63);
+    if (done_mask.load() & bit) return PF_OK;
+    const int bytes = (int)kSmallMaxBytes;
+    PF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small_vcycle<float>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    PF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small_vcycle<bf16s>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    PF_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_small_vcycle<fp16s>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    done_mask.fetch_or(bit);
+    return PF_OK;
+}
+
+// One V-cycle: z0 = M^{-1} r0 (no mean removal at level 0).
+template <typename ST>
+static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hipStream_t s) {
+    const int L = mg.nlev;
+    const int sm = mg.small_from;             // first LDS level (L if none)
+    const int G = sm < L ? sm : L - 1;        // levels 0..G-1 run out of HBM with restriction
+    auto P = [&](int l) -> ST* { return l == 0 ? z0 : Lv<ST>::p(mg.lev[l]); };
+    auto B = [&](int l) -> const float* { return l == 0 ? r0 : mg.lev[l].b; };
+
+    auto smooth = [&](int l, int sweeps, bool reverse, bool from_zero) -> int {
+        const pf_mg_level& lv = mg.lev[l];
+        const Dims d = ldims(lv);
+        const int64_t nthr = (int64_t)d.nx * d.ny * ((d.nz + 1) / 2);
+        const unsigned grid = cdiv(nthr, RT);
+        const ST *fx = Lv<ST>::fx(lv), *fy = Lv<ST>::fy(lv), *fz = Lv<ST>::fz(lv);
+        ST* p = P(l);
+        const float* b = B(l);
+        if (from_zero && sweeps == 0) {
+            k_zero<ST><<<red_blocks(ncell(lv)), RT, 0, s>>>(p, ncell(lv), done);
+            PF_CHECK_LAUNCH();
+        }
+        for (int sw = 0; sw < sweeps; ++sw) {
+            for (int h = 0; h < 2; ++h) {
+                const int colour = reverse ? 1 - h : h;
+                if (from_zero && sw == 0 && h == 0)
+                    k_rbgs<ST, true><<<grid, RT, 0, s>>>(d, fx, fy, fz, p, b, colour, mg.omega, done);
+                else
+                    k_rbgs<ST, false><<<grid, RT, 0, s>>>(d, fx, fy, fz, p, b, colour, mg.omega, done);
+                PF_CHECK_LAUNCH();
+            }
+        }
+        return PF_OK;
+    };
+
+    for (int l = 0; l < G; ++l) {
+        int rc = smooth(l, mg.pre, false, true);
+        if (rc) return rc;
+        const pf_mg_level& lv = mg.lev[l];
+        const pf_mg_level& cv = mg.lev[l + 1];
+        k_resid_restrict<ST><<<cdiv(ncell(cv), RT), RT, 0, s>>>(ldims(lv), ldims(cv), Lv<ST>::fx(lv),
+                                                              Lv<ST>::fy(lv), Lv<ST>::fz(lv), P(l), B(l),
+                                                              mg.lev[l + 1].b, done);
+        PF_CHECK_LAUNCH();
+    }
+    if (sm < L) {
+        int rc0 = ensure_small_lds();
+        if (rc0) return rc0;
+        SmallArgs a;
+        a.s = sm;
+        a.L = L;
+        a.pre = mg.pre;
+        a.post = mg.post;
+        a.coarsest = mg.coarsest;
+        a.omega = mg.omega;
+        for (int l = sm; l < L; ++l) {
+            a.d[l] = ldims(mg.lev[l]);
+            a.bx[l] = mg.lev[l].bx;
+            a.by[l] = mg.lev[l].by;
+            a.bz[l] = mg.lev[l].bz;
+        }
+        const int64_t nf = small_lds_floats(mg, sm, a.off);
+        a.b_top = B(sm);
+        a.done = done;
+        const size_t lds = (size_t)nf * sizeof(float);
+        if (sm == 0) {
+            a.p_top = z0;
+            k_small_vcycle<ST><<<1, 1024, lds, s>>>(a);
+        } else {
+            a.p_top = mg.lev[sm].p;
+            k_small_vcycle<float><<<1, 1024, lds, s>>>(a);
+        }
+        PF_CHECK_LAUNCH();
+    } else {
+        // coarsest level out of HBM: sweeps, then mean removal
+        const int l = L - 1;
+        int rc = smooth(l, mg.coarsest, false, true);
+        if (rc) return rc;
+        CGState* st = static_cast<CGState*>(mg.scalars);
+        const int64_t n = ncell(mg.lev[l]);
+        // p is ST; the mean is taken in FP64 and the shifted value stored back
+        k_mean<ST><<<red_blocks(n), RT, 0, s>>>(P(l), n, st, 6, &st->mu_x, 0);
+        PF_CHECK_LAUNCH();
+        k_shift<ST, ST><<<red_blocks(n), RT, 0, s>>>(P(l), P(l), n, &st->mu_x);
+        PF_CHECK_LAUNCH();
+    }
+    for (int l = G - 1; l >= 0; --l) {
+        const pf_mg_level& lv = mg.lev[l];
+        const pf_mg_level& cv = mg.lev[l + 1];
+        if (l + 1 == sm)
+            k_prolong<ST, float><<<cdiv(ncell(lv), RT), RT, 0, s>>>(ldims(lv), ldims(cv), P(l), cv.p, done);
+        else
+            k_prolong<ST, ST><<<cdiv(ncell(lv), RT), RT, 0, s>>>(ldims(lv), ldims(cv), P(l), P(l + 1), done);
+        PF_CHECK_LAUNCH();
+        int rc = smooth(l, mg.post, true, false);
+        if (rc) return rc;
+    }
+    return PF_OK;
+}
+
+struct HostPoll {
+    int* flag = nullptr;  // pinned ring of 2 ints
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int device = -1;
+};
+
+static int get_poll(HostPoll*& out) {
+    static thread_local HostPoll hp;
+    int dev = 0;
+    PF_CHECK_HIP(hipGetDevice(&dev));
+    if (hp.flag == nullptr || hp.device != dev) {
+        PF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&hp.flag), 2 * sizeof(int), hipHostMallocDefault));
+        for (int q = 0; q < 2; ++q) PF_CHECK_HIP(hipEventCreateWithFlags(&hp.ev[q], hipEventDisableTiming));
+        hp.device = dev;
+    }
+    out = &hp;
+    return PF_OK;
+}
+
+template <typename ST>
+static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int max_iter,
+                      int* it_out, double* rel_out, hipStream_t s) {
+    CGState* st = static_cast<CGState*>(mg.scalars);
+    const pf_mg_level& l0 = mg.lev[0];
+    const Dims d = ldims(l0);
+    const int64_t n = ncell(l0);
+    ST* z = (sizeof(ST) == 4) ? reinterpret_cast<ST*>(mg.z) : Lv<ST>::p(l0);
+    CGK<ST> a;
+    a.d = d;
+    a.n = n;
+    a.bx = l0.bx;
+    a.by = l0.by;
+    a.bz = l0.bz;
+    a.b = b;
+    a.rhs = mg.rhs;
+    a.x = x;
+    a.r = mg.res;
+    a.q = mg.q;
+    a.Ap = mg.Ap;
+    a.z = z;
+    a.st = st;
+    a.tol = tol;
+    const unsigned nb = red_blocks(n);
+    const int* done = &st->done;
+
+    k_mean<float><<<nb, RT, 0, s>>>(b, n, st, 0, &st->mu, 1);
+    PF_CHECK_LAUNCH();
+    k_cg_init<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    int init_done = 0;
+    PF_CHECK_HIP(hipMemcpyAsync(&init_done, done, sizeof(int), hipMemcpyDeviceToHost, s));
+    PF_CHECK_HIP(hipStreamSynchronize(s));
+    if (init_done == 2) {  // b == 0 after mean removal: x = 0 (poisson.py:366-368)
+        *it_out = 0;
+        *rel_out = 0.0;
+        return PF_OK;
+    }
+    int rc = vcycle<ST>(mg, a.r, z, done, s);
+    if (rc) return rc;
+    k_cg_first<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    k_cg_q<ST><<<nb, RT, 0, s>>>(a, 1);
+    PF_CHECK_LAUNCH();
+
+    HostPoll* hp = nullptr;
+    rc = get_poll(hp);
+    if (rc) return rc;
+    for (int it = 1; it <= max_iter; ++it) {
+        k_cg_apply<ST><<<nb, RT, 0, s>>>(a);
+        PF_CHECK_LAUNCH();
+        k_cg_update<ST><<<nb, RT, 0, s>>>(a);
+        PF_CHECK_LAUNCH();
+        const int slot = it & 1;
+        PF_CHECK_HIP(hipMemcpyAsync(&hp->flag[slot], done, sizeof(int), hipMemcpyDeviceToHost, s));
+        PF_CHECK_HIP(hipEventRecord(hp->ev[slot], s));
+        if (it < max_iter) {
+            rc = vcycle<ST>(mg, a.r, z, done, s);
+            if (rc) return rc;
+            k_cg_beta<ST><<<nb, RT, 0, s>>>(a);
+            PF_CHECK_LAUNCH();
+            k_cg_q<ST><<<nb, RT, 0, s>>>(a, 0);
+            PF_CHECK_LAUNCH();
+        }
+        // poll one iteration behind, so the GPU always has queued work
+        if (it >= 2) {
+            PF_CHECK_HIP(hipEventSynchronize(hp->ev[slot ^ 1]));
+            if (hp->flag[slot ^ 1] != 0) break;
+        }
+    }
+    // _remove_mean(x) (poisson.py:408)
+    k_mean<float><<<nb, RT, 0, s>>>(x, n, st, 7, &st->mu_x, 0);
+    PF_CHECK_LAUNCH();
+    k_shift<float, float><<<nb, RT, 0, s>>>(x, x, n, &st->mu_x);
+    PF_CHECK_LAUNCH();
+    struct {
+        int it;
+        double rel;
+    } res;
+    PF_CHECK_HIP(hipMemcpyAsync(&res.it, &st->it, sizeof(int), hipMemcpyDeviceToHost, s));
+    PF_CHECK_HIP(hipMemcpyAsync(&res.rel, &st->rel, sizeof(double), hipMemcpyDeviceToHost, s));
+    PF_CHECK_HIP(hipStreamSynchronize(s));
+    *it_out = res.it;
+    *rel_out = res.rel;
+    return PF_OK;
+}
+
+template <typename ST>
+static int precondition_impl(const pf_mg& mg, const float* r, float* z, hipStream_t s) {
+    CGState* st = static_cast<CGState*>(mg.scalars);
+    const int64_t n = ncell(mg.lev[0]);
+    ST* zt = (sizeof(ST) == 4) ? reinterpret_cast<ST*>(z) : Lv<ST>::p(mg.lev[0]);
+    int rc = vcycle<ST>(mg, r, zt, nullptr, s);
+    if (rc) return rc;
+    k_mean<ST><<<red_blocks(n), RT, 0, s>>>(zt, n, st, 6, &st->mu_x, 0);
+    PF_CHECK_LAUNCH();
+    k_shift<ST, float><<<red_blocks(n), RT, 0, s>>>(zt, z, n, &st->mu_x);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+static int check_mg(const pf_mg* mg) {
+    PF_REQUIRE(mg, "null pf_mg");
+    PF_REQUIRE(mg->nlev >= 1 && mg->nlev <= PF_MAX_LEVELS, "pf_mg: bad level count %d", mg->nlev);
+    PF_REQUIRE(mg->precision >= 0 && mg->precision <= 2, "pf_mg: bad precision %d", mg->precision);
+    PF_REQUIRE(mg->pre >= 0 && mg->post >= 0 && mg->coarsest >= 0, "pf_mg: negative sweep count");
+    PF_REQUIRE(mg->scalars, "pf_mg: null scalar workspace");
+    PF_REQUIRE(mg->small_from >= 0 && mg->small_from <= mg->nlev, "pf_mg: call pf_mg_setup first");
+    for (int l = 0; l < mg->nlev; ++l) {
+        const pf_mg_level& v = mg->lev[l];
+        PF_REQUIRE(v.nx > 0 && v.ny > 0 && v.nz > 0, "pf_mg: level %d has bad dims", l);
+        PF_REQUIRE(v.bx && v.by && v.bz && v.b, "pf_mg: level %d missing arrays", l);
+        if (l > 0) {
+            const pf_mg_level& f = mg->lev[l - 1];
+            PF_REQUIRE(f.nx == 2 * v.nx && f.ny == 2 * v.ny && f.nz == 2 * v.nz,
+                       "pf_mg: level %d is not the 2x coarsening of level %d", l, l - 1);
+        }
+        if (mg->precision != 0 && l < mg->small_from)
+            PF_REQUIRE(v.bx_h && v.by_h && v.bz_h && v.p_h, "pf_mg: level %d missing 16-bit arrays", l);
+        // level 0's 16-bit iterate holds the preconditioned vector z
+        if (mg->precision != 0 && l == 0) PF_REQUIRE(v.p_h, "pf_mg: level 0 missing its 16-bit iterate");
+        if (l > 0 && (mg->precision == 0 || l >= mg->small_from)) PF_REQUIRE(v.p, "pf_mg: level %d missing p", l);
+    }
+    return PF_OK;
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+PF_API size_t pf_mg_scalar_bytes(void) { return sizeof(CGState); }
+
+PF_API int pf_mg_setup(pf_mg* mg) {
+    PF_REQUIRE(mg, "pf_mg_setup: null pf_mg");
+    PF_REQUIRE(mg->nlev >= 1 && mg->nlev <= PF_MAX_LEVELS, "pf_mg_setup: bad level count");
+    int sm = mg->nlev;
+    for (int l = mg->nlev - 1; l >= 0; --l) {
+        const int64_t nc = (int64_t)mg->lev[l].nx * mg->lev[l].ny * mg->lev[l].nz;
+        const int64_t bytes = small_lds_floats(*mg, l, nullptr) * (int64_t)sizeof(float);
+        if (nc <= kSmallMaxCells && bytes <= kSmallMaxBytes)
+            sm = l;
+        else
+            break;
+    }
+    mg->small_from = sm;
+    return PF_OK;
+}
+
+PF_API int pf_mg_build(const pf_mg* mg, const float* bx, const float* by, const float* bz,
+                       pf_stream_t stream) {
+    int rc = check_mg(mg);
+    if (rc) return rc;
+    PF_REQUIRE(bx && by && bz, "pf_mg_build: null coefficients");
+    hipStream_t s = S(stream);
+    const pf_mg_level& l0 = mg->lev[0];
+    const Dims d0 = ldims(l0);
+    const float* src[3] = {bx, by, bz};
+    float* dst[3] = {l0.bx, l0.by, l0.bz};
+    for (int q = 0; q < 3; ++q)
+        if (src[q] != dst[q])
+            PF_CHECK_HIP(hipMemcpyAsync(dst[q], src[q], face_count(d0, q) * sizeof(float),
+                                        hipMemcpyDeviceToDevice, s));
+    for (int l = 0; l < mg->nlev; ++l) {
+        const pf_mg_level& v = mg->lev[l];
+        const Dims d = ldims(v);
+        if (l > 0) {
+            const pf_mg_level& f = mg->lev[l - 1];
+            const int64_t n = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+            k_galerkin<<<cdiv(n, RT), RT, 0, s>>>(ldims(f), d, f.bx, f.by, f.bz, v.bx, v.by, v.bz);
+            PF_CHECK_LAUNCH();
+        }
+        const int64_t n = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+        const bool copy = mg->precision != 0 && l < mg->small_from;
+        if (mg->precision == 2)
+            k_diag_copy<fp16s><<<cdiv(n, RT), RT, 0, s>>>(
+                d, v.bx, v.by, v.bz, v.diag, copy ? reinterpret_cast<fp16s*>(v.bx_h) : nullptr,
+                reinterpret_cast<fp16s*>(v.by_h), reinterpret_cast<fp16s*>(v.bz_h));
+        else
+            k_diag_copy<bf16s><<<cdiv(n, RT), RT, 0, s>>>(
+                d, v.bx, v.by, v.bz, v.diag, copy ? reinterpret_cast<bf16s*>(v.bx_h) : nullptr,
+                reinterpret_cast<bf16s*>(v.by_h), reinterpret_cast<bf16s*>(v.bz_h));
+        PF_CHECK_LAUNCH();
+    }
+    return PF_OK;
+}
+
+PF_API int pf_mg_precondition(const pf_mg* mg, const float* r, float* z, pf_stream_t stream) {
+    int rc = check_mg(mg);
+    if (rc) return rc;
+    PF_REQUIRE(r && z, "pf_mg_precondition: null vector");
+    switch (mg->precision) {
+        case 0: return precondition_impl<float>(*mg, r, z, S(stream));
+        case 1: return precondition_impl<bf16s>(*mg, r, z, S(stream));
+        default: return precondition_impl<fp16s>(*mg, r, z, S(stream));
+    }
+}
+
+PF_API int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol, int32_t max_iter,
+                          int32_t* it_out, double* rel_out, pf_stream_t stream) {
+    int rc = check_mg(mg);
+    if (rc) return rc;
+    PF_REQUIRE(b && x && it_out && rel_out, "pf_mgpcg_solve: null argument");
+    PF_REQUIRE(mg->rhs && mg->res && mg->q && mg->Ap, "pf_mgpcg_solve: missing CG vectors");
+    PF_REQUIRE(mg->precision != 0 || mg->z, "pf_mgpcg_solve: missing z");
+    PF_REQUIRE(max_iter >= 0, "pf_mgpcg_solve: negative max_iter");
+    switch (mg->precision) {
+        case 0: return solve_impl<float>(*mg, b, x, tol, max_iter, it_out, rel_out, S(stream));
+        case 1: return solve_impl<bf16s>(*mg, b, x, tol, max_iter, it_out, rel_out, S(stream));
+        default: return solve_impl<fp16s>(*mg, b, x, tol, max_iter, it_out, rel_out, S(stream));
+    }
+}
+
+PF_API int pf_apply_A(int nx, int ny, int nz, const float* p, const float* bx, const float* by,
+                      const float* bz, const float* diag, float* out, pf_stream_t stream) {
+    PF_REQUIRE(nx > 0 && ny > 0 && nz > 0, "pf_apply_A: bad grid");
+    PF_REQUIRE(p && bx && by && bz && out, "pf_apply_A: null argument");
+    const int64_t n = (int64_t)nx * ny * nz;
+    k_apply_A<<<cdiv(n, RT), RT, 0, S(stream)>>>(Dims{nx, ny, nz}, p, bx, by, bz, diag, out);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}

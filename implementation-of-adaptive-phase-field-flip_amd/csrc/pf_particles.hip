@@ -1,0 +1,884 @@
+// Particle-side kernels: binning + stable counting sort, tiled P2G with LDS
+// accumulators, fused G2P + RK3 advection + wall reflection (+ solids), and
+// the CFL speed reduction.
+#include "pf_common.hpp"
+
+namespace pf {
+
+int fill_u32(void* dst, int64_t n, uint32_t v, hipStream_t s);
+
+constexpr int T = PF_TILE;
+constexpr int T3 = T * T * T;
+
+struct TileGeom {
+    int nx, ny, nz, ntx, nty, ntz;
+};
+
+static TileGeom tile_geom(int nx, int ny, int nz) {
+    TileGeom g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.ntx = (nx + T - 1) / T;
+    g.nty = (ny + T - 1) / T;
+    g.ntz = (nz + T - 1) / T;
+    return g;
+}
+
+static int64_t n_tiles(const TileGeom& g) { return (int64_t)g.ntx * g.nty * g.ntz; }
+
+// ============================================================== binning
+__global__ __launch_bounds__(256) void k_bin(const float* __restrict__ px,
+                                             const int* __restrict__ ptype,
+                                             const int* __restrict__ pstate, int64_t n,
+                                             TileGeom g, float dx, uint32_t* __restrict__ keys,
+                                             uint32_t* __restrict__ rank, int* __restrict__ count,
+                                             uint8_t* __restrict__ pflag, int* n_active) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int active = 0;
+    if (p < n) {
+        const int st = pstate[p];
+        const int ty = ptype[p];
+        pflag[p] = (uint8_t)((ty == 1 ? 1 : 0) | (st == 0 ? 2 : 0) | (st == 2 ? 4 : 0));
+        if (st != 0) {
+            keys[p] = 0xFFFFFFFFu;
+        } else {
+            active = 1;
+            const float x = px[3 * p], y = px[3 * p + 1], z = px[3 * p + 2];
+            int i = (int)floorf(x / dx), j = (int)floorf(y / dx), k = (int)floorf(z / dx);
+            i = min(max(i, 0), g.nx - 1);
+            j = min(max(j, 0), g.ny - 1);
+            k = min(max(k, 0), g.nz - 1);
+            const int tile = ((i / T) * g.nty + (j / T)) * g.ntz + (k / T);
+            const int loc = ((i % T) * T + (j % T)) * T + (k % T);
+            const uint32_t key = (uint32_t)tile * T3 + (uint32_t)loc;
+            keys[p] = key;
+            rank[p] = (uint32_t)atomicAdd(&count[key], 1);
+        }
+    }
+    if (n_active) {
+        // one integer atomic per wave
+        const unsigned long long b = __ballot(active);
+        if ((threadIdx.x & 63) == 0 && b) atomicAdd(n_active, __popcll(b));
+    }
+}
+
+// ============================================================== exclusive scan
+constexpr int SCAN_THREADS = 256, SCAN_ITEMS = 16, SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int* __restrict__ in, int64_t n,
+                                                              int* __restrict__ block_sums) {
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        const int64_t i = base + q * SCAN_THREADS + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ int sh[SCAN_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < SCAN_THREADS / 64; ++w) t += sh[w];
+        block_sums[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of block sums in one workgroup; offsets[nb] = total.
+__global__ __launch_bounds__(1024) void k_scan_blocks(const int* __restrict__ block_sums, int nb,
+                                                      int* __restrict__ offsets) {
+    __shared__ int sh[1024];
+    int carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += 1024) {
+        const int i = c0 + threadIdx.x;
+        const int v = i < nb ? block_sums[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) offsets[i] = carry + sh[threadIdx.x] - v;
+        const int total = sh[1023];
+        __syncthreads();
+        carry += total;
+    }
+    if (threadIdx.x == 0) offsets[nb] = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int* __restrict__ in, int64_t n,
+                                                             const int* __restrict__ offsets, int nb,
+                                                             int* __restrict__ out) {
+    __shared__ int tile[SCAN_TILE];
+    __shared__ int wsum[SCAN_THREADS / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        const int e = q * SCAN_THREADS + threadIdx.x;
+        tile[e] = (base + e < n) ? in[base + e] : 0;
+    }
+    __syncthreads();
+    int loc[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        loc[q] = s;
+        s += tile[threadIdx.x * SCAN_ITEMS + q];
+    }
+    // inclusive wave scan of per-thread totals
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int inc = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < wid; ++w) wbase += wsum[w];
+    const int excl = offsets[blockIdx.x] + wbase + inc - s;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) tile[threadIdx.x * SCAN_ITEMS + q] = excl + loc[q];
+    __syncthreads();
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        const int e = q * SCAN_THREADS + threadIdx.x;
+        if (base + e < n) out[base + e] = tile[e];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = offsets[nb];
+}
+
+static int exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStream_t s) {
+    const int nb = (int)cdiv(n, SCAN_TILE);
+    int* block_sums = tmp;
+    int* offsets = tmp + nb;
+    k_scan_reduce<<<nb, SCAN_THREADS, 0, s>>>(in, n, block_sums);
+    PF_CHECK_LAUNCH();
+    k_scan_blocks<<<1, 1024, 0, s>>>(block_sums, nb, offsets);
+    PF_CHECK_LAUNCH();
+    k_scan_apply<<<nb, SCAN_THREADS, 0, s>>>(in, n, offsets, nb, out);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+// ============================================================== scatter + per-key order
+__global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ keys,
+                                                 const uint32_t* __restrict__ rank, int64_t n,
+                                                 const int* __restrict__ start,
+                                                 uint32_t* __restrict__ perm) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t key = keys[p];
+    if (key == 0xFFFFFFFFu) return;
+    perm[start[key] + rank[p]] = (uint32_t)p;
+}
+
+// Slots inside one key were handed out by atomics; sorting each key's slice
+// by particle index makes the permutation unique (a stable sort by key).
+// One wave per slab of T*T keys (a tile's x-slab, contiguous in `perm`):
+// the slab is staged in LDS, each lane insertion-sorts one key's slice there,
+// and the slab is written back.  Slabs larger than the LDS buffer
+// (pathological clustering) are sorted in place in global memory.  A key
+// holding more than kMaxSortLen particles keeps its slot order, so the cost
+// stays bounded; P2G does not depend on the order within a key.
+constexpr int SLAB_KEYS = T * T;
+constexpr int SLAB_CAP = 4096;
+constexpr int kMaxSortLen = 1024;
+
+__device__ inline void insertion_sort(uint32_t* v, int n) {
+    if (n > kMaxSortLen) return;
+    for (int a = 1; a < n; ++a) {
+        const uint32_t x = v[a];
+        int b = a - 1;
+        while (b >= 0 && v[b] > x) {
+            v[b + 1] = v[b];
+            --b;
+        }
+        v[b + 1] = x;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_key_sort(const int* __restrict__ start,
+                                                 uint32_t* __restrict__ perm) {
+    __shared__ uint32_t buf[SLAB_CAP];
+    const int key0 = blockIdx.x * SLAB_KEYS;
+    const int s0 = start[key0], e0 = start[key0 + SLAB_KEYS];
+    const int n = e0 - s0;
+    const int lane = threadIdx.x;
+    const int ks = start[key0 + lane] - s0, ke = start[key0 + lane + 1] - s0;
+    if (n <= SLAB_CAP) {
+        for (int e = lane; e < n; e += 64) buf[e] = perm[s0 + e];
+        __syncthreads();
+        insertion_sort(buf + ks, ke - ks);
+        __syncthreads();
+        for (int e = lane; e < n; e += 64) perm[s0 + e] = buf[e];
+    } else {
+        insertion_sort(perm + s0 + ks, ke - ks);
+    }
+}
+
+// ============================================================== P2G
+// Tile kernel: one workgroup per T^3-cell tile accumulates its particles'
+// eq. (6) contributions into LDS over the tile plus a one-cell halo, then
+// stores that region to a staging buffer with plain coalesced stores.  A
+// gather kernel sums, for every face and cell, the <= 8 tile regions that
+// overlap it, in a fixed tile order: no global atomics and no pre-zeroing of
+// the outputs, and the cross-tile sums are reproducible.
+struct P2GArgs {
+    const float* px;
+    const float* pv;
+    const float* pmass;
+    const uint32_t* perm;
+    const int* start;
+    TileGeom g;
+    float dx, inv_r2;
+    float* stage;
+};
+
+constexpr int E = T + 2;  // tangential / cell extent of a tile region (1-cell halo)
+constexpr int F = T + 1;  // normal extent of a face region
+constexpr int NXR = F * E * E, NYR = E * F * E, NZR = E * E * F, NCR = E * E * E;
+// LDS / staging layout of one tile region
+constexpr int O_XM = 0, O_XU = NXR, O_YM = 2 * NXR, O_YU = 2 * NXR + NYR;
+constexpr int O_ZM = 2 * NXR + 2 * NYR, O_ZU = O_ZM + NZR, O_CM = O_ZU + NZR, O_CW = O_CM + NCR;
+constexpr int REG = O_CM + NCR;        // floats per tile without the weight sum
+constexpr int REG_W = O_CW + NCR;      // with it
+
+__device__ inline float cube_w(float d2, float inv_r2) {
+    float t = fmaxf(1.0f - d2 * inv_r2, 0.0f);
+    return t * t * t;
+}
+
+// Per-axis data of one particle: squared distances to the two normal
+// (o = 0) and the two tangential (o = 1/2) candidate planes, and the
+// region-local index of the first candidate.  A candidate outside the tile
+// region (only possible for particles outside the domain) gets a squared
+// distance that makes its weight zero.
+struct AxisCand {
+    float n0, n1, t0, t1;
+    int ln, lt;
+};
+
+__device__ inline AxisCand axis_cand(float x, float dx, int origin) {
+    AxisCand c;
+    const float gx = x / dx;
+    const int in = (int)floorf(gx);
+    const int it = (int)floorf(gx - 0.5f);
+    const float fn0 = (float)in * dx - x;
+    const float fn1 = (float)(in + 1) * dx - x;
+    const float ft0 = ((float)it + 0.5f) * dx - x;
+    const float ft1 = ((float)(it + 1) + 0.5f) * dx - x;
+    c.ln = in - origin;
+    c.lt = it - (origin - 1);
+    const float big = 1e30f;
+    c.n0 = ((unsigned)c.ln < (unsigned)F) ? fn0 * fn0 : big;
+    c.n1 = ((unsigned)(c.ln + 1) < (unsigned)F) ? fn1 * fn1 : big;
+    c.t0 = ((unsigned)c.lt < (unsigned)E) ? ft0 * ft0 : big;
+    c.t1 = ((unsigned)(c.lt + 1) < (unsigned)E) ? ft1 * ft1 : big;
+    return c;
+}
+
+// Accumulate one particle into the 2x2x2 candidates of one staggered grid.
+// q*[2] are per-axis squared distances; base is the LDS offset of candidate
+// (0,0,0); SX, SY are the LDS strides of the first / second index.
+template <int SX, int SY, bool MOM>
+__device__ inline void splat8(const float (&qx)[2], const float (&qy)[2], const float (&qz)[2],
+                              int base, float m, float u, float inv_r2, float* sm, float* su) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const float pxy = qx[a] + qy[b];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const float w = cube_w(pxy + qz[c], inv_r2);
+                if (w > 0.0f) {
+                    const int addr = base + a * SX + b * SY + c;
+                    const float wm = w * m;
+                    atomicAdd(&sm[addr], wm);
+                    if (MOM) atomicAdd(&su[addr], wm * u);
+                }
+            }
+        }
+}
+
+struct PData {
+    float x, y, z, vx, vy, vz, m;
+};
+
+__device__ inline PData load_particle(const P2GArgs& a, uint32_t p) {
+    PData d;
+    const int64_t b = 3 * (int64_t)p;
+    d.x = a.px[b];
+    d.y = a.px[b + 1];
+    d.z = a.px[b + 2];
+    d.vx = a.pv[b];
+    d.vy = a.pv[b + 1];
+    d.vz = a.pv[b + 2];
+    d.m = a.pmass[p];
+    return d;
+}
+
+template <bool WSUM>
+__device__ inline void splat_particle(const PData& d, float dx, float inv_r2, int X0, int Y0, int Z0,
+                                      float* s) {
+    const AxisCand cx = axis_cand(d.x, dx, X0), cy = axis_cand(d.y, dx, Y0), cz = axis_cand(d.z, dx, Z0);
+    const float nx_[2] = {cx.n0, cx.n1}, tx_[2] = {cx.t0, cx.t1};
+    const float ny_[2] = {cy.n0, cy.n1}, ty_[2] = {cy.t0, cy.t1};
+    const float nz_[2] = {cz.n0, cz.n1}, tz_[2] = {cz.t0, cz.t1};
+    // x-faces: region [F][E][E], origin (X0, Y0-1, Z0-1)
+    splat8<E * E, E, true>(nx_, ty_, tz_, (cx.ln * E + cy.lt) * E + cz.lt, d.m, d.vx, inv_r2,
+                           s + O_XM, s + O_XU);
+    // y-faces: region [E][F][E], origin (X0-1, Y0, Z0-1)
+    splat8<F * E, E, true>(tx_, ny_, tz_, (cx.lt * F + cy.ln) * E + cz.lt, d.m, d.vy, inv_r2,
+                           s + O_YM, s + O_YU);
+    // z-faces: region [E][E][F], origin (X0-1, Y0-1, Z0)
+    splat8<E * F, F, true>(tx_, ty_, nz_, (cx.lt * E + cy.lt) * F + cz.ln, d.m, d.vz, inv_r2,
+                           s + O_ZM, s + O_ZU);
+    // cells: region [E][E][E], origin (X0-1, Y0-1, Z0-1)
+    if (WSUM) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const float pxy = tx_[i] + ty_[j];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const float w = cube_w(pxy + tz_[k], inv_r2);
+                    if (w > 0.0f) {
+                        const int addr = ((cx.lt + i) * E + (cy.lt + j)) * E + cz.lt + k;
+                        atomicAdd(&s[O_CM + addr], w * d.m);
+                        atomicAdd(&s[O_CW + addr], w);
+                    }
+                }
+            }
+    } else {
+        splat8<E * E, E, false>(tx_, ty_, tz_, (cx.lt * E + cy.lt) * E + cz.lt, d.m, 0.f, inv_r2,
+                                s + O_CM, nullptr);
+    }
+}
+
+template <bool WSUM>
+__global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
+    constexpr int R = WSUM ? REG_W : REG;
+    __shared__ float s[R];
+
+    const TileGeom g = a.g;
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const int tz = tile % g.ntz, ty = (tile / g.ntz) % g.nty, tx = tile / (g.ntz * g.nty);
+    const int X0 = tx * T, Y0 = ty * T, Z0 = tz * T;
+
+    for (int e = threadIdx.x; e < R; e += blockDim.x) s[e] = 0.f;
+    __syncthreads();
+
+    const float dx = a.dx, inv_r2 = a.inv_r2;
+    const int key0 = tile * T3;
+    // lanes of a wave take different cells of one x-slab, so the LDS atomics
+    // of one instruction mostly hit distinct addresses
+    for (int lc = threadIdx.x; lc < T3; lc += blockDim.x) {
+        const int s0 = a.start[key0 + lc], e0 = a.start[key0 + lc + 1];
+        if (s0 >= e0) continue;
+        // software pipeline: the next particle's data and the index after it
+        // are in flight while the current particle is splatted
+        uint32_t p_next = a.perm[s0];
+        uint32_t p_after = (s0 + 1 < e0) ? a.perm[s0 + 1] : 0u;
+        PData nxt = load_particle(a, p_next);
+        for (int q = s0; q < e0; ++q) {
+            const PData cur = nxt;
+            if (q + 1 < e0) {
+                nxt = load_particle(a, p_after);
+                if (q + 2 < e0) p_after = a.perm[q + 2];
+            }
+            splat_particle<WSUM>(cur, dx, inv_r2, X0, Y0, Z0, s);
+        }
+    }
+    __syncthreads();
+    float* out = a.stage + (int64_t)tile * R;
+    for (int e = threadIdx.x; e < R; e += blockDim.x) out[e] = s[e];
+}
+
+// Tiles whose region contains index i of an axis: normal (face-normal) axes
+// span [tT, tT+T], tangential / cell axes [tT-1, tT+T].  Returns the count
+// (<= 2) in ascending tile order with the region-local index of i.
+__device__ inline int tiles_normal(int i, int nt, int (&t)[2], int (&l)[2]) {
+    int c = 0;
+    for (int tt = i / T - 1; tt <= i / T; ++tt) {
+        const int li = i - tt * T;
+        if (tt >= 0 && tt < nt && li >= 0 && li <= T) {
+            t[c] = tt;
+            l[c] = li;
+            ++c;
+        }
+    }
+    return c;
+}
+__device__ inline int tiles_tangential(int i, int nt, int (&t)[2], int (&l)[2]) {
+    int c = 0;
+    const int a0 = (i + 1) / T;
+    for (int tt = a0 - 1; tt <= a0; ++tt) {
+        const int li = i - tt * T + 1;
+        if (tt >= 0 && tt < nt && li >= 0 && li <= T + 1) {
+            t[c] = tt;
+            l[c] = li;
+            ++c;
+        }
+    }
+    return c;
+}
+
+template <int A>
+__device__ inline void gather_face(const TileGeom& g, const float* __restrict__ stage, int R, int i,
+                                   int j, int k, float* mass, float* mom) {
+    int tX[2], lX[2], tY[2], lY[2], tZ[2], lZ[2];
+    const int cX = A == 0 ? tiles_normal(i, g.ntx, tX, lX) : tiles_tangential(i, g.ntx, tX, lX);
+    const int cY = A == 1 ? tiles_normal(j, g.nty, tY, lY) : tiles_tangential(j, g.nty, tY, lY);
+    const int cZ = A == 2 ? tiles_normal(k, g.ntz, tZ, lZ) : tiles_tangential(k, g.ntz, tZ, lZ);
+    constexpr int SX = A == 0 ? E * E : (A == 1 ? F * E : E * F);
+    constexpr int SY = A == 2 ? F : E;
+    constexpr int OM = A == 0 ? O_XM : (A == 1 ? O_YM : O_ZM);
+    constexpr int OU = A == 0 ? O_XU : (A == 1 ? O_YU : O_ZU);
+    float am = 0.f, au = 0.f;
+    for (int a = 0; a < cX; ++a)
+        for (int b = 0; b < cY; ++b)
+            for (int c = 0; c < cZ; ++c) {
+                const int64_t tile = ((int64_t)tX[a] * g.nty + tY[b]) * g.ntz + tZ[c];
+                const int64_t off = tile * R + lX[a] * SX + lY[b] * SY + lZ[c];
+                am += stage[off + OM];
+                au += stage[off + OU];
+            }
+    const Dims d{g.nx, g.ny, g.nz};
+    const int f = face_index<A>(d, i, j, k);
+    mass[f] = am;
+    mom[f] = au;
+}
+
+struct GatherArgs {
+    TileGeom g;
+    const float* stage;
+    int R;
+    float* mass[3];
+    float* mom[3];
+    float* cmass;
+    float* cwsum;
+};
+
+__global__ __launch_bounds__(256) void k_p2g_gather(GatherArgs a) {
+    const TileGeom g = a.g;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ez = g.nz + 1, ey = g.ny + 1;
+    if (t >= (int64_t)(g.nx + 1) * ey * ez) return;
+    const int k = (int)(t % ez), j = (int)((t / ez) % ey), i = (int)(t / ((int64_t)ez * ey));
+    const bool ci = i < g.nx, cj = j < g.ny, ck = k < g.nz;
+    if (cj && ck) gather_face<0>(g, a.stage, a.R, i, j, k, a.mass[0], a.mom[0]);
+    if (ci && ck) gather_face<1>(g, a.stage, a.R, i, j, k, a.mass[1], a.mom[1]);
+    if (ci && cj) gather_face<2>(g, a.stage, a.R, i, j, k, a.mass[2], a.mom[2]);
+    if (ci && cj && ck) {
+        int tX[2], lX[2], tY[2], lY[2], tZ[2], lZ[2];
+        const int cX = tiles_tangential(i, g.ntx, tX, lX);
+        const int cY = tiles_tangential(j, g.nty, tY, lY);
+        const int cZ = tiles_tangential(k, g.ntz, tZ, lZ);
+        float am = 0.f, aw = 0.f;
+        for (int x = 0; x < cX; ++x)
+            for (int y = 0; y < cY; ++y)
+                for (int z = 0; z < cZ; ++z) {
+                    const int64_t tile = ((int64_t)tX[x] * g.nty + tY[y]) * g.ntz + tZ[z];
+                    const int64_t off = tile * a.R + (lX[x] * E + lY[y]) * E + lZ[z];
+                    am += a.stage[off + O_CM];
+                    if (a.cwsum) aw += a.stage[off + O_CW];
+                }
+        const int c = (i * g.ny + j) * g.nz + k;
+        a.cmass[c] = am;
+        if (a.cwsum) a.cwsum[c] = aw;
+    }
+}
+
+// ============================================================== G2P + advection
+struct G2PArgs {
+    float* px;
+    float* pv;
+    const int* ptype;
+    const int* pstate;
+    const uint8_t* pflag;
+    int64_t n;
+    Dims d;
+    int flags;
+    const float* vel[3];
+    const float* dvel[3];
+    const float* vel_liq[3];
+    const int* solid;
+    float dx, dt, alpha_l, alpha_a;
+    float lo[3], hi[3];
+};
+
+// trilinear taps of the face grid of axis A at grid coordinates g = x/dx
+// (kernels.py:52-79: indices clamped per axis)
+template <int A>
+struct Taps {
+    int idx[8];
+    float w[8];
+};
+
+template <int A>
+__device__ inline Taps<A> make_taps(float gx, float gy, float gz, const Dims& d) {
+    Taps<A> t;
+    const float ox = A == 0 ? 0.f : 0.5f, oy = A == 1 ? 0.f : 0.5f, oz = A == 2 ? 0.f : 0.5f;
+    const int n0 = d.nx + (A == 0), n1 = d.ny + (A == 1), n2 = d.nz + (A == 2);
+    const float hx = gx - ox, hy = gy - oy, hz = gz - oz;
+    const int i0 = (int)floorf(hx), j0 = (int)floorf(hy), k0 = (int)floorf(hz);
+    const float fx = hx - (float)i0, fy = hy - (float)j0, fz = hz - (float)k0;
+#pragma unroll
+    for (int di = 0; di < 2; ++di) {
+        const float wi = di == 0 ? 1.0f - fx : fx;
+        const int ii = min(max(i0 + di, 0), n0 - 1);
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const float wj = dj == 0 ? 1.0f - fy : fy;
+            const int jj = min(max(j0 + dj, 0), n1 - 1);
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                const float wk = dk == 0 ? 1.0f - fz : fz;
+                const int kk = min(max(k0 + dk, 0), n2 - 1);
+                const int q = di * 4 + dj * 2 + dk;
+                t.w[q] = wi * wj * wk;
+                t.idx[q] = (ii * n1 + jj) * n2 + kk;
+            }
+        }
+    }
+    return t;
+}
+
+template <int A>
+__device__ inline float gather(const Taps<A>& t, const float* __restrict__ f) {
+    float acc = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc += t.w[q] * f[t.idx[q]];
+    return acc;
+}
+
+__device__ inline void sample_vel(const float* const* f, float x, float y, float z, float dx,
+                                  const Dims& d, float& u, float& v, float& w) {
+    const float gx = x / dx, gy = y / dx, gz = z / dx;
+    u = gather(make_taps<0>(gx, gy, gz, d), f[0]);
+    v = gather(make_taps<1>(gx, gy, gz, d), f[1]);
+    w = gather(make_taps<2>(gx, gy, gz, d), f[2]);
+}
+
+__global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    bool liquid, active, dead;
+    if (a.pflag) {
+        const uint8_t fl = a.pflag[p];
+        liquid = fl & 1;
+        active = fl & 2;
+        dead = fl & 4;
+    } else {
+        const int st = a.pstate[p];
+        liquid = a.ptype[p] == 1;
+        active = st == 0;
+        dead = st == 2;
+    }
+    if (dead) return;
+    float x = a.px[3 * p], y = a.px[3 * p + 1], z = a.px[3 * p + 2];
+    float vx = a.pv[3 * p], vy = a.pv[3 * p + 1], vz = a.pv[3 * p + 2];
+    const Dims d = a.d;
+    const float dx = a.dx;
+
+    float pic[3] = {0.f, 0.f, 0.f};
+    bool have_pic = false;
+    if (active && (a.flags & PF_GA_G2P)) {
+        // eq. (12): v = alpha (v + I(du)) + (1 - alpha) I(u)
+        const float gx = x / dx, gy = y / dx, gz = z / dx;
+        const Taps<0> t0 = make_taps<0>(gx, gy, gz, d);
+        const Taps<1> t1 = make_taps<1>(gx, gy, gz, d);
+        const Taps<2> t2 = make_taps<2>(gx, gy, gz, d);
+        const float dux = gather(t0, a.dvel[0]), duy = gather(t1, a.dvel[1]),
+                    duz = gather(t2, a.dvel[2]);
+        pic[0] = gather(t0, a.vel[0]);
+        pic[1] = gather(t1, a.vel[1]);
+        pic[2] = gather(t2, a.vel[2]);
+        have_pic = true;
+        const float al = liquid ? a.alpha_l : a.alpha_a;
+        vx = al * (vx + dux) + (1.0f - al) * pic[0];
+        vy = al * (vy + duy) + (1.0f - al) * pic[1];
+        vz = al * (vz + duz) + (1.0f - al) * pic[2];
+    }
+    if (active && (a.flags & PF_GA_ADVECT)) {
+        // Ralston RK3 (kernels.py:508-549); liquid samples vel_liq, air vel
+        const float* f[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) f[q] = liquid ? a.vel_liq[q] : a.vel[q];
+        float k1x, k1y, k1z;
+        if (!liquid && have_pic) {
+            k1x = pic[0];
+            k1y = pic[1];
+            k1z = pic[2];
+        } else {
+            sample_vel(f, x, y, z, dx, d, k1x, k1y, k1z);
+        }
+        const float h1 = 0.5f * a.dt, h2 = 0.75f * a.dt;
+        float k2x, k2y, k2z, k3x, k3y, k3z;
+        sample_vel(f, x + h1 * k1x, y + h1 * k1y, z + h1 * k1z, dx, d, k2x, k2y, k2z);
+        sample_vel(f, x + h2 * k2x, y + h2 * k2y, z + h2 * k2z, dx, d, k3x, k3y, k3z);
+        const float c1 = 2.0f / 9.0f, c2 = 3.0f / 9.0f, c3 = 4.0f / 9.0f;
+        x = x + a.dt * (c1 * k1x + c2 * k2x + c3 * k3x);
+        y = y + a.dt * (c1 * k1y + c2 * k2y + c3 * k3y);
+        z = z + a.dt * (c1 * k1z + c2 * k2z + c3 * k3z);
+        x = fminf(fmaxf(x, a.lo[0]), a.hi[0]);
+        y = fminf(fmaxf(y, a.lo[1]), a.hi[1]);
+        z = fminf(fmaxf(z, a.lo[2]), a.hi[2]);
+    }
+    if (a.flags & PF_GA_REFLECT) {
+        // kernels.py:567-595
+        float xs[3] = {x, y, z}, vs[3] = {vx, vy, vz};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (xs[ax] <= a.lo[ax]) {
+                xs[ax] = a.lo[ax];
+                vs[ax] = fmaxf(vs[ax], 0.f);
+            } else if (xs[ax] >= a.hi[ax]) {
+                xs[ax] = a.hi[ax];
+                vs[ax] = fminf(vs[ax], 0.f);
+            }
+        }
+        x = xs[0];
+        y = xs[1];
+        z = xs[2];
+        vx = vs[0];
+        vy = vs[1];
+        vz = vs[2];
+    }
+    if ((a.flags & PF_GA_SOLIDS) && a.solid) {
+        // kernels.py:630-679
+        const int i = min(max((int)(x / dx), 0), d.nx - 1);
+        const int j = min(max((int)(y / dx), 0), d.ny - 1);
+        const int k = min(max((int)(z / dx), 0), d.nz - 1);
+        if (a.solid[cell_index(d, i, j, k)] != 0) {
+            float best = 1.0e9f;
+            int bax = 0, bsg = 1;
+            for (int dd = 0; dd < 6; ++dd) {
+                const int ax = dd / 2, sg = (dd % 2 == 0) ? -1 : 1;
+                for (int t = 1; t < 8; ++t) {
+                    const int ii = i + (ax == 0 ? sg * t : 0);
+                    const int jj = j + (ax == 1 ? sg * t : 0);
+                    const int kk = k + (ax == 2 ? sg * t : 0);
+                    if (ii < 0 || jj < 0 || kk < 0 || ii >= d.nx || jj >= d.ny || kk >= d.nz) break;
+                    if (a.solid[cell_index(d, ii, jj, kk)] == 0) {
+                        const float dist = (float)t;
+                        if (dist < best) {
+                            best = dist;
+                            bax = ax;
+                            bsg = sg;
+                        }
+                        break;
+                    }
+                }
+            }
+            if (best <= 1.0e8f) {
+                const float off = (best + 0.5f) * dx * (float)bsg;
+                float xs[3] = {x, y, z}, vs[3] = {vx, vy, vz};
+                xs[bax] = xs[bax] + off;
+                if ((float)bsg * vs[bax] < 0.f) vs[bax] = 0.f;
+                x = xs[0];
+                y = xs[1];
+                z = xs[2];
+                vx = vs[0];
+                vy = vs[1];
+                vz = vs[2];
+            }
+        }
+    }
+    a.px[3 * p] = x;
+    a.px[3 * p + 1] = y;
+    a.px[3 * p + 2] = z;
+    a.pv[3 * p] = vx;
+    a.pv[3 * p + 1] = vy;
+    a.pv[3 * p + 2] = vz;
+}
+
+// ============================================================== CFL speed
+__global__ __launch_bounds__(256) void k_max_speed(const float* __restrict__ pv,
+                                                   const int* __restrict__ pstate, int64_t n,
+                                                   float* out) {
+    __shared__ float sh[4];
+    float m = 0.f;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        if (pstate[p] != 0) continue;
+        const float x = pv[3 * p], y = pv[3 * p + 1], z = pv[3 * p + 2];
+        m = fmaxf(m, sqrtf(x * x + y * y + z * z));
+    }
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) atomic_max_nonneg(out, m);
+}
+
+}  // namespace pf
+
+using namespace pf;
+
+// ============================================================== entry points
+PF_API int64_t pf_bin_nkeys(int nx, int ny, int nz) {
+    return n_tiles(tile_geom(nx, ny, nz)) * T3;
+}
+
+PF_API size_t pf_bin_scan_tmp_bytes(int nx, int ny, int nz) {
+    const int64_t nk = pf_bin_nkeys(nx, ny, nz);
+    // count array (nkeys ints) + block sums + offsets
+    return (size_t)(nk + 2 * (cdiv(nk + 1, SCAN_TILE) + 2)) * sizeof(int);
+}
+
+PF_API int pf_bin_sort(const float* px, const int32_t* ptype, const int32_t* pstate, int64_t n,
+                       int nx, int ny, int nz, float dx, uint32_t* keys, uint32_t* rank,
+                       uint32_t* perm, int32_t* cell_start, uint8_t* pflag, void* scan_tmp,
+                       int32_t* n_active, pf_stream_t stream) {
+    PF_REQUIRE(nx > 0 && ny > 0 && nz > 0 && n >= 0, "pf_bin_sort: bad sizes");
+    PF_REQUIRE(n < (int64_t)1 << 31, "pf_bin_sort: more than 2^31 particles");
+    PF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 31, "pf_bin_sort: grid too large");
+    PF_REQUIRE(!(n > 0 && (!px || !ptype || !pstate || !keys || !rank || !perm || !pflag)),
+               "pf_bin_sort: null particle buffer");
+    PF_REQUIRE(cell_start && scan_tmp, "pf_bin_sort: null cell_start / scan_tmp");
+    PF_REQUIRE(dx > 0.f, "pf_bin_sort: dx must be positive");
+    hipStream_t s = S(stream);
+    const TileGeom g = tile_geom(nx, ny, nz);
+    const int64_t nkeys = n_tiles(g) * T3;
+    int* count = static_cast<int*>(scan_tmp);
+    int* tmp = count + nkeys;
+    int rc = fill_u32(count, nkeys, 0u, s);
+    if (rc) return rc;
+    if (n_active) {
+        rc = fill_u32(n_active, 1, 0u, s);
+        if (rc) return rc;
+    }
+    if (n > 0) {
+        k_bin<<<cdiv(n, 256), 256, 0, s>>>(px, ptype, pstate, n, g, dx, keys, rank, count, pflag,
+                                           n_active);
+        PF_CHECK_LAUNCH();
+    }
+    rc = exclusive_scan(count, nkeys, cell_start, tmp, s);
+    if (rc) return rc;
+    if (n > 0) {
+        k_scatter<<<cdiv(n, 256), 256, 0, s>>>(keys, rank, n, cell_start, perm);
+        PF_CHECK_LAUNCH();
+        k_key_sort<<<(unsigned)(nkeys / SLAB_KEYS), 64, 0, s>>>(cell_start, perm);
+        PF_CHECK_LAUNCH();
+    }
+    return PF_OK;
+}
+
+PF_API size_t pf_p2g_stage_bytes(int nx, int ny, int nz, int with_wsum) {
+    const TileGeom g = tile_geom(nx, ny, nz);
+    return (size_t)n_tiles(g) * (with_wsum ? REG_W : REG) * sizeof(float);
+}
+
+PF_API int pf_p2g(const float* px, const float* pv, const float* pmass, const uint32_t* perm,
+                  const int32_t* cell_start, int nx, int ny, int nz, float dx, float rp,
+                  float* mass_x, float* mass_y, float* mass_z, float* mom_x, float* mom_y,
+                  float* mom_z, float* cell_mass, float* cell_wsum, void* stage,
+                  size_t stage_bytes, pf_stream_t stream) {
+    PF_REQUIRE(nx > 0 && ny > 0 && nz > 0, "pf_p2g: bad grid");
+    PF_REQUIRE((int64_t)(nx + 1) * (ny + 1) * (nz + 1) < (int64_t)1 << 31, "pf_p2g: grid too large");
+    PF_REQUIRE(dx > 0.f && rp > 0.f, "pf_p2g: dx and rp must be positive");
+    // the tiled kernel visits the 2 candidate planes per axis that a support
+    // radius of one cell can reach (kernels.py:137-146 with R = rp/dx = 1)
+    PF_REQUIRE(rp == dx, "pf_p2g: the tiled splat supports r_p == dx only (got rp=%g dx=%g)",
+               (double)rp, (double)dx);
+    PF_REQUIRE(mass_x && mass_y && mass_z && mom_x && mom_y && mom_z && cell_mass,
+               "pf_p2g: null grid output");
+    PF_REQUIRE(perm && cell_start && stage, "pf_p2g: null perm / cell_start / stage");
+    PF_REQUIRE(stage_bytes >= pf_p2g_stage_bytes(nx, ny, nz, cell_wsum != nullptr),
+               "pf_p2g: stage buffer too small (%zu < %zu bytes)", stage_bytes,
+               pf_p2g_stage_bytes(nx, ny, nz, cell_wsum != nullptr));
+    hipStream_t s = S(stream);
+    P2GArgs a;
+    a.px = px;
+    a.pv = pv;
+    a.pmass = pmass;
+    a.perm = perm;
+    a.start = cell_start;
+    a.g = tile_geom(nx, ny, nz);
+    a.dx = dx;
+    a.inv_r2 = 1.0f / (rp * rp);
+    a.stage = static_cast<float*>(stage);
+    const int64_t nt = n_tiles(a.g);
+    if (cell_wsum)
+        k_p2g<true><<<(unsigned)nt, 256, 0, s>>>(a);
+    else
+        k_p2g<false><<<(unsigned)nt, 256, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    GatherArgs ga;
+    ga.g = a.g;
+    ga.stage = a.stage;
+    ga.R = cell_wsum ? REG_W : REG;
+    ga.mass[0] = mass_x;
+    ga.mass[1] = mass_y;
+    ga.mass[2] = mass_z;
+    ga.mom[0] = mom_x;
+    ga.mom[1] = mom_y;
+    ga.mom[2] = mom_z;
+    ga.cmass = cell_mass;
+    ga.cwsum = cell_wsum;
+    const int64_t n = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+    k_p2g_gather<<<cdiv(n, 256), 256, 0, s>>>(ga);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream) {
+    PF_REQUIRE(args, "pf_g2p_advect: null args");
+    const pf_g2p_args& in = *args;
+    PF_REQUIRE(in.nx > 0 && in.ny > 0 && in.nz > 0 && in.n >= 0, "pf_g2p_advect: bad sizes");
+    PF_REQUIRE(in.n < (int64_t)1 << 31, "pf_g2p_advect: more than 2^31 particles");
+    PF_REQUIRE(in.dx > 0.f, "pf_g2p_advect: dx must be positive");
+    if (in.n == 0) return PF_OK;
+    PF_REQUIRE(in.px && in.pv, "pf_g2p_advect: null particles");
+    PF_REQUIRE(in.pflag || (in.ptype && in.pstate), "pf_g2p_advect: need pflag or ptype+pstate");
+    if (in.flags & PF_GA_G2P)
+        for (int q = 0; q < 3; ++q)
+            PF_REQUIRE(in.vel[q] && in.dvel[q], "pf_g2p_advect: G2P needs vel and dvel");
+    if (in.flags & PF_GA_ADVECT)
+        for (int q = 0; q < 3; ++q)
+            PF_REQUIRE(in.vel[q] && in.vel_liq[q], "pf_g2p_advect: advection needs vel, vel_liq");
+    PF_REQUIRE(!(in.flags & PF_GA_SOLIDS) || in.solid, "pf_g2p_advect: solids flag without mask");
+    G2PArgs a;
+    a.px = in.px;
+    a.pv = in.pv;
+    a.ptype = in.ptype;
+    a.pstate = in.pstate;
+    a.pflag = in.pflag;
+    a.n = in.n;
+    a.d = Dims{in.nx, in.ny, in.nz};
+    a.flags = in.flags;
+    for (int q = 0; q < 3; ++q) {
+        a.vel[q] = in.vel[q];
+        a.dvel[q] = in.dvel[q];
+        a.vel_liq[q] = in.vel_liq[q];
+        a.lo[q] = in.lo[q];
+        a.hi[q] = in.hi[q];
+    }
+    a.solid = in.solid;
+    a.dx = in.dx;
+    a.dt = in.dt;
+    a.alpha_l = in.alpha_liquid;
+    a.alpha_a = in.alpha_air;
+    k_g2p_advect<<<cdiv(in.n, 256), 256, 0, S(stream)>>>(a);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_max_speed(const float* pv, const int32_t* pstate, int64_t n, float* out,
+                        pf_stream_t stream) {
+    PF_REQUIRE(out, "pf_max_speed: null output");
+    PF_REQUIRE(n == 0 || (pv && pstate), "pf_max_speed: null particles");
+    hipStream_t s = S(stream);
+    int rc = fill_u32(out, 1, 0u, s);
+    if (rc) return rc;
+    if (n == 0) return PF_OK;
+    const int64_t nb = cdiv(n, 256 * 8);
+    k_max_speed<<<(unsigned)(nb < 4096 ? nb : 4096), 256, 0, s>>>(pv, pstate, n, out);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}

@@ -1,0 +1,155 @@
+"""Variable-coefficient pressure Poisson solver (drop-in for pfflip/poisson.py).
+
+``MGPCG`` keeps the reference's constructor, level table (``levels[i].bx``,
+``.by``, ``.bz``, ``.diag``, ``.p``, ``.b``, ``.r``), ``nlev``, ``ncells``,
+``build_hierarchy``, ``precondition`` and ``solve`` (pfflip/poisson.py:226-409);
+the work is done by ``pf_mg_*`` / ``pf_mgpcg_solve`` in libpfflip_hip.so.
+
+Differences from the reference that do not change the algorithm:
+
+* the FPCG scalars stay on the GPU; ``solve`` synchronises with the host
+  about once per iteration (one iteration behind) instead of six times;
+* dot products and means are deterministic FP64 reductions instead of FP32
+  atomics, and the Polak-Ribiere numerator uses r_prev = r + alpha Ap, so no
+  r_prev copy is kept (``r_prev`` exists as an attribute but is unused);
+* ``precision="bf16"`` / ``"fp16"`` stores the V-cycle's face coefficients and
+  iterates in 16 bits on the levels that run out of HBM (FP32 arithmetic);
+  the flexible CG absorbs the inexact preconditioner.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+
+PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
+_HALF = {1: torch.bfloat16, 2: torch.float16}
+
+
+def level_dims(res, min_dim):
+    """pfflip/poisson.py:256-263."""
+    out = []
+    r = tuple(int(n) for n in res)
+    while True:
+        out.append(r)
+        if min(r) // 2 < min_dim or any(n % 2 for n in r):
+            break
+        r = tuple(n // 2 for n in r)
+    return out
+
+
+class _Level:
+    """One level of the hierarchy (pfflip/poisson.py:226-237)."""
+
+    def __init__(self, res, device):
+        nx, ny, nz = res
+        self.res = (nx, ny, nz)
+        z = lambda shape: torch.zeros(shape, dtype=torch.float32, device=device)  # noqa: E731
+        self.bx = z((nx + 1, ny, nz))
+        self.by = z((nx, ny + 1, nz))
+        self.bz = z((nx, ny, nz + 1))
+        self.diag = z((nx, ny, nz))
+        self.p = z((nx, ny, nz))
+        self.b = z((nx, ny, nz))
+        self.r = z((nx, ny, nz))
+        self.bx_h = self.by_h = self.bz_h = self.p_h = None
+
+
+def _require_cuda(device):
+    dev = torch.device(device)
+    if dev.type != "cuda" or not torch.cuda.is_available():
+        raise RuntimeError(f"pfflip_amd runs on the GPU only (device {device!r} is not available)")
+    return dev
+
+
+class MGPCG:
+    """Multigrid-preconditioned flexible CG (pfflip/poisson.py:240-409)."""
+
+    def __init__(self, res, device, *, omega=6.0 / 7.0, pre=2, post=2, coarsest=60, min_dim=4,
+                 precision="fp32"):
+        self.device = _require_cuda(device)
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.omega, self.pre, self.post, self.coarsest = omega, pre, post, coarsest
+        self.precision = precision
+        dims = level_dims(res, min_dim)
+        if len(dims) > _lib.MAX_LEVELS:
+            raise ValueError("too many multigrid levels")
+        self.levels = [_Level(r, self.device) for r in dims]
+        self.nlev = len(self.levels)
+        self.ncells = int(res[0] * res[1] * res[2])
+        z = lambda: torch.zeros(tuple(res), dtype=torch.float32, device=self.device)  # noqa: E731
+        self.rhs, self.res_v, self.z, self.q, self.Ap, self.r_prev = z(), z(), z(), z(), z(), z()
+        lib = _lib.load()
+        self._scalars = torch.zeros(int(lib.pf_mg_scalar_bytes()), dtype=torch.uint8,
+                                    device=self.device)
+        mg = _lib.MG()
+        mg.nlev = self.nlev
+        mg.pre, mg.post, mg.coarsest = int(pre), int(post), int(coarsest)
+        mg.omega = float(omega)
+        mg.precision = PRECISIONS[precision]
+        for i, L in enumerate(self.levels):
+            mg.lev[i].nx, mg.lev[i].ny, mg.lev[i].nz = L.res
+        _lib.check(lib.pf_mg_setup(ctypes.byref(mg)), "pf_mg_setup")
+        self.small_from = int(mg.small_from)
+        if mg.precision:
+            # 16-bit faces and iterates on the levels run out of HBM; level 0's
+            # 16-bit iterate is the preconditioned vector z in every case
+            hd = _HALF[mg.precision]
+            for i, L in enumerate(self.levels):
+                nx, ny, nz = L.res
+                if i < self.small_from:
+                    L.bx_h = torch.zeros((nx + 1, ny, nz), dtype=hd, device=self.device)
+                    L.by_h = torch.zeros((nx, ny + 1, nz), dtype=hd, device=self.device)
+                    L.bz_h = torch.zeros((nx, ny, nz + 1), dtype=hd, device=self.device)
+                if i < self.small_from or i == 0:
+                    L.p_h = torch.zeros((nx, ny, nz), dtype=hd, device=self.device)
+        for i, L in enumerate(self.levels):
+            lv = mg.lev[i]
+            for name in ("bx", "by", "bz", "diag", "p", "b", "r", "bx_h", "by_h", "bz_h", "p_h"):
+                t = getattr(L, name)
+                setattr(lv, name, None if t is None else t.data_ptr())
+        mg.rhs, mg.res, mg.z = self.rhs.data_ptr(), self.res_v.data_ptr(), self.z.data_ptr()
+        mg.q, mg.Ap, mg.scalars = self.q.data_ptr(), self.Ap.data_ptr(), self._scalars.data_ptr()
+        self._mg = mg
+
+    # ---------------------------------------------------------------- hierarchy
+    def build_hierarchy(self, bx, by, bz):
+        """Level 0 = beta (copied unless it already is level 0's storage),
+        Galerkin coarsening all the way down (pfflip/poisson.py:295-309)."""
+        for t, L in zip((bx, by, bz), (self.levels[0].bx, self.levels[0].by, self.levels[0].bz)):
+            if tuple(t.shape) != tuple(L.shape) or t.dtype != torch.float32 or not t.is_contiguous():
+                raise ValueError("build_hierarchy: coefficients must be contiguous float32 face arrays")
+        _lib.call("pf_mg_build", ctypes.byref(self._mg), _lib.ptr(bx), _lib.ptr(by), _lib.ptr(bz),
+                  _lib.stream())
+
+    def precondition(self, r_in, z_out):
+        """z = V-cycle(r) with the mean removed (pfflip/poisson.py:345-352)."""
+        _lib.call("pf_mg_precondition", ctypes.byref(self._mg), _lib.ptr(r_in), _lib.ptr(z_out),
+                  _lib.stream())
+
+    # ---------------------------------------------------------------- solve
+    def solve(self, b, p_out, *, tol=1e-4, max_iter=200, verbose=False):
+        """Flexible PCG with Polak-Ribiere beta (pfflip/poisson.py:355-409).
+
+        Writes ``p_out`` (mean removed) and returns ``(iterations, rel)`` with
+        ``rel = max|r| / max|b - mean(b)|`` of the recursive residual.
+        """
+        it = ctypes.c_int32(0)
+        rel = ctypes.c_double(0.0)
+        _lib.call("pf_mgpcg_solve", ctypes.byref(self._mg), _lib.ptr(b), _lib.ptr(p_out),
+                  float(tol), int(max_iter), ctypes.byref(it), ctypes.byref(rel), _lib.stream())
+        if verbose:
+            print(f"    pcg it={it.value:3d}  |r|inf/|b|inf = {rel.value:.3e}")
+        return int(it.value), float(rel.value)
+
+
+def apply_A(p, bx, by, bz, diag, out):
+    """``out = A p`` (the ``apply_A`` kernel, pfflip/poisson.py:54-88)."""
+    nx, ny, nz = p.shape
+    _lib.call("pf_apply_A", nx, ny, nz, _lib.ptr(p), _lib.ptr(bx), _lib.ptr(by), _lib.ptr(bz),
+              _lib.ptr(diag), _lib.ptr(out), _lib.stream())
+    return out

@@ -1,0 +1,369 @@
+"""PF-FLIP simulator on the uniform MAC grid (drop-in for pfflip/sim.py).
+
+Same construction, attributes (``vel``, ``vel_star``, ``vel_liq``, ``dvel``,
+``mass``, ``mom``, ``beta``, ``phi_face``, ``valid``, ``cell_mass``,
+``cell_wsum``, ``phi_cell``, ``is_liquid``, ``solid``, ``pressure``, ``rhs``,
+``px``, ``pv``, ``ptype``, ``pmass``, ``prad``, ``pstate``, ``pdrop_r``, ...),
+stage methods and ``step`` record as the reference ``Simulator``
+(pfflip/sim.py:37-424).  Arrays are torch tensors on the GPU with the
+reference's shapes and dtypes; every stage runs in libpfflip_hip.so.
+
+``step`` fuses stages that the reference launches separately (it produces
+the same fields): binning + tiled P2G; eq. (7) / beta / walls / vel_star /
+gravity / RHS in one grid pass; projection + FLIP delta; G2P + RK3 +
+reflection in one particle pass.  The stage methods (``_p2g``,
+``_phase_field_and_coefficients``, ``_add_forces``, ``_pressure_solve``,
+``_project``, ``_max_divergence``, ``_g2p``, ``_extrapolate_liquid_velocity``,
+``_advect``) keep their individual reference semantics.
+
+Not implemented: the escaped-particle / droplet stage (pfflip/sim.py:343-401)
+and ``rasterize_spray``.  With ``cfg.droplets`` True the stage is skipped
+with a warning.  Kernel radius: ``prad`` must stay ``cfg.r_p`` (= dx), which
+is what the reference's uniform path always sets (sim.py:153).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+import warnings
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import Config
+from .poisson import MGPCG, _require_cuda
+
+
+def _rho_tilde_0(cfg: Config) -> float:
+    """Kernel weight sum of a perfect lattice (pfflip/sim.py:119-140)."""
+    dx, r = cfg.dx, cfg.r_p
+    per_axis = round(cfg.ppc ** (1.0 / 3.0))
+    step = dx / per_axis
+    R = int(math.ceil(r / step)) + 1
+    acc = 0.0
+    for a in range(-R, R + 1):
+        for b in range(-R, R + 1):
+            for c in range(-R, R + 1):
+                d2 = ((a + 0.5) ** 2 + (b + 0.5) ** 2 + (c + 0.5) ** 2) * step * step
+                t = max(1.0 - d2 / (r * r), 0.0)
+                acc += t * t * t
+    return acc
+
+
+class Simulator:
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self.dev = _require_cuda(cfg.device)
+        nx, ny, nz = (int(n) for n in cfg.res)
+        self.res = (nx, ny, nz)
+        dev = self.dev
+        z = lambda shape: torch.zeros(shape, dtype=torch.float32, device=dev)  # noqa: E731
+        zi = lambda shape: torch.zeros(shape, dtype=torch.int32, device=dev)  # noqa: E731
+        zb = lambda shape: torch.zeros(shape, dtype=torch.uint8, device=dev)  # noqa: E731
+        self.face_shapes = ((nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1))
+
+        self.solver = MGPCG(self.res, cfg.device, omega=cfg.mg_omega, pre=cfg.mg_pre_sweeps,
+                            post=cfg.mg_post_sweeps, coarsest=cfg.mg_coarsest_sweeps,
+                            min_dim=cfg.mg_min_dim, precision=cfg.solver_precision)
+
+        self.vel = [z(s) for s in self.face_shapes]
+        self.vel_star = [z(s) for s in self.face_shapes]
+        self.vel_liq = [z(s) for s in self.face_shapes]
+        self.dvel = [z(s) for s in self.face_shapes]
+        self.mass = [z(s) for s in self.face_shapes]
+        self.mom = [z(s) for s in self.face_shapes]
+        # beta is stored directly in the solver's level-0 coefficients, so
+        # build_hierarchy does not copy it (the reference copies, poisson.py:298)
+        L0 = self.solver.levels[0]
+        self.beta = [L0.bx, L0.by, L0.bz]
+        self.phi_face = [z(s) for s in self.face_shapes]
+        self.valid = [zi(s) for s in self.face_shapes]
+        self._known = [zb(s) for s in self.face_shapes]
+        self._known2 = [zb(s) for s in self.face_shapes]
+        self._tmp_face = [z(s) for s in self.face_shapes]
+
+        self.cell_mass = z(self.res)
+        self.cell_wsum = z(self.res)
+        self.phi_cell = z(self.res)
+        self.is_liquid = zi(self.res)
+        self.solid = zi(self.res)
+        self.pressure = z(self.res)
+        self.rhs = z(self.res)
+        self._scalar = z((4,))
+
+        self.px = self.pv = self.ptype = self.pmass = self.prad = self.pstate = self.pdrop_r = None
+        self.n_particles = 0
+        self.droplet_stats = {"emitted": 0, "bubbles": 0, "merged": 0, "spray": 0}
+
+        self.rho_tilde_0 = _rho_tilde_0(cfg)
+        self.rho_0l = self.rho_tilde_0 * cfg.rho_l
+        self.rho_min = cfg.eta_phi * cfg.rho_g * self.rho_tilde_0
+
+        # optional per-stage GPU timing: set to {} to accumulate milliseconds
+        # per stage of step() (HIP events on the current stream)
+        self.stage_ms = None
+        self._marks = []
+
+        self.has_solids = False
+        self.time = 0.0
+        self.step_index = 0
+        self.stats: list[dict] = []
+        self._warned_droplets = False
+
+        lib = _lib.load()
+        self._nkeys = int(lib.pf_bin_nkeys(nx, ny, nz))
+        self.cell_start = torch.zeros(self._nkeys + 1, dtype=torch.int32, device=dev)
+        nb = int(lib.pf_bin_scan_tmp_bytes(nx, ny, nz))
+        self._scan_tmp = torch.zeros((nb + 3) // 4, dtype=torch.int32, device=dev)
+        self._n_active = zi((1,))
+        sb = int(lib.pf_p2g_stage_bytes(nx, ny, nz, 1 if cfg.k_div > 0.0 else 0))
+        self._p2g_stage = torch.empty((sb + 3) // 4, dtype=torch.float32, device=dev)
+
+    # ------------------------------------------------------------------ setup
+    def set_solids(self, mask):
+        """Obstacle mask on the cell grid, True = solid (sim.py:113-116)."""
+        m = np.ascontiguousarray(mask, np.int32)
+        if m.shape != self.res:
+            raise ValueError(f"solid mask shape {m.shape} != grid {self.res}")
+        self.solid.copy_(torch.from_numpy(m))
+        self.has_solids = bool(m.any())
+
+    def set_particles(self, pos, vel, ptype):
+        """Upload particles (sim.py:142-155); pmass = rho_phase, prad = r_p."""
+        cfg = self.cfg
+        pos = np.asarray(pos)
+        n = len(pos)
+        if n >= 2**31:
+            raise ValueError("at most 2^31 - 1 particles")
+        self.n_particles = n
+        dev = self.dev
+        self.px = torch.from_numpy(np.ascontiguousarray(pos, np.float32).reshape(n, 3)).to(dev)
+        self.pv = torch.from_numpy(np.ascontiguousarray(vel, np.float32).reshape(n, 3)).to(dev)
+        pt = np.ascontiguousarray(ptype, np.int32)
+        self.ptype = torch.from_numpy(pt).to(dev)
+        self.pmass = torch.from_numpy(np.where(pt == 1, cfg.rho_l, cfg.rho_g).astype(np.float32)).to(dev)
+        self.prad = torch.full((n,), cfg.r_p, dtype=torch.float32, device=dev)
+        self.pstate = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.pdrop_r = torch.zeros(n, dtype=torch.float32, device=dev)
+        # binning buffers (keys, slot ranks, sorted indices, per-particle flags)
+        self._keys = torch.empty(n, dtype=torch.int32, device=dev)
+        self._rank = torch.empty(n, dtype=torch.int32, device=dev)
+        self.perm = torch.empty(n, dtype=torch.int32, device=dev)
+        self.pflag = torch.empty(n, dtype=torch.uint8, device=dev)
+
+    # ------------------------------------------------------------------ helpers
+    def _read(self, t, count=1):
+        out = (ctypes.c_float * count)()
+        _lib.call("pf_read_scalars", _lib.ptr(t), ctypes.cast(out, ctypes.c_void_p),
+                  4 * count, _lib.stream())
+        return list(out)
+
+    def _lohi(self):
+        eps = 1e-4 * self.cfg.dx
+        return (eps, eps, eps), tuple(s - eps for s in self.cfg.domain_size)
+
+    def _phase_args(self, flags, dt=0.0):
+        cfg = self.cfg
+        a = _lib.PhaseArgs()
+        a.nx, a.ny, a.nz = self.res
+        a.flags = flags
+        a.mass = _lib.ptr3(self.mass)
+        a.mom = _lib.ptr3(self.mom)
+        a.vel = _lib.ptr3(self.vel)
+        a.vel_star = _lib.ptr3(self.vel_star)
+        a.valid = _lib.ptr3(self.valid)
+        a.cell_mass = self.cell_mass.data_ptr()
+        a.cell_wsum = self.cell_wsum.data_ptr()
+        a.phi_cell = self.phi_cell.data_ptr()
+        a.is_liquid = self.is_liquid.data_ptr()
+        a.phi_face = _lib.ptr3(self.phi_face)
+        a.beta = _lib.ptr3(self.beta)
+        a.solid = self.solid.data_ptr() if self.has_solids else None
+        a.rhs = self.rhs.data_ptr()
+        a.mass_eps = 1e-12 * self.rho_0l
+        a.rho_min = self.rho_min
+        a.alpha_phi = cfg.alpha_phi
+        a.rho_0l = self.rho_0l
+        a.rho_l = cfg.rho_l
+        a.rho_g = cfg.rho_g
+        a.gravity = (ctypes.c_float * 3)(*[float(g) for g in cfg.gravity])
+        a.dt = float(dt)
+        a.dx = cfg.dx
+        a.rho_tilde_0 = self.rho_tilde_0
+        a.k_div = cfg.k_div
+        return a
+
+    def _phase(self, flags, dt=0.0):
+        a = self._phase_args(flags, dt)
+        _lib.call("pf_phase_beta_forces", ctypes.byref(a), _lib.stream())
+
+    def _g2p_args(self, flags, dt=0.0, use_pflag=False):
+        cfg = self.cfg
+        a = _lib.G2PArgs()
+        a.px, a.pv = self.px.data_ptr(), self.pv.data_ptr()
+        a.ptype, a.pstate = self.ptype.data_ptr(), self.pstate.data_ptr()
+        a.pflag = self.pflag.data_ptr() if use_pflag else None
+        a.n = self.n_particles
+        a.nx, a.ny, a.nz = self.res
+        a.flags = flags | (_lib.GA_SOLIDS if (self.has_solids and flags & _lib.GA_REFLECT) else 0)
+        a.vel = _lib.ptr3(self.vel)
+        a.dvel = _lib.ptr3(self.dvel)
+        a.vel_liq = _lib.ptr3(self.vel_liq)
+        a.solid = self.solid.data_ptr() if self.has_solids else None
+        a.dx, a.dt = cfg.dx, float(dt)
+        a.alpha_liquid, a.alpha_air = cfg.alpha_flip_liquid, cfg.alpha_flip_air
+        lo, hi = self._lohi()
+        a.lo = (ctypes.c_float * 3)(*lo)
+        a.hi = (ctypes.c_float * 3)(*hi)
+        return a
+
+    def _bin(self):
+        """Cell-sort the FLIP particles (pf_bin_sort)."""
+        nx, ny, nz = self.res
+        _lib.call("pf_bin_sort", _lib.ptr(self.px), _lib.ptr(self.ptype), _lib.ptr(self.pstate),
+                  self.n_particles, nx, ny, nz, self.cfg.dx, _lib.ptr(self._keys),
+                  _lib.ptr(self._rank), _lib.ptr(self.perm), _lib.ptr(self.cell_start),
+                  _lib.ptr(self.pflag), _lib.ptr(self._scan_tmp), _lib.ptr(self._n_active),
+                  _lib.stream())
+
+    def _splat(self):
+        nx, ny, nz = self.res
+        # cell_wsum is only read when k_div > 0 (kernels.py:360-362); otherwise
+        # it keeps its zeros
+        wsum = self.cell_wsum if self.cfg.k_div > 0.0 else None
+        _lib.call("pf_p2g", _lib.ptr(self.px), _lib.ptr(self.pv), _lib.ptr(self.pmass),
+                  _lib.ptr(self.perm), _lib.ptr(self.cell_start), nx, ny, nz, self.cfg.dx,
+                  self.cfg.r_p, *[_lib.ptr(t) for t in self.mass], *[_lib.ptr(t) for t in self.mom],
+                  _lib.ptr(self.cell_mass), _lib.ptr(wsum), _lib.ptr(self._p2g_stage),
+                  4 * self._p2g_stage.numel(), _lib.stream())
+
+    def _mark(self, name):
+        if self.stage_ms is not None:
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            self._marks.append((name, e))
+
+    def _collect_marks(self):
+        if self.stage_ms is None or not self._marks:
+            return
+        for (_, e0), (name, e1) in zip(self._marks[:-1], self._marks[1:]):
+            self.stage_ms[name] = self.stage_ms.get(name, 0.0) + e0.elapsed_time(e1)
+        self._marks = []
+
+    # ------------------------------------------------------------------ step
+    def dt_from_cfl(self) -> float:
+        """pfflip/sim.py:158-166 (speed over pstate == 0 particles only)."""
+        cfg = self.cfg
+        _lib.call("pf_max_speed", _lib.ptr(self.pv), _lib.ptr(self.pstate), self.n_particles,
+                  _lib.ptr(self._scalar), _lib.stream())
+        vmax = float(self._read(self._scalar)[0])
+        g = float(np.linalg.norm(cfg.gravity))
+        vmax = max(vmax, math.sqrt(2.0 * g * cfg.dx) if g > 0 else 0.0, 1e-8)
+        return float(np.clip(cfg.cfl * cfg.dx / vmax, cfg.dt_min, cfg.dt_max))
+
+    def step(self, dt: float | None = None, verbose: bool = False) -> dict:
+        """One time step (pfflip/sim.py:168-202); droplet stage not included."""
+        cfg = self.cfg
+        if cfg.droplets and not self._warned_droplets:
+            warnings.warn("Config.droplets=True: the escaped-particle / droplet stage is not "
+                          "implemented and is skipped", RuntimeWarning, stacklevel=2)
+            self._warned_droplets = True
+        if dt is None:
+            dt = self.dt_from_cfl()
+        t0 = time.perf_counter()
+        self._mark("start")
+        self._bin()
+        self._mark("bin_sort")
+        self._splat()
+        self._mark("p2g")
+        self._phase(_lib.PH_VEL_FROM_MOM | _lib.PH_PHASE | _lib.PH_ZERO | _lib.PH_STAR
+                    | _lib.PH_GRAVITY | _lib.PH_RHS, dt)
+        self._mark("phase_beta_rhs")
+        self.solver.build_hierarchy(*self.beta)
+        self._mark("mg_build")
+        pcg_it, pcg_res = self.solver.solve(self.rhs, self.pressure, tol=cfg.pcg_tol,
+                                            max_iter=cfg.pcg_max_iter, verbose=verbose)
+        self._mark("mgpcg_solve")
+        nx, ny, nz = self.res
+        _lib.call("pf_project", _lib.ptr(self.pressure), _lib.ptr3(self.beta), _lib.ptr3(self.vel),
+                  _lib.ptr3(self.vel_star), _lib.ptr3(self.dvel), nx, ny, nz, cfg.dx, float(dt),
+                  _lib.ptr(self._scalar), _lib.stream())
+        self._mark("project")
+        self._extrapolate_liquid_velocity()
+        self._mark("extrapolate")
+        a = self._g2p_args(_lib.GA_G2P | _lib.GA_ADVECT | _lib.GA_REFLECT, dt, use_pflag=True)
+        _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
+        self._mark("g2p_advect")
+        div_after = float(self._read(self._scalar)[0])   # synchronises
+        wall = time.perf_counter() - t0
+        self._collect_marks()
+        self.time += dt
+        self.step_index += 1
+        rec = {"step": self.step_index, "t": self.time, "dt": dt, "pcg_iter": pcg_it,
+               "pcg_res": pcg_res, "max_div": div_after, "droplets": 0, "wall_s": wall}
+        self.stats.append(rec)
+        return rec
+
+    # ------------------------------------------------------------------ stages
+    def _p2g(self):
+        """P2G + face velocities (pfflip/sim.py:205-236)."""
+        self._bin()
+        self._splat()
+        self._phase(_lib.PH_VEL_FROM_MOM)
+
+    def _phase_field_and_coefficients(self):
+        """Eq. (7), beta, walls and solids (pfflip/sim.py:238-268)."""
+        self._phase(_lib.PH_PHASE | _lib.PH_ZERO)
+
+    def _add_forces(self, dt: float):
+        """Gravity, then walls and solids again (pfflip/sim.py:270-285)."""
+        self._phase(_lib.PH_GRAVITY | _lib.PH_ZERO, dt)
+
+    def _pressure_solve(self, dt: float, verbose: bool = False):
+        """RHS, hierarchy, MG-PCG (pfflip/sim.py:287-299)."""
+        cfg = self.cfg
+        self._phase(_lib.PH_RHS, dt)
+        self.solver.build_hierarchy(*self.beta)
+        return self.solver.solve(self.rhs, self.pressure, tol=cfg.pcg_tol,
+                                 max_iter=cfg.pcg_max_iter, verbose=verbose)
+
+    def _project(self, dt: float):
+        """u -= dt beta grad p / dx (pfflip/sim.py:301-308)."""
+        nx, ny, nz = self.res
+        _lib.call("pf_project", _lib.ptr(self.pressure), _lib.ptr3(self.beta), _lib.ptr3(self.vel),
+                  None, None, nx, ny, nz, self.cfg.dx, float(dt), None, _lib.stream())
+
+    def _max_divergence(self) -> float:
+        """max |div u / dx| (pfflip/sim.py:310-314)."""
+        nx, ny, nz = self.res
+        _lib.call("pf_project", None, None, _lib.ptr3(self.vel), None, None, nx, ny, nz,
+                  self.cfg.dx, 0.0, _lib.ptr(self._scalar), _lib.stream())
+        return float(self._read(self._scalar)[0])
+
+    def _g2p(self):
+        """dvel = vel - vel_star, then eq. (12) (pfflip/sim.py:316-327)."""
+        nx, ny, nz = self.res
+        _lib.call("pf_project", None, None, _lib.ptr3(self.vel), _lib.ptr3(self.vel_star),
+                  _lib.ptr3(self.dvel), nx, ny, nz, self.cfg.dx, 0.0, None, _lib.stream())
+        a = self._g2p_args(_lib.GA_G2P)
+        _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
+
+    def _extrapolate_liquid_velocity(self):
+        """Seed + Jacobi sweeps into vel_liq (pfflip/sim.py:329-341)."""
+        nx, ny, nz = self.res
+        _lib.call("pf_extrapolate", _lib.ptr3(self.vel), _lib.ptr(self.is_liquid),
+                  _lib.ptr3(self.vel_liq), _lib.ptr3(self._tmp_face), _lib.ptr3(self._known),
+                  _lib.ptr3(self._known2), nx, ny, nz, int(self.cfg.extrap_cells), _lib.stream())
+
+    def _advect(self, dt: float):
+        """RK3, wall reflection, solid push-out (pfflip/sim.py:408-424)."""
+        a = self._g2p_args(_lib.GA_ADVECT | _lib.GA_REFLECT, dt)
+        _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
+
+    def state_counts(self):
+        st = self.pstate.cpu().numpy()
+        return {"flip": int((st == 0).sum()), "droplet": int((st == 1).sum()),
+                "dead": int((st == 2).sum())}

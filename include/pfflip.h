@@ -1,0 +1,258 @@
+/*
+ * pfflip.h -- C ABI of the MI355X (gfx950) PF-FLIP time-step library
+ * (libpfflip_hip.so).
+ *
+ * The reference (a Python + Warp package) has no FFI layer: its boundary is
+ * the Python object API of `Simulator` / `MGPCG`, whose stages launch Warp
+ * kernels.  Each entry point below replaces the stage (or kernel group) cited
+ * next to it, as `path:line` in the reference repository.  The Python
+ * drop-in (`Simulator`, `MGPCG`, `apply_A`) binds these with ctypes; the
+ * binding a maintainer would add is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *  - Return 0 on success, a negative PF_ERR_* for invalid arguments, or a
+ *    positive hipError_t.  pf_last_error() gives a thread-local message.
+ *  - All arrays are caller-owned device memory, C-contiguous with the last
+ *    index fastest.  Cell (i,j,k) is element (i*ny + j)*nz + k.  Face arrays
+ *    have shapes (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1).  Particle
+ *    positions/velocities are float32 (n,3) arrays of structures.
+ *  - Every call is asynchronous on `stream` (a hipStream_t), except
+ *    pf_mgpcg_solve and pf_read_scalar*, which block until their result is
+ *    known.  No entry point allocates device memory unless it says so.
+ *  - No exceptions cross the ABI.
+ */
+#ifndef PFFLIP_H
+#define PFFLIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_OK 0
+#define PF_ERR_INVALID (-1)   /* bad pointer, size or flag combination      */
+#define PF_ERR_SIZE (-2)      /* a size exceeds what a kernel supports      */
+#define PF_ERR_UNSUPPORTED (-3)
+
+#define PF_MAX_LEVELS 16
+#define PF_TILE 8 /* P2G tile edge in cells; binning keys are tile-major */
+
+typedef void* pf_stream_t; /* hipStream_t */
+
+/* ---------------------------------------------------------------- common */
+const char* pf_last_error(void);
+int pf_abi_version(void);
+
+/* Fill `n` 32-bit words with `value` (a kernel, safe under graph capture). */
+int pf_fill_u32(void* dst, int64_t n, uint32_t value, pf_stream_t stream);
+
+/* Read one device float / double / int into host memory (blocks). */
+int pf_read_scalars(const void* dev_src, void* host_dst, size_t bytes, pf_stream_t stream);
+
+/* ------------------------------------------------ particle binning + sort
+ * Build-added: the reference splats with per-particle global atomics in
+ * particle order (pfflip/kernels.py:100-209) and has no sort.  Keys feed
+ * pf_p2g.  For every particle with pstate == 0:
+ *   cell = clamp(floor(x / dx), 0, n-1) per axis (FP32 division),
+ *   key  = tile_id * PF_TILE^3 + local_id,
+ *   tile_id = (tx*nty + ty)*ntz + tz, local_id = (li*T + lj)*T + lk.
+ * Other particles get key 0xFFFFFFFF and are left out of `perm`.
+ * `perm` lists particle indices sorted by key, ties by index (a unique
+ * permutation); cell_start[key] .. cell_start[key+1] is a key's range.
+ * pflag[p] = (ptype==1) | (pstate==0)<<1 | (pstate==2)<<2.
+ * Buffers: keys, rank, perm, pflag: n entries; cell_start: nkeys+1 with
+ * nkeys = pf_bin_nkeys(); scan_tmp: pf_bin_scan_tmp_bytes() bytes.
+ * *n_active (device int32, may be NULL) receives the number of FLIP
+ * particles. */
+int64_t pf_bin_nkeys(int nx, int ny, int nz);
+size_t pf_bin_scan_tmp_bytes(int nx, int ny, int nz);
+int pf_bin_sort(const float* px, const int32_t* ptype, const int32_t* pstate, int64_t n,
+                int nx, int ny, int nz, float dx,
+                uint32_t* keys, uint32_t* rank, uint32_t* perm, int32_t* cell_start,
+                uint8_t* pflag, void* scan_tmp, int32_t* n_active, pf_stream_t stream);
+
+/* ---------------------------------------------------------------- P2G
+ * Replaces Simulator._p2g's zeroing + p2g_face x3 + p2g_cell
+ * (pfflip/sim.py:205-228, pfflip/kernels.py:100-209): eq. (6) weights
+ * w = max(1 - d^2/r^2, 0)^3 with r = rp (= dx), mass += w*m,
+ * mom_a += w*m*v_a, cell_mass += w*m, cell_wsum += w (if non-NULL).
+ * Every output element is written (no pre-zeroing needed).  Consumes
+ * perm/cell_start from pf_bin_sort.  `stage` is scratch of
+ * pf_p2g_stage_bytes(nx, ny, nz, cell_wsum != NULL) bytes. */
+size_t pf_p2g_stage_bytes(int nx, int ny, int nz, int with_wsum);
+int pf_p2g(const float* px, const float* pv, const float* pmass,
+           const uint32_t* perm, const int32_t* cell_start,
+           int nx, int ny, int nz, float dx, float rp,
+           float* mass_x, float* mass_y, float* mass_z,
+           float* mom_x, float* mom_y, float* mom_z,
+           float* cell_mass, float* cell_wsum, void* stage, size_t stage_bytes,
+           pf_stream_t stream);
+
+/* ------------------------------------------ phase field / beta / forces
+ * One fused grid pass; `flags` selects the reference stages it performs,
+ * applied per face in the reference's order:
+ *   PF_PH_VEL_FROM_MOM  vel = mom/mass if mass > mass_eps else 0, valid
+ *                       (face_velocity_from_momentum, kernels.py:212-228);
+ *                       without it `vel` is read.
+ *   PF_PH_PHASE         phi_cell, is_liquid (cell_phase, kernels.py:242-255),
+ *                       phi_face, beta (face_beta, kernels.py:258-311)
+ *   PF_PH_ZERO          walls and solid faces: beta = 0, vel = 0
+ *                       (kernels.py:314-326, 601-627)
+ *   PF_PH_STAR          vel_star = vel after zeroing (sim.py:176-177)
+ *   PF_PH_GRAVITY       vel += g_a*dt, then re-zero (sim.py:270-285)
+ *   PF_PH_RHS           rhs from the final vel (build_rhs, kernels.py:338-363)
+ * `solid` may be NULL (no obstacles). */
+#define PF_PH_VEL_FROM_MOM 1
+#define PF_PH_PHASE 2
+#define PF_PH_ZERO 4
+#define PF_PH_STAR 8
+#define PF_PH_GRAVITY 16
+#define PF_PH_RHS 32
+
+typedef struct pf_phase_args {
+    int32_t nx, ny, nz, flags;
+    const float* mass[3];
+    const float* mom[3];
+    float* vel[3];
+    float* vel_star[3];
+    int32_t* valid[3];      /* may be NULL */
+    const float* cell_mass;
+    const float* cell_wsum; /* read only when k_div > 0 */
+    float* phi_cell;
+    int32_t* is_liquid;
+    float* phi_face[3];
+    float* beta[3];
+    const int32_t* solid; /* may be NULL */
+    float* rhs;
+    float mass_eps, rho_min, alpha_phi, rho_0l, rho_l, rho_g;
+    float gravity[3];
+    float dt, dx, rho_tilde_0, k_div;
+} pf_phase_args;
+
+int pf_phase_beta_forces(const pf_phase_args* args, pf_stream_t stream);
+
+/* ---------------------------------------------------------------- MG-PCG
+ * Replaces MGPCG (pfflip/poisson.py:226-409).  Levels halve while
+ * min(n)//2 >= min_dim and all dims are even (poisson.py:256-263).
+ * precision: 0 = FP32 V-cycle (the reference's arithmetic), 1 = BF16 face
+ * coefficients and smoother iterates with FP32 accumulation, 2 = FP16.  CG
+ * vectors are FP32; CG dot products, alpha/beta, the stopping test and the
+ * null-space (mean) projections accumulate in FP64 in every mode. */
+typedef struct pf_mg_level {
+    int32_t nx, ny, nz, _pad;
+    float* bx;
+    float* by;
+    float* bz;
+    float* diag;
+    float* p;
+    float* b;
+    float* r;
+    /* reduced-precision copies, used when precision != 0 on the levels below
+     * small_from; level 0's p_h (the preconditioned vector) always */
+    uint16_t* bx_h;
+    uint16_t* by_h;
+    uint16_t* bz_h;
+    uint16_t* p_h;
+} pf_mg_level;
+
+typedef struct pf_mg {
+    int32_t nlev, pre, post, coarsest;
+    float omega;
+    int32_t precision;
+    int32_t small_from; /* out: first level solved inside one workgroup (set by pf_mg_setup) */
+    int32_t _pad;
+    pf_mg_level lev[PF_MAX_LEVELS];
+    float* rhs; /* CG work vectors, level-0 sized */
+    float* res;
+    float* z;
+    float* q;
+    float* Ap;
+    void* scalars; /* pf_mg_scalar_bytes() of device memory */
+} pf_mg;
+
+size_t pf_mg_scalar_bytes(void);
+/* Validate the level table and choose small_from. */
+int pf_mg_setup(pf_mg* mg);
+/* build_hierarchy (poisson.py:295-309): copy beta into level 0 unless the
+ * pointers already alias it, diag per level, Galerkin coarse faces = sum of
+ * the 4 fine faces (poisson.py:130-163), reduced-precision copies. */
+int pf_mg_build(const pf_mg* mg, const float* bx, const float* by, const float* bz,
+                pf_stream_t stream);
+/* precondition (poisson.py:345-352): z = V-cycle(r) minus its mean. */
+int pf_mg_precondition(const pf_mg* mg, const float* r, float* z, pf_stream_t stream);
+/* solve (poisson.py:355-409): flexible PCG, Polak-Ribiere beta clamped at 0,
+ * stop when max|r|/max|b| < tol (recursive residual, b mean-removed).
+ * Writes x (mean removed); *it_out / *rel_out are host pointers.  Blocks. */
+int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol, int32_t max_iter,
+                   int32_t* it_out, double* rel_out, pf_stream_t stream);
+/* apply_A (poisson.py:54-88): out = diag*p - sum_f beta_f p_nbr with
+ * neighbour indices clamped at the walls.  diag may be NULL (then it is
+ * summed from the six faces in compute_diag's order, poisson.py:40-51). */
+int pf_apply_A(int nx, int ny, int nz, const float* p, const float* bx, const float* by,
+               const float* bz, const float* diag, float* out, pf_stream_t stream);
+
+/* ---------------------------------------------------------------- projection
+ * Replaces _project + _max_divergence (pfflip/sim.py:301-314,
+ * kernels.py:366-400): on interior faces with beta != 0,
+ * vel -= dt*beta*(p_i - p_{i-1})/dx.  If dvel[a] and vel_star[a] are
+ * non-NULL, dvel = vel - vel_star (field_difference, kernels.py:499-502).
+ * If max_div (device float) is non-NULL it receives max |div(vel)/dx| of the
+ * projected field.  Pass dt = 0 to only measure the divergence. */
+int pf_project(const float* p, float* const beta[3], float* const vel[3],
+               float* const vel_star[3], float* const dvel[3],
+               int nx, int ny, int nz, float dx, float dt, float* max_div,
+               pf_stream_t stream);
+
+/* ---------------------------------------------------------------- extrapolation
+ * Replaces _extrapolate_liquid_velocity (pfflip/sim.py:329-341,
+ * kernels.py:406-462): seed faces touching a liquid cell, then `sweeps`
+ * Jacobi sweeps of the mean of known face-index neighbours.  Result in
+ * vel_liq; tmp (3 face arrays) and known/known2 (3+3 uint8 face arrays) are
+ * scratch. */
+int pf_extrapolate(float* const vel[3], const int32_t* is_liquid, float* const vel_liq[3],
+                   float* const tmp[3], uint8_t* const known[3], uint8_t* const known2[3],
+                   int nx, int ny, int nz, int sweeps, pf_stream_t stream);
+
+/* ---------------------------------------------------------------- G2P / advection
+ * Replaces _g2p (sim.py:316-327, kernels.py:468-502) and _advect
+ * (sim.py:408-424, kernels.py:508-549, 567-595, 630-679).  flags:
+ *   PF_GA_G2P     v = a(v + I(dvel)) + (1-a) I(vel), a per phase (eq. 12)
+ *   PF_GA_ADVECT  Ralston RK3 through vel_liq (liquid) / vel (air), clamp
+ *   PF_GA_REFLECT clamp into [lo,hi] and kill the inward component
+ *   PF_GA_SOLIDS  push particles out of solid cells (needs `solid`)
+ * Per-particle phase comes from pflag (pf_bin_sort) when non-NULL, else from
+ * ptype/pstate. */
+#define PF_GA_G2P 1
+#define PF_GA_ADVECT 2
+#define PF_GA_REFLECT 4
+#define PF_GA_SOLIDS 8
+
+typedef struct pf_g2p_args {
+    float* px;
+    float* pv;
+    const int32_t* ptype;
+    const int32_t* pstate;
+    const uint8_t* pflag;
+    int64_t n;
+    int32_t nx, ny, nz, flags;
+    const float* vel[3];
+    const float* dvel[3];
+    const float* vel_liq[3];
+    const int32_t* solid;
+    float dx, dt, alpha_liquid, alpha_air;
+    float lo[3], hi[3];
+} pf_g2p_args;
+
+int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream);
+
+/* particle_max_speed (kernels.py:552-564): *out = max |v| over pstate == 0
+ * (device float, overwritten). */
+int pf_max_speed(const float* pv, const int32_t* pstate, int64_t n, float* out,
+                 pf_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFFLIP_H */

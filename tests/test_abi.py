@@ -1,0 +1,105 @@
+"""CPU checks of the C ABI: the library loads without a GPU, exports every
+entry point ``include/pfflip.h`` declares, and the ctypes structure layouts
+match the C compiler's."""
+
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT, load_pkg
+
+HEADER = os.path.join(ROOT, "include", "pfflip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pf_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    load_pkg()
+    from importlib import import_module
+    L = import_module("implementation-of-adaptive-phase-field-flip_amd._lib")
+    lib = L.load()
+    names = declared_functions()
+    assert len(names) >= 19
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+    # every declared symbol has a ctypes signature, and vice versa
+    assert sorted(L.SIGNATURES) == names
+    assert lib.pf_abi_version() == 1
+    assert lib.pf_bin_nkeys(16, 9, 8) == 2 * 2 * 1 * 512
+
+
+def test_error_path_without_gpu():
+    """Argument validation runs before any device call and reports through
+    pf_last_error (no exception crosses the ABI)."""
+    load_pkg()
+    from importlib import import_module
+    L = import_module("implementation-of-adaptive-phase-field-flip_amd._lib")
+    lib = L.load()
+    rc = lib.pf_p2g(None, None, None, None, None, 8, 8, 8, 0.125, 0.25,
+                    None, None, None, None, None, None, None, None, None, 0, None)
+    assert rc == -1
+    assert b"r_p == dx" in lib.pf_last_error()
+    rc = lib.pf_apply_A(0, 4, 4, None, None, None, None, None, None, None)
+    assert rc == -1 and b"bad grid" in lib.pf_last_error()
+
+
+C_LAYOUT = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "pfflip.h"
+#define P(s, f) printf(#s "." #f " %zu\n", offsetof(s, f))
+int main(void) {
+  printf("pf_phase_args %zu\n", sizeof(pf_phase_args));
+  printf("pf_mg_level %zu\n", sizeof(pf_mg_level));
+  printf("pf_mg %zu\n", sizeof(pf_mg));
+  printf("pf_g2p_args %zu\n", sizeof(pf_g2p_args));
+  P(pf_phase_args, rhs); P(pf_phase_args, gravity); P(pf_phase_args, k_div);
+  P(pf_mg, lev); P(pf_mg, rhs); P(pf_mg, scalars);
+  P(pf_mg_level, p_h);
+  P(pf_g2p_args, n); P(pf_g2p_args, solid); P(pf_g2p_args, hi);
+  return 0;
+}
+"""
+
+
+def test_struct_layouts_match_c(tmp_path):
+    load_pkg()
+    from importlib import import_module
+    import ctypes
+    L = import_module("implementation-of-adaptive-phase-field-flip_amd._lib")
+    src = tmp_path / "layout.c"
+    src.write_text(C_LAYOUT)
+    exe = tmp_path / "layout"
+    try:
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       check=True, capture_output=True)
+    except (OSError, subprocess.CalledProcessError) as e:  # pragma: no cover
+        pytest.skip(f"gcc unavailable: {e}")
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run(
+        [str(exe)], check=True, capture_output=True, text=True).stdout.strip().splitlines())
+    py = {
+        "pf_phase_args": ctypes.sizeof(L.PhaseArgs), "pf_mg_level": ctypes.sizeof(L.MGLevel),
+        "pf_mg": ctypes.sizeof(L.MG), "pf_g2p_args": ctypes.sizeof(L.G2PArgs),
+        "pf_phase_args.rhs": L.PhaseArgs.rhs.offset, "pf_phase_args.gravity": L.PhaseArgs.gravity.offset,
+        "pf_phase_args.k_div": L.PhaseArgs.k_div.offset, "pf_mg.lev": L.MG.lev.offset,
+        "pf_mg.rhs": L.MG.rhs.offset, "pf_mg.scalars": L.MG.scalars.offset,
+        "pf_mg_level.p_h": L.MGLevel.p_h.offset, "pf_g2p_args.n": L.G2PArgs.n.offset,
+        "pf_g2p_args.solid": L.G2PArgs.solid.offset, "pf_g2p_args.hi": L.G2PArgs.hi.offset,
+    }
+    assert {k: int(v) for k, v in out.items()} == py
+
+
+def test_product_has_no_oracle_dependency():
+    """The shipped package never imports the oracle (it is test-only)."""
+    pkg = os.path.join(ROOT, "implementation-of-adaptive-phase-field-flip_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in text.replace("the oracle is", ""), f
