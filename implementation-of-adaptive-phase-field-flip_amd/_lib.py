@@ -82,11 +82,11 @@ SIGNATURES = {
     "pf_read_scalars": (ctypes.c_int, [P, P, SZ, P]),
     "pf_bin_nkeys": (I64, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pf_bin_scan_tmp_bytes": (SZ, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
-    "pf_bin_sort": (ctypes.c_int, [P, P, P, I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32,
+    "pf_bin_sort": (ctypes.c_int, [P, P, I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32,
                                    P, P, P, P, P, P, P, P]),
     "pf_p2g_stage_bytes": (SZ, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pf_p2g": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32, F32,
-                              P, P, P, P, P, P, P, P, P, SZ, P]),
+                              P, P, P, P, P, P, P, P, P, P, SZ, P]),
     "pf_phase_beta_forces": (ctypes.c_int, [ctypes.POINTER(PhaseArgs), P]),
     "pf_mg_scalar_bytes": (SZ, []),
     "pf_mg_setup": (ctypes.c_int, [ctypes.POINTER(MG)]),
@@ -100,7 +100,7 @@ SIGNATURES = {
     "pf_extrapolate": (ctypes.c_int, [PP, P, PP, PP, PP, PP, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, P]),
     "pf_g2p_advect": (ctypes.c_int, [ctypes.POINTER(G2PArgs), P]),
-    "pf_max_speed": (ctypes.c_int, [P, P, I64, P, P]),
+    "pf_max_speed": (ctypes.c_int, [P, P, P, I64, P, P]),
 }
 
 _LIB = None

@@ -9,6 +9,9 @@
 // device; dot products are deterministic FP64 block reductions, and the
 // host only polls a "done" flag one iteration behind the GPU.
 #include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
 
 #include "pf_common.hpp"
 
@@ -20,7 +23,7 @@ constexpr int MAXB = 1024;  // blocks of a reduction launch
 constexpr int RT = 256;     // threads of element-wise / reduction kernels
 
 struct CGState {
-    double rz, alpha, beta, mu, b_norm, rel, qAq, mu_x;
+    double rz, alpha, beta, mu, b_norm, rel, qAq, mu_x, tol;
     int it, done, pad0, pad1;
     unsigned ticket[8];
     double part[6 * MAXB];
@@ -30,6 +33,24 @@ struct CGState {
 static unsigned red_blocks(int64_t n) {
     const int64_t nb = cdiv(n, (int64_t)RT * 4);
     return (unsigned)(nb < 1 ? 1 : (nb > MAXB ? MAXB : nb));
+}
+
+// Combine per-block partials in the last block: every thread sums a fixed
+// strided subset, then a fixed-order block reduction (deterministic).
+template <int NV>
+__device__ inline void combine_sums(const double* part, int nparts, double (&v)[NV], double* sh) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] += part[q * MAXB + b];
+    block_sum<NV>(v, sh);
+}
+
+__device__ inline float combine_max(const float* part, int nparts, float* sh) {
+    float m = 0.0f;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmaxf(m, part[b]);
+    return block_max(m, sh);
 }
 
 // ================================================================ stencil
@@ -154,7 +175,9 @@ __global__ __launch_bounds__(RT) void k_rbgs(Dims d, const ST* __restrict__ bx,
                                              const ST* __restrict__ by, const ST* __restrict__ bz,
                                              ST* p, const float* __restrict__ b, int colour,
                                              float omega, const int* __restrict__ done) {
-    if (done && *done) return;
+    // the flag is only consulted at the stores, so its load does not delay
+    // the stencil loads (a converged solve's extra iteration stores nothing)
+    const bool skip = done && *done;
     const int nkp = (d.nz + 1) >> 1;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)d.nx * d.ny * nkp) return;
@@ -173,7 +196,7 @@ __global__ __launch_bounds__(RT) void k_rbgs(Dims d, const ST* __restrict__ bx,
                                  to_f32(bz[face_index<2>(d, i, j, k)]) + to_f32(bz[face_index<2>(d, i, j, k + 1)]);
                 if (dg > 0.0f) v = omega * b[cell_index(d, i, j, k)] / dg;
             }
-            p[cell_index(d, i, j, k)] = from_f32<ST>(v);
+            if (!skip) p[cell_index(d, i, j, k)] = from_f32<ST>(v);
         }
     } else {
         const int k = 2 * kp + ((i + j + colour) & 1);
@@ -183,7 +206,7 @@ __global__ __launch_bounds__(RT) void k_rbgs(Dims d, const ST* __restrict__ bx,
         if (dg <= 0.0f) return;
         const int c = cell_index(d, i, j, k);
         const float res = b[c] - Ap;
-        p[c] = from_f32<ST>(pc + omega * res / dg);
+        if (!skip) p[c] = from_f32<ST>(pc + omega * res / dg);
     }
 }
 
@@ -196,7 +219,7 @@ __global__ __launch_bounds__(RT) void k_resid_restrict(Dims d, Dims dc, const ST
                                                        const float* __restrict__ b,
                                                        float* __restrict__ bc,
                                                        const int* __restrict__ done) {
-    if (done && *done) return;
+    const bool skip = done && *done;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)dc.nx * dc.ny * dc.nz) return;
     const int K = (int)(t % dc.nz), J = (int)((t / dc.nz) % dc.ny), I = (int)(t / ((int64_t)dc.nz * dc.ny));
@@ -212,19 +235,19 @@ __global__ __launch_bounds__(RT) void k_resid_restrict(Dims d, Dims dc, const ST
                 stencil(d, bx, by, bz, p, i, j, k, dg, Ap, pc);
                 acc += b[cell_index(d, i, j, k)] - Ap;
             }
-    bc[t] = acc;
+    if (!skip) bc[t] = acc;
 }
 
 // P: piecewise-constant injection added onto the fine level (poisson.py:178-182)
 template <typename ST, typename CT>
 __global__ __launch_bounds__(RT) void k_prolong(Dims d, Dims dc, ST* p, const CT* __restrict__ pc,
                                                 const int* __restrict__ done) {
-    if (done && *done) return;
+    const bool skip = done && *done;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)d.nx * d.ny * d.nz) return;
     const int k = (int)(t % d.nz), j = (int)((t / d.nz) % d.ny), i = (int)(t / ((int64_t)d.nz * d.ny));
     const float v = to_f32(p[t]) + to_f32(pc[cell_index(dc, i / 2, j / 2, k / 2)]);
-    p[t] = from_f32<ST>(v);
+    if (!skip) p[t] = from_f32<ST>(v);
 }
 
 // ================================================================ reductions
@@ -239,9 +262,10 @@ __global__ __launch_bounds__(RT) void k_mean(const T* __restrict__ x, int64_t n,
     block_sum<1>(v, sh);
     if (threadIdx.x == 0) st->part[blockIdx.x] = v[0];
     if (take_last_ticket(&st->ticket[slot])) {
+        double t[1];
+        combine_sums<1>(st->part, gridDim.x, t, sh);
         if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (unsigned q = 0; q < gridDim.x; ++q) s += st->part[q];
+            const double s = t[0];
             *out_mean = s / (double)n;
             if (reset_solve) {
                 st->done = 0;
@@ -281,6 +305,8 @@ struct SmallArgs {
     const int* done;
 };
 
+constexpr int kWaveLevelCells = 512;  // levels this small are smoothed by one wave
+
 struct LdsLevel {
     Dims d;
     float* bx;
@@ -317,6 +343,29 @@ __device__ inline void lds_half_sweep(const LdsLevel& v, int colour, float omega
         v.p[c] = pc + omega * (v.b[c] - Ap) / dg;
     }
     __syncthreads();
+}
+
+// Half-sweep run by wave 0 alone (levels of at most a few hundred cells).
+// LDS operations of one wave execute in issue order, so a wave-level
+// barrier separates the colours without a workgroup barrier.
+__device__ inline void wave_half_sweep(const LdsLevel& v, int colour, float omega) {
+    const Dims& d = v.d;
+    const int nkp = (d.nz + 1) >> 1;
+    const int n = d.nx * d.ny * nkp;
+    for (int t = threadIdx.x; t < n; t += 64) {
+        const int kp = t % nkp, row = t / nkp;
+        const int j = row % d.ny, i = row / d.ny;
+        const int k = 2 * kp + ((i + j + colour) & 1);
+        if (k >= d.nz) continue;
+        float dg, Ap, pc;
+        stencil(d, v.bx, v.by, v.bz, v.p, i, j, k, dg, Ap, pc);
+        if (dg <= 0.0f) continue;
+        const int c = cell_index(d, i, j, k);
+        v.p[c] = pc + omega * (v.b[c] - Ap) / dg;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ inline void lds_smooth(const LdsLevel& v, int sweeps, bool reverse, float omega) {
@@ -375,16 +424,31 @@ __global__ __launch_bounds__(1024) void k_small_vcycle(SmallArgs a) {
         const int n = v.d.nx * v.d.ny * v.d.nz;
         for (int e = tid; e < n; e += nt) v.p[e] = 0.0f;
         __syncthreads();
-        lds_smooth(v, a.coarsest, false, a.omega);
-        __shared__ double sh[16];
-        double s[1] = {0.0};
-        for (int e = tid; e < n; e += nt) s[0] += (double)v.p[e];
-        block_sum<1>(s, sh);
-        __shared__ float shift;
-        if (tid == 0) shift = (float)(-(s[0] / (double)n));
-        __syncthreads();
-        for (int e = tid; e < n; e += nt) v.p[e] = v.p[e] + shift;
-        __syncthreads();
+        if (n <= kWaveLevelCells) {
+            if (tid < 64) {
+                for (int sw = 0; sw < a.coarsest; ++sw) {
+                    wave_half_sweep(v, 0, a.omega);
+                    wave_half_sweep(v, 1, a.omega);
+                }
+                double sum = 0.0;
+                for (int e = tid; e < n; e += 64) sum += (double)v.p[e];
+                sum = wave_sum(sum);
+                const float shift = (float)(-(sum / (double)n));
+                for (int e = tid; e < n; e += 64) v.p[e] = v.p[e] + shift;
+            }
+            __syncthreads();
+        } else {
+            lds_smooth(v, a.coarsest, false, a.omega);
+            __shared__ double sh[16];
+            double s[1] = {0.0};
+            for (int e = tid; e < n; e += nt) s[0] += (double)v.p[e];
+            block_sum<1>(s, sh);
+            __shared__ float shift;
+            if (tid == 0) shift = (float)(-(s[0] / (double)n));
+            __syncthreads();
+            for (int e = tid; e < n; e += nt) v.p[e] = v.p[e] + shift;
+            __syncthreads();
+        }
     }
     // ascend
     for (int l = a.L - 2; l >= a.s; --l) {
@@ -441,10 +505,10 @@ __global__ __launch_bounds__(RT) void k_cg_init(CGK<ZT> a) {
     m = block_max(m, sh);
     if (threadIdx.x == 0) a.st->fpart[blockIdx.x] = m;
     if (take_last_ticket(&a.st->ticket[1])) {
+        const float mm = combine_max(a.st->fpart, gridDim.x, sh);
         if (threadIdx.x == 0) {
-            float mm = 0.0f;
-            for (unsigned q = 0; q < gridDim.x; ++q) mm = fmaxf(mm, a.st->fpart[q]);
             a.st->b_norm = (double)mm;
+            a.st->tol = a.tol;
             if (mm == 0.0f) {
                 a.st->done = 2;
                 a.st->rel = 0.0;
@@ -469,10 +533,9 @@ __global__ __launch_bounds__(RT) void k_cg_first(CGK<ZT> a) {
     if (threadIdx.x == 0)
         for (int q = 0; q < 3; ++q) a.st->part[q * MAXB + blockIdx.x] = v[q];
     if (take_last_ticket(&a.st->ticket[2])) {
+        double s[3];
+        combine_sums<3>(a.st->part, gridDim.x, s, sh);
         if (threadIdx.x == 0) {
-            double s[3] = {0.0, 0.0, 0.0};
-            for (int q = 0; q < 3; ++q)
-                for (unsigned bb = 0; bb < gridDim.x; ++bb) s[q] += a.st->part[q * MAXB + bb];
             const double mu = s[0] / (double)a.n;
             a.st->mu = mu;
             a.st->rz = s[2] - mu * s[1];
@@ -509,9 +572,10 @@ __global__ __launch_bounds__(RT) void k_cg_apply(CGK<ZT> a) {
     block_sum<1>(v, sh);
     if (threadIdx.x == 0) a.st->part[blockIdx.x] = v[0];
     if (take_last_ticket(&a.st->ticket[3])) {
+        double t[1];
+        combine_sums<1>(a.st->part, gridDim.x, t, sh);
         if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (unsigned bb = 0; bb < gridDim.x; ++bb) s += a.st->part[bb];
+            const double s = t[0];
             a.st->qAq = s;
             a.st->it += 1;
             if (s <= 0.0)
@@ -538,12 +602,11 @@ __global__ __launch_bounds__(RT) void k_cg_update(CGK<ZT> a) {
     m = block_max(m, sh);
     if (threadIdx.x == 0) a.st->fpart[blockIdx.x] = m;
     if (take_last_ticket(&a.st->ticket[4])) {
+        const float mm = combine_max(a.st->fpart, gridDim.x, sh);
         if (threadIdx.x == 0) {
-            float mm = 0.0f;
-            for (unsigned bb = 0; bb < gridDim.x; ++bb) mm = fmaxf(mm, a.st->fpart[bb]);
             const double rel = (double)mm / a.st->b_norm;
             a.st->rel = rel;
-            if (rel < a.tol) a.st->done = 1;
+            if (rel < a.st->tol) a.st->done = 1;
         }
     }
 }
@@ -568,10 +631,9 @@ __global__ __launch_bounds__(RT) void k_cg_beta(CGK<ZT> a) {
     if (threadIdx.x == 0)
         for (int q = 0; q < 5; ++q) a.st->part[q * MAXB + blockIdx.x] = v[q];
     if (take_last_ticket(&a.st->ticket[5])) {
+        double s[5];
+        combine_sums<5>(a.st->part, gridDim.x, s, sh);
         if (threadIdx.x == 0) {
-            double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int q = 0; q < 5; ++q)
-                for (unsigned bb = 0; bb < gridDim.x; ++bb) s[q] += a.st->part[q * MAXB + bb];
             const double mu = s[0] / (double)a.n;
             const double rz_new = s[3] - mu * s[1];
             const double apz = s[4] - mu * s[2];
@@ -779,6 +841,122 @@ static int get_poll(HostPoll*& out) {
     return PF_OK;
 }
 
+// Kernels of one FPCG iteration; `flag` (pinned host memory) receives the
+// done flag after the residual test.  Every kernel returns at once when the
+// flag is set, so an iteration enqueued past convergence does nothing.
+template <typename ST>
+static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, int* flag,
+                             unsigned nb, hipStream_t s) {
+    k_cg_apply<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    k_cg_update<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    PF_CHECK_HIP(hipMemcpyAsync(flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
+    int rc = vcycle<ST>(mg, a.r, z, done, s);
+    if (rc) return rc;
+    k_cg_beta<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    k_cg_q<ST><<<nb, RT, 0, s>>>(a, 0);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+// First preconditioner application and search direction (poisson.py:378-380).
+template <typename ST>
+static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, unsigned nb,
+                            hipStream_t s) {
+    int rc = vcycle<ST>(mg, a.r, z, done, s);
+    if (rc) return rc;
+    k_cg_first<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    k_cg_q<ST><<<nb, RT, 0, s>>>(a, 1);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+// HIP graphs of the prologue and of one iteration (two instances that differ
+// only in the pinned flag slot), cached per solver layout.  A solve issues
+// one graph launch per iteration instead of ~50 kernel launches.
+struct SolveGraphs {
+    std::vector<unsigned char> key;
+    hipGraphExec_t pro = nullptr;
+    hipGraphExec_t it[2] = {nullptr, nullptr};
+};
+
+static void destroy_graphs(SolveGraphs& g) {
+    if (g.pro) (void)hipGraphExecDestroy(g.pro);
+    for (auto& e : g.it)
+        if (e) (void)hipGraphExecDestroy(e);
+    g.pro = g.it[0] = g.it[1] = nullptr;
+}
+
+template <typename F>
+static int capture(hipStream_t cs, F&& enqueue, hipGraphExec_t* out) {
+    PF_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue(cs);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return set_error((int)e, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return set_error((int)ei, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    return PF_OK;
+}
+
+template <typename ST>
+static int get_graphs(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, HostPoll* hp,
+                      unsigned nb, SolveGraphs*& out) {
+    static thread_local std::vector<SolveGraphs> cache;
+    int dev = 0;
+    PF_CHECK_HIP(hipGetDevice(&dev));
+    std::vector<unsigned char> key(sizeof(pf_mg) + 6 * sizeof(void*) + 2 * sizeof(int));
+    unsigned char* k = key.data();
+    std::memcpy(k, &mg, sizeof(pf_mg));
+    const void* extra[6] = {a.b, a.x, z, hp->flag, (const void*)(uintptr_t)sizeof(ST), nullptr};
+    std::memcpy(k + sizeof(pf_mg), extra, sizeof(extra));
+    const int ints[2] = {dev, (int)nb};
+    std::memcpy(k + sizeof(pf_mg) + sizeof(extra), ints, sizeof(ints));
+    for (auto& c : cache)
+        if (c.key == key) {
+            out = &c;
+            return PF_OK;
+        }
+    int rc = ensure_small_lds();
+    if (rc) return rc;
+    SolveGraphs g;
+    g.key = key;
+    hipStream_t cs = nullptr;
+    PF_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    rc = capture(cs, [&](hipStream_t t) { return enqueue_prologue<ST>(mg, a, z, done, nb, t); }, &g.pro);
+    for (int q = 0; q < 2 && !rc; ++q)
+        rc = capture(cs, [&](hipStream_t t) { return enqueue_iteration<ST>(mg, a, z, done, &hp->flag[q], nb, t); },
+                     &g.it[q]);
+    (void)hipStreamDestroy(cs);
+    if (rc) {
+        destroy_graphs(g);
+        return rc;
+    }
+    if (cache.size() >= 8) {
+        destroy_graphs(cache.front());
+        cache.erase(cache.begin());
+    }
+    cache.push_back(std::move(g));
+    out = &cache.back();
+    return PF_OK;
+}
+
+static bool use_graphs() {
+    static const bool on = [] {
+        const char* e = std::getenv("PFFLIP_SOLVER_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 template <typename ST>
 static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int max_iter,
                       int* it_out, double* rel_out, hipStream_t s) {
@@ -817,32 +995,29 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
         *rel_out = 0.0;
         return PF_OK;
     }
-    int rc = vcycle<ST>(mg, a.r, z, done, s);
-    if (rc) return rc;
-    k_cg_first<ST><<<nb, RT, 0, s>>>(a);
-    PF_CHECK_LAUNCH();
-    k_cg_q<ST><<<nb, RT, 0, s>>>(a, 1);
-    PF_CHECK_LAUNCH();
-
     HostPoll* hp = nullptr;
-    rc = get_poll(hp);
+    int rc = get_poll(hp);
     if (rc) return rc;
+    SolveGraphs* gr = nullptr;
+    if (use_graphs() && max_iter > 0) {
+        rc = get_graphs<ST>(mg, a, z, done, hp, nb, gr);
+        if (rc) return rc;
+    }
+    if (gr) {
+        PF_CHECK_HIP(hipGraphLaunch(gr->pro, s));
+    } else {
+        rc = enqueue_prologue<ST>(mg, a, z, done, nb, s);
+        if (rc) return rc;
+    }
     for (int it = 1; it <= max_iter; ++it) {
-        k_cg_apply<ST><<<nb, RT, 0, s>>>(a);
-        PF_CHECK_LAUNCH();
-        k_cg_update<ST><<<nb, RT, 0, s>>>(a);
-        PF_CHECK_LAUNCH();
         const int slot = it & 1;
-        PF_CHECK_HIP(hipMemcpyAsync(&hp->flag[slot], done, sizeof(int), hipMemcpyDeviceToHost, s));
-        PF_CHECK_HIP(hipEventRecord(hp->ev[slot], s));
-        if (it < max_iter) {
-            rc = vcycle<ST>(mg, a.r, z, done, s);
+        if (gr) {
+            PF_CHECK_HIP(hipGraphLaunch(gr->it[slot], s));
+        } else {
+            rc = enqueue_iteration<ST>(mg, a, z, done, &hp->flag[slot], nb, s);
             if (rc) return rc;
-            k_cg_beta<ST><<<nb, RT, 0, s>>>(a);
-            PF_CHECK_LAUNCH();
-            k_cg_q<ST><<<nb, RT, 0, s>>>(a, 0);
-            PF_CHECK_LAUNCH();
         }
+        PF_CHECK_HIP(hipEventRecord(hp->ev[slot], s));
         // poll one iteration behind, so the GPU always has queued work
         if (it >= 2) {
             PF_CHECK_HIP(hipEventSynchronize(hp->ev[slot ^ 1]));

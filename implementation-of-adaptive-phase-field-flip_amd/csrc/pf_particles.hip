@@ -28,39 +28,57 @@ static TileGeom tile_geom(int nx, int ny, int nz) {
 static int64_t n_tiles(const TileGeom& g) { return (int64_t)g.ntx * g.nty * g.ntz; }
 
 // ============================================================== binning
+// Particles are visited in `order` (the previous step's sorted permutation,
+// or index order).  Particles move at most about one cell per step, so equal
+// keys arrive in runs of consecutive lanes: each run takes its slots with one
+// atomic of its length, and ranks inside the run follow lane order.  Inactive
+// particles (pstate != 0) go to the extra bucket nkeys, after all cells.
+__device__ inline uint32_t cell_key(float x, float y, float z, float dx, const TileGeom& g) {
+    int i = (int)floorf(x / dx), j = (int)floorf(y / dx), k = (int)floorf(z / dx);
+    i = min(max(i, 0), g.nx - 1);
+    j = min(max(j, 0), g.ny - 1);
+    k = min(max(k, 0), g.nz - 1);
+    const int tile = ((i / T) * g.nty + (j / T)) * g.ntz + (k / T);
+    const int loc = ((i % T) * T + (j % T)) * T + (k % T);
+    return (uint32_t)tile * T3 + (uint32_t)loc;
+}
+
 __global__ __launch_bounds__(256) void k_bin(const float* __restrict__ px,
-                                             const int* __restrict__ ptype,
                                              const int* __restrict__ pstate, int64_t n,
-                                             TileGeom g, float dx, uint32_t* __restrict__ keys,
-                                             uint32_t* __restrict__ rank, int* __restrict__ count,
-                                             uint8_t* __restrict__ pflag, int* n_active) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int active = 0;
-    if (p < n) {
-        const int st = pstate[p];
-        const int ty = ptype[p];
-        pflag[p] = (uint8_t)((ty == 1 ? 1 : 0) | (st == 0 ? 2 : 0) | (st == 2 ? 4 : 0));
-        if (st != 0) {
-            keys[p] = 0xFFFFFFFFu;
+                                             const uint32_t* __restrict__ order, TileGeom g,
+                                             float dx, uint32_t nkeys, uint32_t* __restrict__ keys,
+                                             uint32_t* __restrict__ rank, int* __restrict__ count) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = idx < n;
+    uint32_t bucket = 0xFFFFFFFEu;  // lanes past the end form their own run
+    bool active = false;
+    if (valid) {
+        const uint32_t p = order ? order[idx] : (uint32_t)idx;
+        if (pstate[p] == 0) {
+            active = true;
+            bucket = cell_key(px[3 * (int64_t)p], px[3 * (int64_t)p + 1], px[3 * (int64_t)p + 2], dx, g);
         } else {
-            active = 1;
-            const float x = px[3 * p], y = px[3 * p + 1], z = px[3 * p + 2];
-            int i = (int)floorf(x / dx), j = (int)floorf(y / dx), k = (int)floorf(z / dx);
-            i = min(max(i, 0), g.nx - 1);
-            j = min(max(j, 0), g.ny - 1);
-            k = min(max(k, 0), g.nz - 1);
-            const int tile = ((i / T) * g.nty + (j / T)) * g.ntz + (k / T);
-            const int loc = ((i % T) * T + (j % T)) * T + (k % T);
-            const uint32_t key = (uint32_t)tile * T3 + (uint32_t)loc;
-            keys[p] = key;
-            rank[p] = (uint32_t)atomicAdd(&count[key], 1);
+            bucket = nkeys;
         }
+        keys[idx] = active ? bucket : 0xFFFFFFFFu;
     }
-    if (n_active) {
-        // one integer atomic per wave
-        const unsigned long long b = __ballot(active);
-        if ((threadIdx.x & 63) == 0 && b) atomicAdd(n_active, __popcll(b));
-    }
+    // runs of equal buckets among consecutive lanes
+    const uint32_t prev = __shfl_up(bucket, 1, 64);
+    const bool head = lane == 0 || prev != bucket;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1);
+    const int run0 = 63 - __clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const int run1 = after ? (__ffsll((long long)after) - 1) : 64;
+    int base = 0;
+    if (valid && lane == run0) base = atomicAdd(&count[bucket], run1 - run0);
+    base = __shfl(base, run0, 64);
+    if (valid) rank[idx] = (uint32_t)(base + lane - run0);
+}
+
+__global__ void k_copy_count(const int* __restrict__ start, int64_t nkeys, int* n_active) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *n_active = start[nkeys];
 }
 
 // ============================================================== exclusive scan
@@ -168,13 +186,14 @@ static int exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStrea
 // ============================================================== scatter + per-key order
 __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ keys,
                                                  const uint32_t* __restrict__ rank, int64_t n,
+                                                 const uint32_t* __restrict__ order, uint32_t nkeys,
                                                  const int* __restrict__ start,
                                                  uint32_t* __restrict__ perm) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t key = keys[p];
-    if (key == 0xFFFFFFFFu) return;
-    perm[start[key] + rank[p]] = (uint32_t)p;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const uint32_t key = keys[idx];
+    const uint32_t bucket = key == 0xFFFFFFFFu ? nkeys : key;
+    perm[start[bucket] + rank[idx]] = order ? order[idx] : (uint32_t)idx;
 }
 
 // Slots inside one key were handed out by atomics; sorting each key's slice
@@ -236,8 +255,55 @@ struct P2GArgs {
     const int* start;
     TileGeom g;
     float dx, inv_r2;
-    float* stage;
+    unsigned long long* stage;
+    const float* bounds;  // [max|v|, max m, max m|v|_inf]
+    const int* cmax;      // max particles in one key
 };
+
+// Fixed-point accumulation.  gfx950 executes LDS float atomics (ds_add_f32)
+// far slower than integer ones, so every contribution q is added as the
+// 64-bit integer round(q * 2^E) with ds_add_u64.  One exponent per quantity
+// (mass, momentum, weight) for the whole grid, chosen from device-side bounds
+// so that one contribution stays below 2^50 and a face's full sum (at most
+// 27 * cmax contributors) below 2^62: no overflow, resolution 2^-E.  Integer
+// sums do not depend on the order of the adds, so P2G is bit-reproducible.
+struct FixScale {
+    float sm, su, sw;        // 2^E for mass, momentum, weight
+    double im, iu, iw;       // 2^-E
+};
+
+__host__ __device__ inline int fix_exponent(double per_item, double n_items) {
+    // largest E with per_item * 2^E <= 2^50 and n_items * per_item * 2^E <= 2^62
+    if (!(per_item > 0.0) || !(per_item < 1e300)) return 0;
+    int e1, e2;
+    frexp(per_item, &e1);                          // per_item < 2^e1
+    frexp(per_item * fmax(n_items, 1.0), &e2);     // total < 2^e2
+    int e = 50 - e1;
+    if (62 - e2 < e) e = 62 - e2;
+    return e < -100 ? -100 : (e > 100 ? 100 : e);
+}
+
+__device__ inline FixScale fix_scale(const float* bounds, const int* cmax) {
+    const double nsup = 27.0 * (double)max(*cmax, 1);
+    const int em = fix_exponent((double)bounds[1], nsup);
+    const int eu = fix_exponent((double)bounds[2], nsup);
+    const int ew = fix_exponent(1.0, nsup);
+    FixScale f;
+    f.sm = ldexpf(1.0f, em);
+    f.su = ldexpf(1.0f, eu);
+    f.sw = ldexpf(1.0f, ew);
+    f.im = ldexp(1.0, -em);
+    f.iu = ldexp(1.0, -eu);
+    f.iw = ldexp(1.0, -ew);
+    return f;
+}
+
+// round(q * s) as a two's-complement 64-bit integer, |q * s| < 2^51
+// (1.5 * 2^52 aligns the binary point of a double with its last mantissa bit)
+__device__ inline unsigned long long to_fix(float q, float s) {
+    const double d = (double)(q * s) + 6755399441055744.0;
+    return (unsigned long long)(__double_as_longlong(d) - 0x4338000000000000LL);
+}
 
 constexpr int E = T + 2;  // tangential / cell extent of a tile region (1-cell halo)
 constexpr int F = T + 1;  // normal extent of a face region
@@ -287,7 +353,8 @@ __device__ inline AxisCand axis_cand(float x, float dx, int origin) {
 // (0,0,0); SX, SY are the LDS strides of the first / second index.
 template <int SX, int SY, bool MOM>
 __device__ inline void splat8(const float (&qx)[2], const float (&qy)[2], const float (&qz)[2],
-                              int base, float m, float u, float inv_r2, float* sm, float* su) {
+                              int base, float m, float u, float inv_r2, const FixScale& fs,
+                              unsigned long long* sm, unsigned long long* su) {
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -299,8 +366,8 @@ __device__ inline void splat8(const float (&qx)[2], const float (&qy)[2], const 
                 if (w > 0.0f) {
                     const int addr = base + a * SX + b * SY + c;
                     const float wm = w * m;
-                    atomicAdd(&sm[addr], wm);
-                    if (MOM) atomicAdd(&su[addr], wm * u);
+                    atomicAdd(&sm[addr], to_fix(wm, fs.sm));
+                    if (MOM) atomicAdd(&su[addr], to_fix(wm * u, fs.su));
                 }
             }
         }
@@ -325,19 +392,19 @@ __device__ inline PData load_particle(const P2GArgs& a, uint32_t p) {
 
 template <bool WSUM>
 __device__ inline void splat_particle(const PData& d, float dx, float inv_r2, int X0, int Y0, int Z0,
-                                      float* s) {
+                                      const FixScale& fs, unsigned long long* s) {
     const AxisCand cx = axis_cand(d.x, dx, X0), cy = axis_cand(d.y, dx, Y0), cz = axis_cand(d.z, dx, Z0);
     const float nx_[2] = {cx.n0, cx.n1}, tx_[2] = {cx.t0, cx.t1};
     const float ny_[2] = {cy.n0, cy.n1}, ty_[2] = {cy.t0, cy.t1};
     const float nz_[2] = {cz.n0, cz.n1}, tz_[2] = {cz.t0, cz.t1};
     // x-faces: region [F][E][E], origin (X0, Y0-1, Z0-1)
-    splat8<E * E, E, true>(nx_, ty_, tz_, (cx.ln * E + cy.lt) * E + cz.lt, d.m, d.vx, inv_r2,
+    splat8<E * E, E, true>(nx_, ty_, tz_, (cx.ln * E + cy.lt) * E + cz.lt, d.m, d.vx, inv_r2, fs,
                            s + O_XM, s + O_XU);
     // y-faces: region [E][F][E], origin (X0-1, Y0, Z0-1)
-    splat8<F * E, E, true>(tx_, ny_, tz_, (cx.lt * F + cy.ln) * E + cz.lt, d.m, d.vy, inv_r2,
+    splat8<F * E, E, true>(tx_, ny_, tz_, (cx.lt * F + cy.ln) * E + cz.lt, d.m, d.vy, inv_r2, fs,
                            s + O_YM, s + O_YU);
     // z-faces: region [E][E][F], origin (X0-1, Y0-1, Z0)
-    splat8<E * F, F, true>(tx_, ty_, nz_, (cx.lt * E + cy.lt) * F + cz.ln, d.m, d.vz, inv_r2,
+    splat8<E * F, F, true>(tx_, ty_, nz_, (cx.lt * E + cy.lt) * F + cz.ln, d.m, d.vz, inv_r2, fs,
                            s + O_ZM, s + O_ZU);
     // cells: region [E][E][E], origin (X0-1, Y0-1, Z0-1)
     if (WSUM) {
@@ -351,13 +418,13 @@ __device__ inline void splat_particle(const PData& d, float dx, float inv_r2, in
                     const float w = cube_w(pxy + tz_[k], inv_r2);
                     if (w > 0.0f) {
                         const int addr = ((cx.lt + i) * E + (cy.lt + j)) * E + cz.lt + k;
-                        atomicAdd(&s[O_CM + addr], w * d.m);
-                        atomicAdd(&s[O_CW + addr], w);
+                        atomicAdd(&s[O_CM + addr], to_fix(w * d.m, fs.sm));
+                        atomicAdd(&s[O_CW + addr], to_fix(w, fs.sw));
                     }
                 }
             }
     } else {
-        splat8<E * E, E, false>(tx_, ty_, tz_, (cx.lt * E + cy.lt) * E + cz.lt, d.m, 0.f, inv_r2,
+        splat8<E * E, E, false>(tx_, ty_, tz_, (cx.lt * E + cy.lt) * E + cz.lt, d.m, 0.f, inv_r2, fs,
                                 s + O_CM, nullptr);
     }
 }
@@ -365,40 +432,51 @@ __device__ inline void splat_particle(const PData& d, float dx, float inv_r2, in
 template <bool WSUM>
 __global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
     constexpr int R = WSUM ? REG_W : REG;
-    __shared__ float s[R];
+    __shared__ unsigned long long s[R];
 
     const TileGeom g = a.g;
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int tz = tile % g.ntz, ty = (tile / g.ntz) % g.nty, tx = tile / (g.ntz * g.nty);
     const int X0 = tx * T, Y0 = ty * T, Z0 = tz * T;
+    const FixScale fs = fix_scale(a.bounds, a.cmax);
 
-    for (int e = threadIdx.x; e < R; e += blockDim.x) s[e] = 0.f;
+    for (int e = threadIdx.x; e < R; e += blockDim.x) s[e] = 0ull;
     __syncthreads();
 
     const float dx = a.dx, inv_r2 = a.inv_r2;
     const int key0 = tile * T3;
-    // lanes of a wave take different cells of one x-slab, so the LDS atomics
-    // of one instruction mostly hit distinct addresses
+    // lanes of a wave take different cells of one x-slab
     for (int lc = threadIdx.x; lc < T3; lc += blockDim.x) {
         const int s0 = a.start[key0 + lc], e0 = a.start[key0 + lc + 1];
         if (s0 >= e0) continue;
         // software pipeline: the next particle's data and the index after it
         // are in flight while the current particle is splatted
-        uint32_t p_next = a.perm[s0];
         uint32_t p_after = (s0 + 1 < e0) ? a.perm[s0 + 1] : 0u;
-        PData nxt = load_particle(a, p_next);
+        PData nxt = load_particle(a, a.perm[s0]);
         for (int q = s0; q < e0; ++q) {
             const PData cur = nxt;
             if (q + 1 < e0) {
                 nxt = load_particle(a, p_after);
                 if (q + 2 < e0) p_after = a.perm[q + 2];
             }
-            splat_particle<WSUM>(cur, dx, inv_r2, X0, Y0, Z0, s);
+            splat_particle<WSUM>(cur, dx, inv_r2, X0, Y0, Z0, fs, s);
         }
     }
     __syncthreads();
-    float* out = a.stage + (int64_t)tile * R;
+    unsigned long long* out = a.stage + (int64_t)tile * R;
     for (int e = threadIdx.x; e < R; e += blockDim.x) out[e] = s[e];
+}
+
+// max particles in one key (bounds the number of contributors to one sum)
+__global__ __launch_bounds__(256) void k_max_count(const int* __restrict__ start, int64_t nkeys,
+                                                   int* cmax) {
+    int m = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nkeys;
+         k += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, start[k + 1] - start[k]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(cmax, m);
 }
 
 // Tiles whose region contains index i of an axis: normal (face-normal) axes
@@ -431,8 +509,9 @@ __device__ inline int tiles_tangential(int i, int nt, int (&t)[2], int (&l)[2]) 
 }
 
 template <int A>
-__device__ inline void gather_face(const TileGeom& g, const float* __restrict__ stage, int R, int i,
-                                   int j, int k, float* mass, float* mom) {
+__device__ inline void gather_face(const TileGeom& g, const unsigned long long* __restrict__ stage,
+                                   int R, const FixScale& fs, int i, int j, int k, float* mass,
+                                   float* mom) {
     int tX[2], lX[2], tY[2], lY[2], tZ[2], lZ[2];
     const int cX = A == 0 ? tiles_normal(i, g.ntx, tX, lX) : tiles_tangential(i, g.ntx, tX, lX);
     const int cY = A == 1 ? tiles_normal(j, g.nty, tY, lY) : tiles_tangential(j, g.nty, tY, lY);
@@ -441,7 +520,7 @@ __device__ inline void gather_face(const TileGeom& g, const float* __restrict__ 
     constexpr int SY = A == 2 ? F : E;
     constexpr int OM = A == 0 ? O_XM : (A == 1 ? O_YM : O_ZM);
     constexpr int OU = A == 0 ? O_XU : (A == 1 ? O_YU : O_ZU);
-    float am = 0.f, au = 0.f;
+    unsigned long long am = 0ull, au = 0ull;
     for (int a = 0; a < cX; ++a)
         for (int b = 0; b < cY; ++b)
             for (int c = 0; c < cZ; ++c) {
@@ -452,14 +531,16 @@ __device__ inline void gather_face(const TileGeom& g, const float* __restrict__ 
             }
     const Dims d{g.nx, g.ny, g.nz};
     const int f = face_index<A>(d, i, j, k);
-    mass[f] = am;
-    mom[f] = au;
+    mass[f] = (float)((double)(long long)am * fs.im);
+    mom[f] = (float)((double)(long long)au * fs.iu);
 }
 
 struct GatherArgs {
     TileGeom g;
-    const float* stage;
+    const unsigned long long* stage;
     int R;
+    const float* bounds;
+    const int* cmax;
     float* mass[3];
     float* mom[3];
     float* cmass;
@@ -471,17 +552,18 @@ __global__ __launch_bounds__(256) void k_p2g_gather(GatherArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int ez = g.nz + 1, ey = g.ny + 1;
     if (t >= (int64_t)(g.nx + 1) * ey * ez) return;
+    const FixScale fs = fix_scale(a.bounds, a.cmax);
     const int k = (int)(t % ez), j = (int)((t / ez) % ey), i = (int)(t / ((int64_t)ez * ey));
     const bool ci = i < g.nx, cj = j < g.ny, ck = k < g.nz;
-    if (cj && ck) gather_face<0>(g, a.stage, a.R, i, j, k, a.mass[0], a.mom[0]);
-    if (ci && ck) gather_face<1>(g, a.stage, a.R, i, j, k, a.mass[1], a.mom[1]);
-    if (ci && cj) gather_face<2>(g, a.stage, a.R, i, j, k, a.mass[2], a.mom[2]);
+    if (cj && ck) gather_face<0>(g, a.stage, a.R, fs, i, j, k, a.mass[0], a.mom[0]);
+    if (ci && ck) gather_face<1>(g, a.stage, a.R, fs, i, j, k, a.mass[1], a.mom[1]);
+    if (ci && cj) gather_face<2>(g, a.stage, a.R, fs, i, j, k, a.mass[2], a.mom[2]);
     if (ci && cj && ck) {
         int tX[2], lX[2], tY[2], lY[2], tZ[2], lZ[2];
         const int cX = tiles_tangential(i, g.ntx, tX, lX);
         const int cY = tiles_tangential(j, g.nty, tY, lY);
         const int cZ = tiles_tangential(k, g.ntz, tZ, lZ);
-        float am = 0.f, aw = 0.f;
+        unsigned long long am = 0ull, aw = 0ull;
         for (int x = 0; x < cX; ++x)
             for (int y = 0; y < cY; ++y)
                 for (int z = 0; z < cZ; ++z) {
@@ -491,8 +573,8 @@ __global__ __launch_bounds__(256) void k_p2g_gather(GatherArgs a) {
                     if (a.cwsum) aw += a.stage[off + O_CW];
                 }
         const int c = (i * g.ny + j) * g.nz + k;
-        a.cmass[c] = am;
-        if (a.cwsum) a.cwsum[c] = aw;
+        a.cmass[c] = (float)((double)(long long)am * fs.im);
+        if (a.cwsum) a.cwsum[c] = (float)((double)(long long)aw * fs.iw);
     }
 }
 
@@ -700,20 +782,36 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
     a.pv[3 * p + 2] = vz;
 }
 
-// ============================================================== CFL speed
+// ============================================================== CFL speed + splat bounds
+// out[0] = max |v| over FLIP particles (particle_max_speed, kernels.py:552-564);
+// with pmass: out[1] = max m, out[2] = max m * max_a |v_a| (bounds that size
+// the fixed-point P2G accumulators).  Maxima of non-negative floats through
+// integer atomics: order-independent.
 __global__ __launch_bounds__(256) void k_max_speed(const float* __restrict__ pv,
+                                                   const float* __restrict__ pmass,
                                                    const int* __restrict__ pstate, int64_t n,
                                                    float* out) {
     __shared__ float sh[4];
-    float m = 0.f;
+    float m = 0.f, mm = 0.f, mu = 0.f;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += (int64_t)gridDim.x * blockDim.x) {
         if (pstate[p] != 0) continue;
         const float x = pv[3 * p], y = pv[3 * p + 1], z = pv[3 * p + 2];
         m = fmaxf(m, sqrtf(x * x + y * y + z * z));
+        if (pmass) {
+            const float ma = fabsf(pmass[p]);
+            mm = fmaxf(mm, ma);
+            mu = fmaxf(mu, ma * fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))));
+        }
     }
     m = block_max(m, sh);
     if (threadIdx.x == 0) atomic_max_nonneg(out, m);
+    if (pmass) {
+        mm = block_max(mm, sh);
+        if (threadIdx.x == 0) atomic_max_nonneg(out + 1, mm);
+        mu = block_max(mu, sh);
+        if (threadIdx.x == 0) atomic_max_nonneg(out + 2, mu);
+    }
 }
 
 }  // namespace pf
@@ -726,42 +824,44 @@ PF_API int64_t pf_bin_nkeys(int nx, int ny, int nz) {
 }
 
 PF_API size_t pf_bin_scan_tmp_bytes(int nx, int ny, int nz) {
-    const int64_t nk = pf_bin_nkeys(nx, ny, nz);
-    // count array (nkeys ints) + block sums + offsets
-    return (size_t)(nk + 2 * (cdiv(nk + 1, SCAN_TILE) + 2)) * sizeof(int);
+    const int64_t nb = pf_bin_nkeys(nx, ny, nz) + 1;  // cells + the inactive bucket
+    // count array + block sums + offsets
+    return (size_t)(nb + 2 * (cdiv(nb + 1, SCAN_TILE) + 2)) * sizeof(int);
 }
 
-PF_API int pf_bin_sort(const float* px, const int32_t* ptype, const int32_t* pstate, int64_t n,
-                       int nx, int ny, int nz, float dx, uint32_t* keys, uint32_t* rank,
-                       uint32_t* perm, int32_t* cell_start, uint8_t* pflag, void* scan_tmp,
-                       int32_t* n_active, pf_stream_t stream) {
+PF_API int pf_bin_sort(const float* px, const int32_t* pstate, int64_t n, int nx, int ny, int nz,
+                       float dx, const uint32_t* order, uint32_t* keys, uint32_t* rank,
+                       uint32_t* perm, int32_t* cell_start, void* scan_tmp, int32_t* n_active,
+                       pf_stream_t stream) {
     PF_REQUIRE(nx > 0 && ny > 0 && nz > 0 && n >= 0, "pf_bin_sort: bad sizes");
     PF_REQUIRE(n < (int64_t)1 << 31, "pf_bin_sort: more than 2^31 particles");
     PF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 31, "pf_bin_sort: grid too large");
-    PF_REQUIRE(!(n > 0 && (!px || !ptype || !pstate || !keys || !rank || !perm || !pflag)),
+    PF_REQUIRE(!(n > 0 && (!px || !pstate || !keys || !rank || !perm)),
                "pf_bin_sort: null particle buffer");
     PF_REQUIRE(cell_start && scan_tmp, "pf_bin_sort: null cell_start / scan_tmp");
     PF_REQUIRE(dx > 0.f, "pf_bin_sort: dx must be positive");
+    PF_REQUIRE(order != perm || n == 0, "pf_bin_sort: order and perm must be distinct buffers");
     hipStream_t s = S(stream);
     const TileGeom g = tile_geom(nx, ny, nz);
     const int64_t nkeys = n_tiles(g) * T3;
+    const int64_t nbuckets = nkeys + 1;
     int* count = static_cast<int*>(scan_tmp);
-    int* tmp = count + nkeys;
-    int rc = fill_u32(count, nkeys, 0u, s);
+    int* tmp = count + nbuckets;
+    int rc = fill_u32(count, nbuckets, 0u, s);
     if (rc) return rc;
-    if (n_active) {
-        rc = fill_u32(n_active, 1, 0u, s);
-        if (rc) return rc;
-    }
     if (n > 0) {
-        k_bin<<<cdiv(n, 256), 256, 0, s>>>(px, ptype, pstate, n, g, dx, keys, rank, count, pflag,
-                                           n_active);
+        k_bin<<<cdiv(n, 256), 256, 0, s>>>(px, pstate, n, order, g, dx, (uint32_t)nkeys, keys, rank,
+                                           count);
         PF_CHECK_LAUNCH();
     }
-    rc = exclusive_scan(count, nkeys, cell_start, tmp, s);
+    rc = exclusive_scan(count, nbuckets, cell_start, tmp, s);
     if (rc) return rc;
+    if (n_active) {
+        k_copy_count<<<1, 64, 0, s>>>(cell_start, nkeys, n_active);
+        PF_CHECK_LAUNCH();
+    }
     if (n > 0) {
-        k_scatter<<<cdiv(n, 256), 256, 0, s>>>(keys, rank, n, cell_start, perm);
+        k_scatter<<<cdiv(n, 256), 256, 0, s>>>(keys, rank, n, order, (uint32_t)nkeys, cell_start, perm);
         PF_CHECK_LAUNCH();
         k_key_sort<<<(unsigned)(nkeys / SLAB_KEYS), 64, 0, s>>>(cell_start, perm);
         PF_CHECK_LAUNCH();
@@ -771,13 +871,14 @@ PF_API int pf_bin_sort(const float* px, const int32_t* ptype, const int32_t* pst
 
 PF_API size_t pf_p2g_stage_bytes(int nx, int ny, int nz, int with_wsum) {
     const TileGeom g = tile_geom(nx, ny, nz);
-    return (size_t)n_tiles(g) * (with_wsum ? REG_W : REG) * sizeof(float);
+    // 64-bit fixed-point tile regions, then one int (max count per key)
+    return (size_t)n_tiles(g) * (with_wsum ? REG_W : REG) * sizeof(unsigned long long) + 256;
 }
 
 PF_API int pf_p2g(const float* px, const float* pv, const float* pmass, const uint32_t* perm,
                   const int32_t* cell_start, int nx, int ny, int nz, float dx, float rp,
-                  float* mass_x, float* mass_y, float* mass_z, float* mom_x, float* mom_y,
-                  float* mom_z, float* cell_mass, float* cell_wsum, void* stage,
+                  const float* bounds, float* mass_x, float* mass_y, float* mass_z, float* mom_x,
+                  float* mom_y, float* mom_z, float* cell_mass, float* cell_wsum, void* stage,
                   size_t stage_bytes, pf_stream_t stream) {
     PF_REQUIRE(nx > 0 && ny > 0 && nz > 0, "pf_p2g: bad grid");
     PF_REQUIRE((int64_t)(nx + 1) * (ny + 1) * (nz + 1) < (int64_t)1 << 31, "pf_p2g: grid too large");
@@ -788,7 +889,7 @@ PF_API int pf_p2g(const float* px, const float* pv, const float* pmass, const ui
                (double)rp, (double)dx);
     PF_REQUIRE(mass_x && mass_y && mass_z && mom_x && mom_y && mom_z && cell_mass,
                "pf_p2g: null grid output");
-    PF_REQUIRE(perm && cell_start && stage, "pf_p2g: null perm / cell_start / stage");
+    PF_REQUIRE(perm && cell_start && stage && bounds, "pf_p2g: null perm / cell_start / stage / bounds");
     PF_REQUIRE(stage_bytes >= pf_p2g_stage_bytes(nx, ny, nz, cell_wsum != nullptr),
                "pf_p2g: stage buffer too small (%zu < %zu bytes)", stage_bytes,
                pf_p2g_stage_bytes(nx, ny, nz, cell_wsum != nullptr));
@@ -802,8 +903,18 @@ PF_API int pf_p2g(const float* px, const float* pv, const float* pmass, const ui
     a.g = tile_geom(nx, ny, nz);
     a.dx = dx;
     a.inv_r2 = 1.0f / (rp * rp);
-    a.stage = static_cast<float*>(stage);
+    a.stage = static_cast<unsigned long long*>(stage);
+    a.bounds = bounds;
     const int64_t nt = n_tiles(a.g);
+    const int R = cell_wsum ? REG_W : REG;
+    int* cmax = reinterpret_cast<int*>(a.stage + nt * R);
+    a.cmax = cmax;
+    int rc = fill_u32(cmax, 1, 0u, s);
+    if (rc) return rc;
+    const int64_t nkeys = nt * T3;
+    const int64_t nbk = cdiv(nkeys, 256 * 8);
+    k_max_count<<<(unsigned)(nbk < 2048 ? nbk : 2048), 256, 0, s>>>(cell_start, nkeys, cmax);
+    PF_CHECK_LAUNCH();
     if (cell_wsum)
         k_p2g<true><<<(unsigned)nt, 256, 0, s>>>(a);
     else
@@ -812,7 +923,9 @@ PF_API int pf_p2g(const float* px, const float* pv, const float* pmass, const ui
     GatherArgs ga;
     ga.g = a.g;
     ga.stage = a.stage;
-    ga.R = cell_wsum ? REG_W : REG;
+    ga.R = R;
+    ga.bounds = bounds;
+    ga.cmax = cmax;
     ga.mass[0] = mass_x;
     ga.mass[1] = mass_y;
     ga.mass[2] = mass_z;
@@ -869,16 +982,16 @@ PF_API int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream) {
     return PF_OK;
 }
 
-PF_API int pf_max_speed(const float* pv, const int32_t* pstate, int64_t n, float* out,
-                        pf_stream_t stream) {
+PF_API int pf_max_speed(const float* pv, const float* pmass, const int32_t* pstate, int64_t n,
+                        float* out, pf_stream_t stream) {
     PF_REQUIRE(out, "pf_max_speed: null output");
     PF_REQUIRE(n == 0 || (pv && pstate), "pf_max_speed: null particles");
     hipStream_t s = S(stream);
-    int rc = fill_u32(out, 1, 0u, s);
+    int rc = fill_u32(out, pmass ? 3 : 1, 0u, s);
     if (rc) return rc;
     if (n == 0) return PF_OK;
     const int64_t nb = cdiv(n, 256 * 8);
-    k_max_speed<<<(unsigned)(nb < 4096 ? nb : 4096), 256, 0, s>>>(pv, pstate, n, out);
+    k_max_speed<<<(unsigned)(nb < 4096 ? nb : 4096), 256, 0, s>>>(pv, pmass, pstate, n, out);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }

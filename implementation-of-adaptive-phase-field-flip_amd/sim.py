@@ -115,12 +115,14 @@ class Simulator:
 
         lib = _lib.load()
         self._nkeys = int(lib.pf_bin_nkeys(nx, ny, nz))
-        self.cell_start = torch.zeros(self._nkeys + 1, dtype=torch.int32, device=dev)
+        self.cell_start = torch.zeros(self._nkeys + 2, dtype=torch.int32, device=dev)
         nb = int(lib.pf_bin_scan_tmp_bytes(nx, ny, nz))
         self._scan_tmp = torch.zeros((nb + 3) // 4, dtype=torch.int32, device=dev)
         self._n_active = zi((1,))
         sb = int(lib.pf_p2g_stage_bytes(nx, ny, nz, 1 if cfg.k_div > 0.0 else 0))
-        self._p2g_stage = torch.empty((sb + 3) // 4, dtype=torch.float32, device=dev)
+        self._p2g_stage = torch.empty((sb + 7) // 8, dtype=torch.int64, device=dev)
+        # [max|v|, max m, max m|v|_inf] of the FLIP particles (pf_max_speed)
+        self._bounds = z((4,))
 
     # ------------------------------------------------------------------ setup
     def set_solids(self, mask):
@@ -148,11 +150,14 @@ class Simulator:
         self.prad = torch.full((n,), cfg.r_p, dtype=torch.float32, device=dev)
         self.pstate = torch.zeros(n, dtype=torch.int32, device=dev)
         self.pdrop_r = torch.zeros(n, dtype=torch.float32, device=dev)
-        # binning buffers (keys, slot ranks, sorted indices, per-particle flags)
+        # binning buffers: keys and slot ranks in visiting order, and two
+        # permutations (this step's sorted order; the previous one, which is
+        # the visiting order of the next binning)
         self._keys = torch.empty(n, dtype=torch.int32, device=dev)
         self._rank = torch.empty(n, dtype=torch.int32, device=dev)
         self.perm = torch.empty(n, dtype=torch.int32, device=dev)
-        self.pflag = torch.empty(n, dtype=torch.uint8, device=dev)
+        self._perm_alt = torch.empty(n, dtype=torch.int32, device=dev)
+        self._perm_valid = False
 
     # ------------------------------------------------------------------ helpers
     def _read(self, t, count=1):
@@ -200,12 +205,12 @@ class Simulator:
         a = self._phase_args(flags, dt)
         _lib.call("pf_phase_beta_forces", ctypes.byref(a), _lib.stream())
 
-    def _g2p_args(self, flags, dt=0.0, use_pflag=False):
+    def _g2p_args(self, flags, dt=0.0):
         cfg = self.cfg
         a = _lib.G2PArgs()
         a.px, a.pv = self.px.data_ptr(), self.pv.data_ptr()
         a.ptype, a.pstate = self.ptype.data_ptr(), self.pstate.data_ptr()
-        a.pflag = self.pflag.data_ptr() if use_pflag else None
+        a.pflag = None
         a.n = self.n_particles
         a.nx, a.ny, a.nz = self.res
         a.flags = flags | (_lib.GA_SOLIDS if (self.has_solids and flags & _lib.GA_REFLECT) else 0)
@@ -221,13 +226,17 @@ class Simulator:
         return a
 
     def _bin(self):
-        """Cell-sort the FLIP particles (pf_bin_sort)."""
+        """Cell-sort the FLIP particles (pf_bin_sort), visiting them in the
+        previous step's sorted order."""
         nx, ny, nz = self.res
-        _lib.call("pf_bin_sort", _lib.ptr(self.px), _lib.ptr(self.ptype), _lib.ptr(self.pstate),
-                  self.n_particles, nx, ny, nz, self.cfg.dx, _lib.ptr(self._keys),
-                  _lib.ptr(self._rank), _lib.ptr(self.perm), _lib.ptr(self.cell_start),
-                  _lib.ptr(self.pflag), _lib.ptr(self._scan_tmp), _lib.ptr(self._n_active),
-                  _lib.stream())
+        order = self.perm if self._perm_valid else None
+        out = self._perm_alt
+        _lib.call("pf_bin_sort", _lib.ptr(self.px), _lib.ptr(self.pstate), self.n_particles,
+                  nx, ny, nz, self.cfg.dx, _lib.ptr(order), _lib.ptr(self._keys),
+                  _lib.ptr(self._rank), _lib.ptr(out), _lib.ptr(self.cell_start),
+                  _lib.ptr(self._scan_tmp), _lib.ptr(self._n_active), _lib.stream())
+        self._perm_alt, self.perm = self.perm, out
+        self._perm_valid = True
 
     def _splat(self):
         nx, ny, nz = self.res
@@ -236,9 +245,14 @@ class Simulator:
         wsum = self.cell_wsum if self.cfg.k_div > 0.0 else None
         _lib.call("pf_p2g", _lib.ptr(self.px), _lib.ptr(self.pv), _lib.ptr(self.pmass),
                   _lib.ptr(self.perm), _lib.ptr(self.cell_start), nx, ny, nz, self.cfg.dx,
-                  self.cfg.r_p, *[_lib.ptr(t) for t in self.mass], *[_lib.ptr(t) for t in self.mom],
-                  _lib.ptr(self.cell_mass), _lib.ptr(wsum), _lib.ptr(self._p2g_stage),
-                  4 * self._p2g_stage.numel(), _lib.stream())
+                  self.cfg.r_p, _lib.ptr(self._bounds), *[_lib.ptr(t) for t in self.mass],
+                  *[_lib.ptr(t) for t in self.mom], _lib.ptr(self.cell_mass), _lib.ptr(wsum),
+                  _lib.ptr(self._p2g_stage), 8 * self._p2g_stage.numel(), _lib.stream())
+
+    def _particle_bounds(self):
+        """max |v| (CFL) and the P2G fixed-point bounds, on the device."""
+        _lib.call("pf_max_speed", _lib.ptr(self.pv), _lib.ptr(self.pmass), _lib.ptr(self.pstate),
+                  self.n_particles, _lib.ptr(self._bounds), _lib.stream())
 
     def _mark(self, name):
         if self.stage_ms is not None:
@@ -257,9 +271,8 @@ class Simulator:
     def dt_from_cfl(self) -> float:
         """pfflip/sim.py:158-166 (speed over pstate == 0 particles only)."""
         cfg = self.cfg
-        _lib.call("pf_max_speed", _lib.ptr(self.pv), _lib.ptr(self.pstate), self.n_particles,
-                  _lib.ptr(self._scalar), _lib.stream())
-        vmax = float(self._read(self._scalar)[0])
+        self._particle_bounds()
+        vmax = float(self._read(self._bounds)[0])
         g = float(np.linalg.norm(cfg.gravity))
         vmax = max(vmax, math.sqrt(2.0 * g * cfg.dx) if g > 0 else 0.0, 1e-8)
         return float(np.clip(cfg.cfl * cfg.dx / vmax, cfg.dt_min, cfg.dt_max))
@@ -272,7 +285,9 @@ class Simulator:
                           "implemented and is skipped", RuntimeWarning, stacklevel=2)
             self._warned_droplets = True
         if dt is None:
-            dt = self.dt_from_cfl()
+            dt = self.dt_from_cfl()     # also leaves the P2G bounds on the device
+        else:
+            self._particle_bounds()
         t0 = time.perf_counter()
         self._mark("start")
         self._bin()
@@ -294,7 +309,7 @@ class Simulator:
         self._mark("project")
         self._extrapolate_liquid_velocity()
         self._mark("extrapolate")
-        a = self._g2p_args(_lib.GA_G2P | _lib.GA_ADVECT | _lib.GA_REFLECT, dt, use_pflag=True)
+        a = self._g2p_args(_lib.GA_G2P | _lib.GA_ADVECT | _lib.GA_REFLECT, dt)
         _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
         self._mark("g2p_advect")
         div_after = float(self._read(self._scalar)[0])   # synchronises
@@ -310,6 +325,7 @@ class Simulator:
     # ------------------------------------------------------------------ stages
     def _p2g(self):
         """P2G + face velocities (pfflip/sim.py:205-236)."""
+        self._particle_bounds()
         self._bin()
         self._splat()
         self._phase(_lib.PH_VEL_FROM_MOM)

@@ -58,33 +58,40 @@ int pf_read_scalars(const void* dev_src, void* host_dst, size_t bytes, pf_stream
  *   cell = clamp(floor(x / dx), 0, n-1) per axis (FP32 division),
  *   key  = tile_id * PF_TILE^3 + local_id,
  *   tile_id = (tx*nty + ty)*ntz + tz, local_id = (li*T + lj)*T + lk.
- * Other particles get key 0xFFFFFFFF and are left out of `perm`.
- * `perm` lists particle indices sorted by key, ties by index (a unique
- * permutation); cell_start[key] .. cell_start[key+1] is a key's range.
- * pflag[p] = (ptype==1) | (pstate==0)<<1 | (pstate==2)<<2.
- * Buffers: keys, rank, perm, pflag: n entries; cell_start: nkeys+1 with
- * nkeys = pf_bin_nkeys(); scan_tmp: pf_bin_scan_tmp_bytes() bytes.
- * *n_active (device int32, may be NULL) receives the number of FLIP
- * particles. */
+ * Particles are visited in `order` (a permutation of 0..n-1, normally the
+ * previous step's `perm`; NULL = index order); keys[i] and rank[i] belong
+ * to the i-th visited particle (key 0xFFFFFFFF = not a FLIP particle).
+ * On return `perm` is a permutation of all n particles: the FLIP particles
+ * sorted by key with ties by particle index (a unique order), then the
+ * others.  cell_start (nkeys + 2 entries, nkeys = pf_bin_nkeys()) gives
+ * key k the range cell_start[k] .. cell_start[k+1]; cell_start[nkeys] is
+ * the number of FLIP particles and cell_start[nkeys+1] = n.
+ * Buffers: keys, rank, perm: n entries; scan_tmp: pf_bin_scan_tmp_bytes().
+ * *n_active (device int32, may be NULL) also receives the FLIP count. */
 int64_t pf_bin_nkeys(int nx, int ny, int nz);
 size_t pf_bin_scan_tmp_bytes(int nx, int ny, int nz);
-int pf_bin_sort(const float* px, const int32_t* ptype, const int32_t* pstate, int64_t n,
-                int nx, int ny, int nz, float dx,
-                uint32_t* keys, uint32_t* rank, uint32_t* perm, int32_t* cell_start,
-                uint8_t* pflag, void* scan_tmp, int32_t* n_active, pf_stream_t stream);
+int pf_bin_sort(const float* px, const int32_t* pstate, int64_t n, int nx, int ny, int nz,
+                float dx, const uint32_t* order, uint32_t* keys, uint32_t* rank,
+                uint32_t* perm, int32_t* cell_start, void* scan_tmp, int32_t* n_active,
+                pf_stream_t stream);
 
 /* ---------------------------------------------------------------- P2G
  * Replaces Simulator._p2g's zeroing + p2g_face x3 + p2g_cell
  * (pfflip/sim.py:205-228, pfflip/kernels.py:100-209): eq. (6) weights
  * w = max(1 - d^2/r^2, 0)^3 with r = rp (= dx), mass += w*m,
  * mom_a += w*m*v_a, cell_mass += w*m, cell_wsum += w (if non-NULL).
- * Every output element is written (no pre-zeroing needed).  Consumes
- * perm/cell_start from pf_bin_sort.  `stage` is scratch of
- * pf_p2g_stage_bytes(nx, ny, nz, cell_wsum != NULL) bytes. */
+ * Sums are formed in 64-bit fixed point (one power-of-two scale per
+ * quantity, sized from `bounds` so they cannot overflow), so the result does
+ * not depend on summation order; each output is that sum rounded to FP32.
+ * bounds: device floats [max|v|, max m, max m*max_a|v_a|] over the FLIP
+ * particles, as written by pf_max_speed(..., pmass, ...).  Every output
+ * element is written (no pre-zeroing needed).  Consumes perm/cell_start from
+ * pf_bin_sort.  `stage` is scratch of pf_p2g_stage_bytes(nx, ny, nz,
+ * cell_wsum != NULL) bytes. */
 size_t pf_p2g_stage_bytes(int nx, int ny, int nz, int with_wsum);
 int pf_p2g(const float* px, const float* pv, const float* pmass,
            const uint32_t* perm, const int32_t* cell_start,
-           int nx, int ny, int nz, float dx, float rp,
+           int nx, int ny, int nz, float dx, float rp, const float* bounds,
            float* mass_x, float* mass_y, float* mass_z,
            float* mom_x, float* mom_y, float* mom_z,
            float* cell_mass, float* cell_wsum, void* stage, size_t stage_bytes,
@@ -222,8 +229,9 @@ int pf_extrapolate(float* const vel[3], const int32_t* is_liquid, float* const v
  *   PF_GA_ADVECT  Ralston RK3 through vel_liq (liquid) / vel (air), clamp
  *   PF_GA_REFLECT clamp into [lo,hi] and kill the inward component
  *   PF_GA_SOLIDS  push particles out of solid cells (needs `solid`)
- * Per-particle phase comes from pflag (pf_bin_sort) when non-NULL, else from
- * ptype/pstate. */
+ * Per-particle phase comes from ptype/pstate, or from a packed byte per
+ * particle in pflag when non-NULL (bit 0 liquid, bit 1 pstate == 0,
+ * bit 2 pstate == 2). */
 #define PF_GA_G2P 1
 #define PF_GA_ADVECT 2
 #define PF_GA_REFLECT 4
@@ -247,10 +255,11 @@ typedef struct pf_g2p_args {
 
 int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream);
 
-/* particle_max_speed (kernels.py:552-564): *out = max |v| over pstate == 0
- * (device float, overwritten). */
-int pf_max_speed(const float* pv, const int32_t* pstate, int64_t n, float* out,
-                 pf_stream_t stream);
+/* particle_max_speed (kernels.py:552-564): out[0] = max |v| over
+ * pstate == 0 (device floats, overwritten).  With pmass non-NULL also
+ * out[1] = max m and out[2] = max m*max_a|v_a| (the P2G bounds). */
+int pf_max_speed(const float* pv, const float* pmass, const int32_t* pstate, int64_t n,
+                 float* out, pf_stream_t stream);
 
 #ifdef __cplusplus
 }

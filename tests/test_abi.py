@@ -41,7 +41,7 @@ def test_error_path_without_gpu():
     from importlib import import_module
     L = import_module("implementation-of-adaptive-phase-field-flip_amd._lib")
     lib = L.load()
-    rc = lib.pf_p2g(None, None, None, None, None, 8, 8, 8, 0.125, 0.25,
+    rc = lib.pf_p2g(None, None, None, None, None, 8, 8, 8, 0.125, 0.25, None,
                     None, None, None, None, None, None, None, None, None, 0, None)
     assert rc == -1
     assert b"r_p == dx" in lib.pf_last_error()
