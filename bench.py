@@ -7,8 +7,9 @@ relative (inf-norm) residual, droplet stage off.  A step is one full
 ``Simulator.step()``: CFL dt, binning, P2G, eq. 7 / beta / gravity / RHS,
 MG-PCG, projection, extrapolation, G2P + RK3 advection.
 
-Prints one JSON line: particle-steps/s over the timed steps, the P2G roofline
-(algorithmic bytes / measured average kernel time vs 8 TB/s), the FP32 vs
+Prints one JSON line: particle-steps/s over the timed steps, the roofline of
+the dominant kernel (the level-0 smoother half-sweep) and of P2G and G2P
+(algorithmic bytes / measured average time vs 8 TB/s), the FP32 vs
 BF16 MG-PCG time to 1e-6 on the last step's captured system, per-stage
 milliseconds, and the CPU baseline (the FP64 numpy restatement of the
 reference step on a bounded sample, one thread).
@@ -30,9 +31,28 @@ HBM_PEAK_GBS = 8000.0
 
 
 def p2g_bytes(n_particles, n_cells):
-    """Algorithmic bytes of one P2G launch: x, v (12 B each) and a 1-byte
-    phase per particle; face mass x3, momentum x3 and cell mass (4 B each)."""
+    """Algorithmic bytes of one P2G: x, v (12 B each) and a 1-byte phase per
+    particle; face mass x3, momentum x3 and cell mass (4 B each) per cell."""
     return 25 * n_particles + 28 * n_cells
+
+
+def g2p_bytes(n_particles, n_cells):
+    """Algorithmic bytes of the fused G2P + RK3 + reflection pass: read x, v,
+    phase (25 B) and write x, v (24 B) per particle; read vel, dvel and
+    vel_liq (3 x 3 faces x 4 B) per cell."""
+    return 49 * n_particles + 36 * n_cells
+
+
+def smoother_bytes(n_cells, s_p=4, s_beta=4, s_b=4):
+    """Algorithmic bytes of one red-black half-sweep: every p and every face
+    coefficient read once, b read and p written for half the cells."""
+    return n_cells * (s_p + 3 * s_beta + s_b / 2 + s_p / 2)
+
+
+def roofline(name, nbytes, ms):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": name, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": nbytes, "avg_ms": ms}
 
 
 def cpu_baseline(seconds_budget=30.0):
@@ -78,12 +98,20 @@ def solve_compare(pf, torch, sim, cfg, tol):
             it, rel = s.solve(rhs, p, tol=tol, max_iter=400)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
-        # true residual of the returned pressure
-        Ap = torch.zeros_like(rhs)
+        # true residual of the returned pressure, in FP64
         L0 = s.levels[0]
-        pf.apply_A(p, L0.bx, L0.by, L0.bz, L0.diag, Ap)
-        b0 = rhs - rhs.double().mean().float()
+        pd = p.double()
+        bx, by, bz = L0.bx.double(), L0.by.double(), L0.bz.double()
+        Ap = L0.diag.double() * pd
+        Ap[1:] -= bx[1:-1] * pd[:-1]
+        Ap[:-1] -= bx[1:-1] * pd[1:]
+        Ap[:, 1:] -= by[:, 1:-1] * pd[:, :-1]
+        Ap[:, :-1] -= by[:, 1:-1] * pd[:, 1:]
+        Ap[:, :, 1:] -= bz[:, :, 1:-1] * pd[:, :, :-1]
+        Ap[:, :, :-1] -= bz[:, :, 1:-1] * pd[:, :, 1:]
+        b0 = rhs.double() - rhs.double().mean()
         true_rel = float((b0 - Ap).abs().max() / b0.abs().max())
+        del pd, bx, by, bz, Ap, b0
         out[prec] = {"ms": 1e3 * min(times), "iters": it, "rel": rel, "true_rel": true_rel}
         del s
     return out
@@ -139,8 +167,15 @@ def main():
     stages = {k: v / args.steps for k, v in sim.stage_ms.items()}
     sim.stage_ms = None
 
-    p2g_ms = stages["p2g"]
-    achieved = p2g_bytes(n_p, n_c) / (p2g_ms * 1e-3) / 1e9
+    # the dominant kernel: the level-0 red-black half-sweep of the V-cycle,
+    # timed with HIP events on the solver's stream right after the timed steps
+    smooth_ms = pf.poisson.time_smoother(sim.solver, 0, reps=10)
+    s16 = 2 if args.precision != "fp32" else 4
+    kernels = [
+        roofline("k_rbgs level-0 half-sweep (128^3)", smoother_bytes(n_c, s16, s16), smooth_ms),
+        roofline("p2g stage (k_max_count + k_p2g + k_p2g_gather)", p2g_bytes(n_p, n_c), stages["p2g"]),
+        roofline("k_g2p_advect (G2P + RK3 + reflect)", g2p_bytes(n_p, n_c), stages["g2p_advect"]),
+    ]
     result = {
         "metric": "PF-FLIP step particle-steps/s at 128^3/27ppc; MG-PCG ms to 1e-6, mixed vs FP32",
         "value": n_p * args.steps / elapsed,
@@ -157,10 +192,8 @@ def main():
         "config": {"workload": f"dam break z-up {n}^3 x {args.ppc} ppc, 1000:1, MG-PCG tol 1e-6",
                    "grid": n, "ppc": args.ppc, "particles": n_p, "pcg_tol": 1e-6,
                    "solver_precision": args.precision, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "p2g (k_p2g + output zeroing)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": p2g_bytes(n_p, n_c), "avg_ms": p2g_ms},
+        "roofline": kernels[0],
+        "roofline_kernels": kernels[1:],
         "pcg_iters": [r["pcg_iter"] for r in recs],
         "stages_ms": stages,
     }

@@ -94,6 +94,7 @@ SIGNATURES = {
     "pf_mg_precondition": (ctypes.c_int, [ctypes.POINTER(MG), P, P, P]),
     "pf_mgpcg_solve": (ctypes.c_int, [ctypes.POINTER(MG), P, P, F64, I32,
                                       ctypes.POINTER(I32), ctypes.POINTER(F64), P]),
+    "pf_mg_time_smoother": (ctypes.c_int, [ctypes.POINTER(MG), I32, I32, ctypes.POINTER(F32), P]),
     "pf_apply_A": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P, P]),
     "pf_project": (ctypes.c_int, [P, PP, PP, PP, PP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   F32, F32, P, P]),

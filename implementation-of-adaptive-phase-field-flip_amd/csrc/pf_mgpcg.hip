@@ -251,29 +251,19 @@ __global__ __launch_bounds__(RT) void k_prolong(Dims d, Dims dc, ST* p, const CT
 }
 
 // ================================================================ reductions
-// sum of a level vector (FP64), combined by the last block into *out_mean
+// Reductions are fence-free: a kernel writes one partial per block, and a
+// one-block combine kernel (k_comb) forms the scalar.  The kernel boundary
+// orders the two, so no device-scope fence (an L2 write-back per block on
+// this multi-XCD part) is needed, and the fixed combine order keeps every
+// result bitwise reproducible.
 template <typename T>
-__global__ __launch_bounds__(RT) void k_mean(const T* __restrict__ x, int64_t n, CGState* st,
-                                             int slot, double* out_mean, int reset_solve) {
+__global__ __launch_bounds__(RT) void k_part_sum(const T* __restrict__ x, int64_t n, double* part) {
     __shared__ double sh[16];
     double v[1] = {0.0};
     for (int64_t c = (int64_t)blockIdx.x * RT + threadIdx.x; c < n; c += (int64_t)gridDim.x * RT)
         v[0] += (double)to_f32(x[c]);
     block_sum<1>(v, sh);
-    if (threadIdx.x == 0) st->part[blockIdx.x] = v[0];
-    if (take_last_ticket(&st->ticket[slot])) {
-        double t[1];
-        combine_sums<1>(st->part, gridDim.x, t, sh);
-        if (threadIdx.x == 0) {
-            const double s = t[0];
-            *out_mean = s / (double)n;
-            if (reset_solve) {
-                st->done = 0;
-                st->it = 0;
-                st->rel = 1.0;
-            }
-        }
-    }
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
 // y = x + shift (FP32 shift of the FP64 mean, as _remove_mean's shift_kernel)
@@ -489,7 +479,7 @@ struct CGK {
     double tol;
 };
 
-// rhs = b - mean(b); r = rhs; x = 0; b_norm = max|rhs| (poisson.py:363-372)
+// rhs = b - mean(b); r = rhs; x = 0; partial max|rhs| (poisson.py:363-372)
 template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_init(CGK<ZT> a) {
     __shared__ float sh[4];
@@ -504,20 +494,9 @@ __global__ __launch_bounds__(RT) void k_cg_init(CGK<ZT> a) {
     }
     m = block_max(m, sh);
     if (threadIdx.x == 0) a.st->fpart[blockIdx.x] = m;
-    if (take_last_ticket(&a.st->ticket[1])) {
-        const float mm = combine_max(a.st->fpart, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            a.st->b_norm = (double)mm;
-            a.st->tol = a.tol;
-            if (mm == 0.0f) {
-                a.st->done = 2;
-                a.st->rel = 0.0;
-            }
-        }
-    }
 }
 
-// after the first V-cycle: mu = mean(z), rz = <r, z - mu>
+// partials of sum(z), sum(r), <r, z> after the first V-cycle
 template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_first(CGK<ZT> a) {
     __shared__ double sh[3 * 16];
@@ -532,15 +511,6 @@ __global__ __launch_bounds__(RT) void k_cg_first(CGK<ZT> a) {
     block_sum<3>(v, sh);
     if (threadIdx.x == 0)
         for (int q = 0; q < 3; ++q) a.st->part[q * MAXB + blockIdx.x] = v[q];
-    if (take_last_ticket(&a.st->ticket[2])) {
-        double s[3];
-        combine_sums<3>(a.st->part, gridDim.x, s, sh);
-        if (threadIdx.x == 0) {
-            const double mu = s[0] / (double)a.n;
-            a.st->mu = mu;
-            a.st->rz = s[2] - mu * s[1];
-        }
-    }
 }
 
 // q = (z - mu) + beta q  (first call: beta = 0)
@@ -555,7 +525,7 @@ __global__ __launch_bounds__(RT) void k_cg_q(CGK<ZT> a, int first) {
     }
 }
 
-// Ap = A q; qAq; alpha = rz / qAq (poisson.py:385-389)
+// Ap = A q; partial <q, Ap> (poisson.py:385-386)
 template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_apply(CGK<ZT> a) {
     __shared__ double sh[16];
@@ -571,22 +541,9 @@ __global__ __launch_bounds__(RT) void k_cg_apply(CGK<ZT> a) {
     }
     block_sum<1>(v, sh);
     if (threadIdx.x == 0) a.st->part[blockIdx.x] = v[0];
-    if (take_last_ticket(&a.st->ticket[3])) {
-        double t[1];
-        combine_sums<1>(a.st->part, gridDim.x, t, sh);
-        if (threadIdx.x == 0) {
-            const double s = t[0];
-            a.st->qAq = s;
-            a.st->it += 1;
-            if (s <= 0.0)
-                a.st->done = 1;
-            else
-                a.st->alpha = a.st->rz / s;
-        }
-    }
 }
 
-// x += alpha q; r -= alpha Ap; rel = max|r| / b_norm (poisson.py:390-398)
+// x += alpha q; r -= alpha Ap; partial max|r| (poisson.py:390-394)
 template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_update(CGK<ZT> a) {
     __shared__ float sh[4];
@@ -601,19 +558,9 @@ __global__ __launch_bounds__(RT) void k_cg_update(CGK<ZT> a) {
     }
     m = block_max(m, sh);
     if (threadIdx.x == 0) a.st->fpart[blockIdx.x] = m;
-    if (take_last_ticket(&a.st->ticket[4])) {
-        const float mm = combine_max(a.st->fpart, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            const double rel = (double)mm / a.st->b_norm;
-            a.st->rel = rel;
-            if (rel < a.st->tol) a.st->done = 1;
-        }
-    }
 }
 
-// Polak-Ribiere with the flexible numerator (poisson.py:400-406):
-// z' = z - mean(z); rz_new = <r, z'>; num = rz_new - <r_prev, z'>
-// = -alpha <Ap, z'> because r_prev = r + alpha Ap.
+// partials of sum(z), sum(r), sum(Ap), <r,z>, <Ap,z> after a V-cycle
 template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_beta(CGK<ZT> a) {
     __shared__ double sh[5 * 16];
@@ -630,18 +577,86 @@ __global__ __launch_bounds__(RT) void k_cg_beta(CGK<ZT> a) {
     block_sum<5>(v, sh);
     if (threadIdx.x == 0)
         for (int q = 0; q < 5; ++q) a.st->part[q * MAXB + blockIdx.x] = v[q];
-    if (take_last_ticket(&a.st->ticket[5])) {
-        double s[5];
-        combine_sums<5>(a.st->part, gridDim.x, s, sh);
+}
+
+// One-block combine of the previous kernel's partials, then the scalar step.
+enum CombOp { C_MEAN = 0, C_MEAN_RESET, C_INIT, C_FIRST, C_APPLY, C_UPDATE, C_BETA };
+
+__global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, int64_t n,
+                                             double* out, double tol) {
+    __shared__ double sh[5 * 16];
+    __shared__ float shf[4];
+    if (op == C_MEAN || op == C_MEAN_RESET) {
+        double t[1];
+        combine_sums<1>(st->part, nparts, t, sh);
         if (threadIdx.x == 0) {
-            const double mu = s[0] / (double)a.n;
-            const double rz_new = s[3] - mu * s[1];
-            const double apz = s[4] - mu * s[2];
-            const double num = -a.st->alpha * apz;
-            const double rz = a.st->rz;
-            a.st->beta = (rz != 0.0) ? fmax(num / rz, 0.0) : 0.0;
-            a.st->mu = mu;
-            a.st->rz = rz_new;
+            *out = t[0] / (double)n;
+            if (op == C_MEAN_RESET) {
+                st->done = 0;
+                st->it = 0;
+                st->rel = 1.0;
+            }
+        }
+        return;
+    }
+    if (op == C_INIT) {
+        // b_norm = max|rhs|; b == 0 returns x = 0 (poisson.py:365-368)
+        const float mm = combine_max(st->fpart, nparts, shf);
+        if (threadIdx.x == 0) {
+            st->b_norm = (double)mm;
+            st->tol = tol;
+            if (mm == 0.0f) {
+                st->done = 2;
+                st->rel = 0.0;
+            }
+        }
+        return;
+    }
+    if (st->done) return;
+    if (op == C_FIRST) {
+        // mu = mean(z); rz = <r, z - mu>
+        double t[3];
+        combine_sums<3>(st->part, nparts, t, sh);
+        if (threadIdx.x == 0) {
+            const double mu = t[0] / (double)n;
+            st->mu = mu;
+            st->rz = t[2] - mu * t[1];
+        }
+    } else if (op == C_APPLY) {
+        // qAq <= 0 ends the solve before any update (poisson.py:386-389)
+        double t[1];
+        combine_sums<1>(st->part, nparts, t, sh);
+        if (threadIdx.x == 0) {
+            st->qAq = t[0];
+            st->it += 1;
+            if (t[0] <= 0.0)
+                st->done = 1;
+            else
+                st->alpha = st->rz / t[0];
+        }
+    } else if (op == C_UPDATE) {
+        // rel = max|r| / max|b| (poisson.py:394-398)
+        const float mm = combine_max(st->fpart, nparts, shf);
+        if (threadIdx.x == 0) {
+            const double rel = (double)mm / st->b_norm;
+            st->rel = rel;
+            if (rel < st->tol) st->done = 1;
+        }
+    } else if (op == C_BETA) {
+        // Polak-Ribiere with the flexible numerator (poisson.py:400-406):
+        // z' = z - mean(z); rz_new = <r, z'>; num = rz_new - <r_prev, z'>
+        // = -alpha <Ap, z'> because r_prev = r + alpha Ap.
+        double t[5];
+        combine_sums<5>(st->part, nparts, t, sh);
+        if (threadIdx.x == 0) {
+            const double mu = t[0] / (double)n;
+            const double rz_new = t[3] - mu * t[1];
+            const double apz = t[4] - mu * t[2];
+            const double num = -st->alpha * apz;
+            const double rz = st->rz;
+            st->beta = (rz != 0.0) ? fmax(num / rz, 0.0) : 0.0;
+            st->mu = mu;
+            st->rz = rz_new;
         }
     }
 }
@@ -803,7 +818,9 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
         CGState* st = static_cast<CGState*>(mg.scalars);
         const int64_t n = ncell(mg.lev[l]);
         // p is ST; the mean is taken in FP64 and the shifted value stored back
-        k_mean<ST><<<red_blocks(n), RT, 0, s>>>(P(l), n, st, 6, &st->mu_x, 0);
+        k_part_sum<ST><<<red_blocks(n), RT, 0, s>>>(P(l), n, st->part);
+        PF_CHECK_LAUNCH();
+        k_comb<<<1, RT, 0, s>>>(st, C_MEAN, (int)red_blocks(n), n, &st->mu_x, 0.0);
         PF_CHECK_LAUNCH();
         k_shift<ST, ST><<<red_blocks(n), RT, 0, s>>>(P(l), P(l), n, &st->mu_x);
         PF_CHECK_LAUNCH();
@@ -847,14 +864,21 @@ static int get_poll(HostPoll*& out) {
 template <typename ST>
 static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, int* flag,
                              unsigned nb, hipStream_t s) {
+    CGState* st = a.st;
     k_cg_apply<ST><<<nb, RT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
+    k_comb<<<1, RT, 0, s>>>(st, C_APPLY, (int)nb, a.n, nullptr, 0.0);
+    PF_CHECK_LAUNCH();
     k_cg_update<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    k_comb<<<1, RT, 0, s>>>(st, C_UPDATE, (int)nb, a.n, nullptr, 0.0);
     PF_CHECK_LAUNCH();
     PF_CHECK_HIP(hipMemcpyAsync(flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
     int rc = vcycle<ST>(mg, a.r, z, done, s);
     if (rc) return rc;
     k_cg_beta<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    k_comb<<<1, RT, 0, s>>>(st, C_BETA, (int)nb, a.n, nullptr, 0.0);
     PF_CHECK_LAUNCH();
     k_cg_q<ST><<<nb, RT, 0, s>>>(a, 0);
     PF_CHECK_LAUNCH();
@@ -868,6 +892,8 @@ static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int*
     int rc = vcycle<ST>(mg, a.r, z, done, s);
     if (rc) return rc;
     k_cg_first<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    k_comb<<<1, RT, 0, s>>>(a.st, C_FIRST, (int)nb, a.n, nullptr, 0.0);
     PF_CHECK_LAUNCH();
     k_cg_q<ST><<<nb, RT, 0, s>>>(a, 1);
     PF_CHECK_LAUNCH();
@@ -983,9 +1009,13 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
     const unsigned nb = red_blocks(n);
     const int* done = &st->done;
 
-    k_mean<float><<<nb, RT, 0, s>>>(b, n, st, 0, &st->mu, 1);
+    k_part_sum<float><<<nb, RT, 0, s>>>(b, n, st->part);
+    PF_CHECK_LAUNCH();
+    k_comb<<<1, RT, 0, s>>>(st, C_MEAN_RESET, (int)nb, n, &st->mu, 0.0);
     PF_CHECK_LAUNCH();
     k_cg_init<ST><<<nb, RT, 0, s>>>(a);
+    PF_CHECK_LAUNCH();
+    k_comb<<<1, RT, 0, s>>>(st, C_INIT, (int)nb, n, nullptr, tol);
     PF_CHECK_LAUNCH();
     int init_done = 0;
     PF_CHECK_HIP(hipMemcpyAsync(&init_done, done, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1025,7 +1055,9 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
         }
     }
     // _remove_mean(x) (poisson.py:408)
-    k_mean<float><<<nb, RT, 0, s>>>(x, n, st, 7, &st->mu_x, 0);
+    k_part_sum<float><<<nb, RT, 0, s>>>(x, n, st->part);
+    PF_CHECK_LAUNCH();
+    k_comb<<<1, RT, 0, s>>>(st, C_MEAN, (int)nb, n, &st->mu_x, 0.0);
     PF_CHECK_LAUNCH();
     k_shift<float, float><<<nb, RT, 0, s>>>(x, x, n, &st->mu_x);
     PF_CHECK_LAUNCH();
@@ -1048,7 +1080,9 @@ static int precondition_impl(const pf_mg& mg, const float* r, float* z, hipStrea
     ST* zt = (sizeof(ST) == 4) ? reinterpret_cast<ST*>(z) : Lv<ST>::p(mg.lev[0]);
     int rc = vcycle<ST>(mg, r, zt, nullptr, s);
     if (rc) return rc;
-    k_mean<ST><<<red_blocks(n), RT, 0, s>>>(zt, n, st, 6, &st->mu_x, 0);
+    k_part_sum<ST><<<red_blocks(n), RT, 0, s>>>(zt, n, st->part);
+    PF_CHECK_LAUNCH();
+    k_comb<<<1, RT, 0, s>>>(st, C_MEAN, (int)red_blocks(n), n, &st->mu_x, 0.0);
     PF_CHECK_LAUNCH();
     k_shift<ST, float><<<red_blocks(n), RT, 0, s>>>(zt, z, n, &st->mu_x);
     PF_CHECK_LAUNCH();
@@ -1164,6 +1198,56 @@ PF_API int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol,
         case 1: return solve_impl<bf16s>(*mg, b, x, tol, max_iter, it_out, rel_out, S(stream));
         default: return solve_impl<fp16s>(*mg, b, x, tol, max_iter, it_out, rel_out, S(stream));
     }
+}
+
+PF_API int pf_mg_time_smoother(const pf_mg* mg, int32_t level, int32_t reps, float* ms_out,
+                               pf_stream_t stream) {
+    int rc = check_mg(mg);
+    if (rc) return rc;
+    PF_REQUIRE(ms_out && reps > 0, "pf_mg_time_smoother: bad arguments");
+    PF_REQUIRE(level >= 0 && level < mg->nlev && level < mg->small_from,
+               "pf_mg_time_smoother: level %d is not smoothed out of HBM", level);
+    hipStream_t s = S(stream);
+    const pf_mg_level& lv = mg->lev[level];
+    const Dims d = ldims(lv);
+    const int64_t nthr = (int64_t)d.nx * d.ny * ((d.nz + 1) / 2);
+    const unsigned grid = cdiv(nthr, RT);
+    hipEvent_t e0, e1;
+    PF_CHECK_HIP(hipEventCreate(&e0));
+    PF_CHECK_HIP(hipEventCreate(&e1));
+    auto run = [&]() -> int {
+        for (int r = 0; r < reps; ++r)
+            for (int c = 0; c < 2; ++c) {
+                if (mg->precision == 0) {
+                    float* pp = level == 0 ? mg->z : lv.p;
+                    k_rbgs<float, false><<<grid, RT, 0, s>>>(d, lv.bx, lv.by, lv.bz, pp,
+                                                             level == 0 ? mg->res : lv.b, c, mg->omega, nullptr);
+                } else if (mg->precision == 1) {
+                    k_rbgs<bf16s, false><<<grid, RT, 0, s>>>(d, Lv<bf16s>::fx(lv), Lv<bf16s>::fy(lv),
+                                                             Lv<bf16s>::fz(lv), Lv<bf16s>::p(lv),
+                                                             level == 0 ? mg->res : lv.b, c, mg->omega, nullptr);
+                } else {
+                    k_rbgs<fp16s, false><<<grid, RT, 0, s>>>(d, Lv<fp16s>::fx(lv), Lv<fp16s>::fy(lv),
+                                                             Lv<fp16s>::fz(lv), Lv<fp16s>::p(lv),
+                                                             level == 0 ? mg->res : lv.b, c, mg->omega, nullptr);
+                }
+                PF_CHECK_LAUNCH();
+            }
+        return PF_OK;
+    };
+    rc = run();  // warm-up
+    if (rc) return rc;
+    PF_CHECK_HIP(hipEventRecord(e0, s));
+    rc = run();
+    if (rc) return rc;
+    PF_CHECK_HIP(hipEventRecord(e1, s));
+    PF_CHECK_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PF_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = ms / (2.0f * reps);
+    return PF_OK;
 }
 
 PF_API int pf_apply_A(int nx, int ny, int nz, const float* p, const float* bx, const float* by,

@@ -147,6 +147,16 @@ class MGPCG:
         return int(it.value), float(rel.value)
 
 
+def time_smoother(solver, level=0, reps=10):
+    """Average milliseconds of one red-black half-sweep kernel of an HBM
+    level (HIP events on the current stream).  Smooths that level's current
+    iterate in place, so call it after a solve."""
+    ms = ctypes.c_float(0.0)
+    _lib.call("pf_mg_time_smoother", ctypes.byref(solver._mg), int(level), int(reps),
+              ctypes.byref(ms), _lib.stream())
+    return float(ms.value)
+
+
 def apply_A(p, bx, by, bz, diag, out):
     """``out = A p`` (the ``apply_A`` kernel, pfflip/poisson.py:54-88)."""
     nx, ny, nz = p.shape

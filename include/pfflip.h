@@ -194,6 +194,12 @@ int pf_mg_precondition(const pf_mg* mg, const float* r, float* z, pf_stream_t st
  * Writes x (mean removed); *it_out / *rel_out are host pointers.  Blocks. */
 int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol, int32_t max_iter,
                    int32_t* it_out, double* rel_out, pf_stream_t stream);
+/* Measurement hook (no reference counterpart): average milliseconds of one
+ * red-black half-sweep kernel of HBM level `level` over `reps` full sweeps,
+ * timed with HIP events on `stream`.  Smooths that level's current iterate
+ * in place (use after a solve).  Blocks. */
+int pf_mg_time_smoother(const pf_mg* mg, int32_t level, int32_t reps, float* ms_out,
+                        pf_stream_t stream);
 /* apply_A (poisson.py:54-88): out = diag*p - sum_f beta_f p_nbr with
  * neighbour indices clamped at the walls.  diag may be NULL (then it is
  * summed from the six faces in compute_diag's order, poisson.py:40-51). */
