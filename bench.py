@@ -129,9 +129,9 @@ def main():
     ap.add_argument("--skip-solve-compare", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part "
-                                   "of it (the path would shard by slab decomposition, see "
-                                   "DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "only single-GPU runs are supported: the step runs on one "
+                                   "GPU (a multi-GPU version would shard the grid into x-slabs "
+                                   "with halo exchange)", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import numpy as np

@@ -6,8 +6,8 @@
  * the Python object API of `Simulator` / `MGPCG`, whose stages launch Warp
  * kernels.  Each entry point below replaces the stage (or kernel group) cited
  * next to it, as `path:line` in the reference repository.  The Python
- * drop-in (`Simulator`, `MGPCG`, `apply_A`) binds these with ctypes; the
- * binding a maintainer would add is shown in INTEGRATION.md.
+ * drop-in (`Simulator`, `MGPCG`, `apply_A`) binds these with ctypes
+ * (implementation-of-adaptive-phase-field-flip_amd/_lib.py).
  *
  * Conventions
  *  - Return 0 on success, a negative PF_ERR_* for invalid arguments, or a
