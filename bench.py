@@ -8,10 +8,12 @@ relative (inf-norm) residual, droplet stage off.  A step is one full
 MG-PCG, projection, extrapolation, G2P + RK3 advection.
 
 Prints one JSON line: particle-steps/s over the timed steps, the roofline of
-the dominant kernel (the level-0 smoother half-sweep) and of P2G and G2P
+the dominant kernel (the level-0 smoother sweep) and of P2G and G2P
 (algorithmic bytes / measured average time vs 8 TB/s), the FP32 vs
-BF16 MG-PCG time to 1e-6 on the last step's captured system, per-stage
-milliseconds, and the CPU baseline (the FP64 numpy restatement of the
+BF16 MG-PCG time to 1e-6 on the last step's captured system (with each
+one's pressure error against an FP64 reference solution), per-stage
+milliseconds, the differentiable 64^3 dam break (adjoint vs central finite
+differences), and the CPU baseline (the FP64 numpy restatement of the
 reference step on a bounded sample, one thread).
 """
 
@@ -49,6 +51,12 @@ def smoother_bytes(n_cells, s_p=4, s_beta=4, s_b=4):
     return n_cells * (s_p + 3 * s_beta + s_b / 2 + s_p / 2)
 
 
+def fused_sweep_bytes(n_cells, s_p=4, s_beta=4, s_b=4):
+    """Algorithmic bytes of one fused full red-black sweep: p, the three face
+    arrays and b read once, p written once (24 B/cell in FP32)."""
+    return n_cells * (2 * s_p + 3 * s_beta + s_b)
+
+
 def roofline(name, nbytes, ms):
     gbs = nbytes / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": name, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -79,11 +87,51 @@ def cpu_baseline(seconds_budget=30.0):
                       f"{steps} steps in {dt:.1f} s"}
 
 
+def fp64_apply_A(bx, by, bz, p):
+    """A p in FP64 (the 7-point operator of pfflip/poisson.py:54-88) for
+    tensors of any device: diag*p - sum_f beta_f p_nbr over interior faces,
+    with diag summed from the faces in FP64 (compute_diag, poisson.py:40-51),
+    so that every row sums to zero exactly as in the operator being solved."""
+    diag = bx[:-1] + bx[1:] + by[:, :-1] + by[:, 1:] + bz[:, :, :-1] + bz[:, :, 1:]
+    Ap = diag * p
+    Ap[1:] -= bx[1:-1] * p[:-1]
+    Ap[:-1] -= bx[1:-1] * p[1:]
+    Ap[:, 1:] -= by[:, 1:-1] * p[:, :-1]
+    Ap[:, :-1] -= by[:, 1:-1] * p[:, 1:]
+    Ap[:, :, 1:] -= bz[:, :, 1:-1] * p[:, :, :-1]
+    Ap[:, :, :-1] -= bz[:, :, 1:-1] * p[:, :, 1:]
+    return Ap
+
+
+def fp64_reference_pressure(solver, rhs, torch, tol=1e-12, max_refine=10):
+    """FP64 solution of A p = b - mean(b) by iterative refinement: the
+    residual is formed in FP64 and each correction comes from the FP32
+    MG-PCG.  Returns (p, rel) with rel = max|b - A p| / max|b| in FP64."""
+    L0 = solver.levels[0]
+    bx, by, bz = (t.double() for t in (L0.bx, L0.by, L0.bz))
+    b0 = rhs.double() - rhs.double().mean()
+    bmax = float(b0.abs().max())
+    p = torch.zeros_like(b0)
+    e = torch.zeros_like(rhs)
+    rel = 1.0
+    for _ in range(max_refine):
+        r = b0 - fp64_apply_A(bx, by, bz, p)
+        rel = float(r.abs().max()) / bmax
+        if rel < tol:
+            break
+        solver.solve(r.float(), e, tol=1e-6, max_iter=400)
+        p += e.double()
+        p -= p.mean()
+    return p, rel
+
+
 def solve_compare(pf, torch, sim, cfg, tol):
-    """Time FP32 and BF16 MG-PCG on the same captured system (beta, rhs)."""
+    """Time FP32 and BF16 MG-PCG on the same captured system (beta, rhs) and
+    give each one's pressure error against an FP64 reference solution."""
     out = {}
     beta = [b.clone() for b in sim.beta]
     rhs = sim.rhs.clone()
+    p_ref = None
     for prec in ("fp32", "bf16"):
         s = pf.MGPCG(sim.res, cfg.device, omega=cfg.mg_omega, pre=cfg.mg_pre_sweeps,
                      post=cfg.mg_post_sweeps, coarsest=cfg.mg_coarsest_sweeps,
@@ -98,23 +146,90 @@ def solve_compare(pf, torch, sim, cfg, tol):
             it, rel = s.solve(rhs, p, tol=tol, max_iter=400)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
-        # true residual of the returned pressure, in FP64
-        L0 = s.levels[0]
-        pd = p.double()
-        bx, by, bz = L0.bx.double(), L0.by.double(), L0.bz.double()
-        Ap = L0.diag.double() * pd
-        Ap[1:] -= bx[1:-1] * pd[:-1]
-        Ap[:-1] -= bx[1:-1] * pd[1:]
-        Ap[:, 1:] -= by[:, 1:-1] * pd[:, :-1]
-        Ap[:, :-1] -= by[:, 1:-1] * pd[:, 1:]
-        Ap[:, :, 1:] -= bz[:, :, 1:-1] * pd[:, :, :-1]
-        Ap[:, :, :-1] -= bz[:, :, 1:-1] * pd[:, :, 1:]
-        b0 = rhs.double() - rhs.double().mean()
-        true_rel = float((b0 - Ap).abs().max() / b0.abs().max())
-        del pd, bx, by, bz, Ap, b0
-        out[prec] = {"ms": 1e3 * min(times), "iters": it, "rel": rel, "true_rel": true_rel}
-        del s
+        if p_ref is None:
+            p_ref, ref_rel = fp64_reference_pressure(s, rhs, torch)
+            out["fp64_reference"] = {"method": "FP64-residual iterative refinement",
+                                     "rel_residual": ref_rel}
+        d = p.double() - p_ref
+        d -= d.mean()
+        p_err = float(d.abs().max() / p_ref.abs().max())
+        out[prec] = {"ms": 1e3 * min(times), "iters": it, "rel": rel, "p_err_vs_fp64": p_err}
+        del s, d
     return out
+
+
+def adjoint_check(pf, torch, n=64, ppc=8, n_steps=16, n_dirs=8, eps=1e-3):
+    """Differentiable dam break: 64^3, 8 ppc, 16 steps, FP32.  Control: a
+    seeded Gaussian-random initial liquid velocity field (sigma 0.1 after a
+    4-cell Gaussian smoothing), sampled trilinearly at the liquid particles.
+    Loss: squared distance of the final liquid centre of mass to
+    (0.6, 0.5, 0.2).  Hand-written adjoint vs central finite differences
+    (eps = 1e-3) along n_dirs seeded smooth directions, dt held fixed."""
+    import numpy as np
+    import scipy.ndimage as ndi
+    import torch.nn.functional as F
+
+    cfg = pf.Config(res=(n, n, n), dx=1.0 / n, ppc=ppc, gravity=(0.0, 0.0, -9.81), droplets=False,
+                    pcg_tol=1e-7, pcg_max_iter=400)
+    pos, vel, pt = pf.scenes.dam_break(cfg, frac=(0.4, 1.0, 0.6), jitter=0.5)
+    sim = pf.Simulator(cfg)
+    sim.set_particles(pos, 0.0 * vel, pt)
+    rng = np.random.default_rng(cfg.seed)
+
+    def smooth_field():
+        f = np.stack([ndi.gaussian_filter(rng.standard_normal((n, n, n)), sigma=4.0, mode="wrap")
+                      for _ in range(3)])
+        return f
+
+    u = smooth_field()
+    u0 = torch.from_numpy((0.1 / u.std() * u).astype(np.float32)).cuda()
+    x0 = sim.px.clone()
+    N = x0.shape[0]
+    liq = (sim.ptype == 1).float()[:, None]
+    L = float(n * cfg.dx)
+    grid = (2.0 * x0 / L - 1.0)[:, [2, 1, 0]].reshape(1, N, 1, 1, 3)
+
+    def sample(field):
+        out = F.grid_sample(field.unsqueeze(0), grid, mode="bilinear", padding_mode="border",
+                            align_corners=False)
+        return (out.reshape(3, N).t() * liq).contiguous()
+
+    target = torch.tensor([0.6, 0.5, 0.2], dtype=torch.float64, device="cuda")
+    lmask = sim.ptype == 1
+
+    def loss_of(xT):
+        return ((xT[lmask].double().mean(0) - target) ** 2).sum()
+
+    sim.stats.clear()
+    with torch.no_grad():
+        pf.rollout(sim, x0, sample(u0), n_steps)
+    dts = [r["dt"] for r in sim.stats]
+    adj = pf.StepAdjoint(sim)
+    uu = u0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xT, _ = pf.rollout(sim, x0, sample(uu), n_steps, dts=dts, adjoint=adj)
+    loss = loss_of(xT)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = uu.grad.detach().double()
+    rels, rows = [], []
+    for _ in range(n_dirs):
+        dvec = smooth_field()
+        d = torch.from_numpy((dvec / np.abs(dvec).max()).astype(np.float32)).cuda()
+        with torch.no_grad():
+            lp = float(loss_of(pf.rollout(sim, x0, sample(u0 + eps * d), n_steps, dts=dts)[0]))
+            lm = float(loss_of(pf.rollout(sim, x0, sample(u0 - eps * d), n_steps, dts=dts)[0]))
+        fd = (lp - lm) / (2.0 * eps)
+        ad = float((g * d.double()).sum())
+        rels.append(abs(fd - ad) / max(abs(fd), 1e-30))
+        rows.append({"fd": fd, "adjoint": ad, "rel": rels[-1]})
+    return {"workload": f"differentiable dam break {n}^3 x {ppc} ppc, {n_steps} steps, FP32",
+            "particles": N, "loss": float(loss), "forward_ms": 1e3 * (t1 - t0),
+            "backward_ms": 1e3 * (t2 - t1), "eps": eps, "max_rel_err": max(rels), "directions": rows}
 
 
 def main():
@@ -127,6 +242,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve-compare", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "only single-GPU runs are supported: the step runs on one "
@@ -167,12 +283,13 @@ def main():
     stages = {k: v / args.steps for k, v in sim.stage_ms.items()}
     sim.stage_ms = None
 
-    # the dominant kernel: the level-0 red-black half-sweep of the V-cycle,
-    # timed with HIP events on the solver's stream right after the timed steps
+    # the dominant kernel: the level-0 red-black sweep of the V-cycle (one
+    # fused kernel per sweep), timed with HIP events on the solver's stream
+    # right after the timed steps
     smooth_ms = pf.poisson.time_smoother(sim.solver, 0, reps=10)
     s16 = 2 if args.precision != "fp32" else 4
     kernels = [
-        roofline("k_rbgs level-0 half-sweep (128^3)", smoother_bytes(n_c, s16, s16), smooth_ms),
+        roofline(f"k_rbgs_full level-0 sweep ({n}^3)", fused_sweep_bytes(n_c, s16, s16), smooth_ms),
         roofline("p2g stage (k_max_count + k_p2g + k_p2g_gather)", p2g_bytes(n_p, n_c), stages["p2g"]),
         roofline("k_g2p_advect (G2P + RK3 + reflect)", g2p_bytes(n_p, n_c), stages["g2p_advect"]),
     ]
@@ -199,6 +316,10 @@ def main():
     }
     if not args.skip_solve_compare:
         result["mgpcg_1e-6"] = solve_compare(pf, torch, sim, cfg, 1e-6)
+    if not args.skip_adjoint:
+        del sim
+        torch.cuda.empty_cache()
+        result["adjoint_64"] = adjoint_check(pf, torch)
     if not args.skip_cpu:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)

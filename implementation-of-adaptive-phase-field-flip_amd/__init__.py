@@ -2,12 +2,15 @@
 
 Exports ``Config``, ``Simulator``, ``MGPCG``, ``apply_A`` and ``scenes`` with
 the reference's names and semantics (pfflip/__init__.py:7-10,
-pfflip/poisson.py:78,240).  Compute runs in hand-written gfx950 HIP kernels
+pfflip/poisson.py:78,240), plus ``rollout`` / ``StepAdjoint``: steps
+differentiable through a hand-written adjoint (``adjoint.py``).  Compute
+runs in hand-written gfx950 HIP kernels
 (``csrc/``, built into ``libpfflip_hip.so`` and bound with ctypes); there is
 no CPU fallback.
 """
 
 from . import scenes
+from .adjoint import StepAdjoint, rollout
 from .config import Config
 from .poisson import MGPCG, apply_A
 from .sim import Simulator
@@ -19,4 +22,4 @@ def build(verbose: bool = False) -> str:
     return _build(verbose=verbose)
 
 
-__all__ = ["Config", "Simulator", "MGPCG", "apply_A", "scenes", "build"]
+__all__ = ["Config", "Simulator", "MGPCG", "apply_A", "scenes", "build", "rollout", "StepAdjoint"]

@@ -40,6 +40,9 @@ class PhaseArgs(ctypes.Structure):
     ]
 
 
+MG_UNFUSED_SMOOTHER = 1
+
+
 class MGLevel(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
@@ -54,7 +57,7 @@ class MG(ctypes.Structure):
         ("nlev", ctypes.c_int32), ("pre", ctypes.c_int32), ("post", ctypes.c_int32),
         ("coarsest", ctypes.c_int32), ("omega", ctypes.c_float),
         ("precision", ctypes.c_int32), ("small_from", ctypes.c_int32),
-        ("_pad", ctypes.c_int32),
+        ("options", ctypes.c_int32),
         ("lev", MGLevel * MAX_LEVELS),
         ("rhs", P), ("res", P), ("z", P), ("q", P), ("Ap", P), ("scalars", P),
     ]
@@ -102,6 +105,15 @@ SIGNATURES = {
                                       ctypes.c_int, ctypes.c_int, P]),
     "pf_g2p_advect": (ctypes.c_int, [ctypes.POINTER(G2PArgs), P]),
     "pf_max_speed": (ctypes.c_int, [P, P, P, I64, P, P]),
+    "pf_adj_g2p_advect": (ctypes.c_int, [ctypes.POINTER(G2PArgs), P, P, P, P, PP, PP, PP, P]),
+    "pf_adj_extrapolate": (ctypes.c_int, [P, PP, PP, PP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, P]),
+    "pf_adj_project": (ctypes.c_int, [P, PP, PP, PP, PP, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      F32, F32, P]),
+    "pf_adj_solve_beta": (ctypes.c_int, [P, P, PP, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]),
+    "pf_adj_phase": (ctypes.c_int, [ctypes.POINTER(PhaseArgs), PP, PP, PP, P, PP, PP, P]),
+    "pf_adj_p2g": (ctypes.c_int, [P, P, P, P, I64, PP, PP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  F32, F32, P, P, P]),
 }
 
 _LIB = None

@@ -82,25 +82,46 @@ __device__ inline void stencil_s(const Dims& d, const FT* __restrict__ bx, const
 // no cancellation error: the reference's form limits the recursively
 // updated CG residual to ~3e-6 (FP32) / ~4e-5 (16-bit preconditioner) at a
 // 1000:1 coefficient jump, this form to ~1e-9 / ~5e-8.
+// f = (x-low, x-high, y-low, y-high, z-low, z-high) faces, pn the matching
+// neighbour values.  Every smoother and residual kernel goes through these
+// helpers, so all of them round identically.
+__device__ inline void stencil_vals(float pc, const float (&f)[6], const float (&pn)[6], float& diag,
+                                    float& Ap) {
+    diag = f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
+    float s = f[0] * (pc - pn[0]);
+    s += f[1] * (pc - pn[1]);
+    s += f[2] * (pc - pn[2]);
+    s += f[3] * (pc - pn[3]);
+    s += f[4] * (pc - pn[4]);
+    s += f[5] * (pc - pn[5]);
+    Ap = s;
+}
+
+// weighted Gauss-Seidel update of one cell (poisson.py:120-126)
+__device__ inline float rb_relax(float pc, float b, float Ap, float dg, float omega) {
+    return pc + omega * (b - Ap) / dg;
+}
+// the same update from p = 0 (every neighbour and the cell itself zero)
+__device__ inline float rb_first(float b, float dg, float omega) { return omega * b / dg; }
+
+template <typename ST>
+__device__ inline float round_st(float v) { return to_f32(from_f32<ST>(v)); }
+
 template <typename FT, typename PT>
 __device__ inline void stencil(const Dims& d, const FT* __restrict__ bx, const FT* __restrict__ by,
                                const FT* __restrict__ bz, const PT* p, int i, int j, int k,
                                float& diag, float& Ap, float& pc) {
-    const float bxl = to_f32(bx[face_index<0>(d, i, j, k)]);
-    const float bxh = to_f32(bx[face_index<0>(d, i + 1, j, k)]);
-    const float byl = to_f32(by[face_index<1>(d, i, j, k)]);
-    const float byh = to_f32(by[face_index<1>(d, i, j + 1, k)]);
-    const float bzl = to_f32(bz[face_index<2>(d, i, j, k)]);
-    const float bzh = to_f32(bz[face_index<2>(d, i, j, k + 1)]);
-    diag = bxl + bxh + byl + byh + bzl + bzh;
+    const float f[6] = {to_f32(bx[face_index<0>(d, i, j, k)]), to_f32(bx[face_index<0>(d, i + 1, j, k)]),
+                        to_f32(by[face_index<1>(d, i, j, k)]), to_f32(by[face_index<1>(d, i, j + 1, k)]),
+                        to_f32(bz[face_index<2>(d, i, j, k)]), to_f32(bz[face_index<2>(d, i, j, k + 1)])};
     pc = to_f32(p[cell_index(d, i, j, k)]);
-    float s = bxl * (pc - to_f32(p[cell_index(d, max(i - 1, 0), j, k)]));
-    s += bxh * (pc - to_f32(p[cell_index(d, min(i + 1, d.nx - 1), j, k)]));
-    s += byl * (pc - to_f32(p[cell_index(d, i, max(j - 1, 0), k)]));
-    s += byh * (pc - to_f32(p[cell_index(d, i, min(j + 1, d.ny - 1), k)]));
-    s += bzl * (pc - to_f32(p[cell_index(d, i, j, max(k - 1, 0))]));
-    s += bzh * (pc - to_f32(p[cell_index(d, i, j, min(k + 1, d.nz - 1))]));
-    Ap = s;
+    const float pn[6] = {to_f32(p[cell_index(d, max(i - 1, 0), j, k)]),
+                         to_f32(p[cell_index(d, min(i + 1, d.nx - 1), j, k)]),
+                         to_f32(p[cell_index(d, i, max(j - 1, 0), k)]),
+                         to_f32(p[cell_index(d, i, min(j + 1, d.ny - 1), k)]),
+                         to_f32(p[cell_index(d, i, j, max(k - 1, 0))]),
+                         to_f32(p[cell_index(d, i, j, min(k + 1, d.nz - 1))])};
+    stencil_vals(pc, f, pn, diag, Ap);
 }
 
 // ================================================================ hierarchy build
@@ -194,7 +215,7 @@ __global__ __launch_bounds__(RT) void k_rbgs(Dims d, const ST* __restrict__ bx,
                 const float dg = to_f32(bx[face_index<0>(d, i, j, k)]) + to_f32(bx[face_index<0>(d, i + 1, j, k)]) +
                                  to_f32(by[face_index<1>(d, i, j, k)]) + to_f32(by[face_index<1>(d, i, j + 1, k)]) +
                                  to_f32(bz[face_index<2>(d, i, j, k)]) + to_f32(bz[face_index<2>(d, i, j, k + 1)]);
-                if (dg > 0.0f) v = omega * b[cell_index(d, i, j, k)] / dg;
+                if (dg > 0.0f) v = rb_first(b[cell_index(d, i, j, k)], dg, omega);
             }
             if (!skip) p[cell_index(d, i, j, k)] = from_f32<ST>(v);
         }
@@ -205,8 +226,249 @@ __global__ __launch_bounds__(RT) void k_rbgs(Dims d, const ST* __restrict__ bx,
         stencil(d, bx, by, bz, p, i, j, k, dg, Ap, pc);
         if (dg <= 0.0f) return;
         const int c = cell_index(d, i, j, k);
-        const float res = b[c] - Ap;
-        if (!skip) p[c] = from_f32<ST>(pc + omega * res / dg);
+        if (!skip) p[c] = from_f32<ST>(rb_relax(pc, b[c], Ap, dg, omega));
+    }
+}
+
+// Pair loads/stores: cells (i,j,k) and (i,j,k+1) with k even, adjacent in
+// memory and 2-element aligned whenever the row length is even.
+template <typename ST>
+__device__ inline void ld_pair(const ST* p, int idx, float& a, float& b);
+template <>
+__device__ inline void ld_pair<float>(const float* p, int idx, float& a, float& b) {
+    const float2 v = *reinterpret_cast<const float2*>(p + idx);
+    a = v.x;
+    b = v.y;
+}
+template <typename ST>
+__device__ inline void ld_pair(const ST* p, int idx, float& a, float& b) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(p + idx);
+    a = to_f32(ST{(uint16_t)(w & 0xFFFFu)});
+    b = to_f32(ST{(uint16_t)(w >> 16)});
+}
+template <typename ST>
+__device__ inline void st_pair(ST* p, int idx, float a, float b);
+template <>
+__device__ inline void st_pair<float>(float* p, int idx, float a, float b) {
+    *reinterpret_cast<float2*>(p + idx) = make_float2(a, b);
+}
+template <typename ST>
+__device__ inline void st_pair(ST* p, int idx, float a, float b) {
+    const uint32_t w = (uint32_t)from_f32<ST>(a).u | ((uint32_t)from_f32<ST>(b).u << 16);
+    *reinterpret_cast<uint32_t*>(p + idx) = w;
+}
+
+// One full red-black sweep (colour c1, then the other colour) in one launch,
+// bitwise equal to two in-place half-sweeps.  A workgroup owns a TI x TJ x TK
+// tile.  It stages the iterate of the tile plus a 2-cell halo in LDS
+// (coordinates clamped into the grid, which reproduces the reference's
+// clamped neighbour indices: a clamped neighbour is the cell itself), updates
+// the first colour on the tile and on its 1-cell halo shell (the shell
+// updates duplicate work of the neighbouring tiles), then the second colour
+// on the tile, and writes the whole tile to `pout`.  Reading `pin` and
+// writing `pout` (ping-pong) keeps neighbouring tiles' halo reads race-free.
+// Each thread owns pairs of cells (k, k+1) of the tile and keeps the pair's
+// faces in registers between the two colours, so faces, b and p are read
+// from memory once per sweep.  Every global load (staging, the tile's faces
+// and b, the shell cells' faces and b) is issued before the first wait, so
+// a workgroup is exposed to memory latency once.  Needs nz even.
+// ZINIT: the input is zero (no staging loads).
+template <int TI, int TJ, int TK>
+struct RbTile {
+    static constexpr int NT = 256;
+    static constexpr int PI = TI + 4, PJ = TJ + 4, PK = TK + 4;
+    static constexpr int NP = PI * PJ * PK;
+    static constexpr int NPR = PK / 2;                    // staged pairs per row
+    static constexpr int NSP = PI * PJ * NPR;             // staged pairs
+    static constexpr int SPT = (NSP + NT - 1) / NT;       // staged pairs per thread
+    static constexpr int PAIRS = TI * TJ * TK / 2;
+    static constexpr int PPT = PAIRS / NT;
+    // first-colour shell cells: i-planes and j-planes hold rows along k of
+    // TK+2 cells (half of each row has colour c1); k-planes rows along j
+    static constexpr int RK = (TK + 2) / 2;
+    static constexpr int S1 = 2 * (TJ + 2) * RK, S2 = 2 * TI * RK, S3 = 2 * TI * (TJ / 2);
+    static constexpr int NS = S1 + S2 + S3;
+    static constexpr int SHT = (NS + NT - 1) / NT;
+    static_assert(PAIRS % NT == 0 && TK % 2 == 0 && TJ % 2 == 0 && PPT >= 1, "tile shape");
+};
+
+template <typename ST, bool ZINIT, int TI, int TJ, int TK>
+__global__ __launch_bounds__(256) void k_rbgs_full(Dims d, const ST* __restrict__ bx,
+                                                   const ST* __restrict__ by, const ST* __restrict__ bz,
+                                                   const ST* __restrict__ pin, ST* __restrict__ pout,
+                                                   const float* __restrict__ b, int c1, float omega,
+                                                   const int* __restrict__ done) {
+    using T = RbTile<TI, TJ, TK>;
+    __shared__ float P[T::NP];
+    const int tid = threadIdx.x;
+    const int ntk = (d.nz + TK - 1) / TK, ntj = (d.ny + TJ - 1) / TJ;
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = (tile / (ntk * ntj)) * TI, j0 = ((tile / ntk) % ntj) * TJ, k0 = (tile % ntk) * TK;
+    constexpr int SI = T::PJ * T::PK, SJ = T::PK;
+    auto lidx = [&](int i, int j, int k) { return ((i - i0 + 2) * T::PJ + (j - j0 + 2)) * T::PK + (k - k0 + 2); };
+
+    // (1) staging loads: pairs along k of the tile + 2-cell halo
+    float2 sv[T::SPT];
+    if (!ZINIT) {
+#pragma unroll
+        for (int r = 0; r < T::SPT; ++r) {
+            const int e = tid + r * T::NT;
+            sv[r] = make_float2(0.0f, 0.0f);
+            if (e < T::NSP) {
+                const int q = e % T::NPR, row = e / T::NPR;
+                const int gi = min(max(i0 - 2 + row / T::PJ, 0), d.nx - 1);
+                const int gj = min(max(j0 - 2 + row % T::PJ, 0), d.ny - 1);
+                // pair (kk, kk+1) with kk even, loaded from the nearest
+                // in-grid pair (fixed up at the LDS write)
+                const int kk = k0 - 2 + 2 * q;
+                const int kc = min(max(kk, 0), d.nz - 2);
+                ld_pair(pin, cell_index(d, gi, gj, kc), sv[r].x, sv[r].y);
+            }
+        }
+    }
+
+    // (2) the tile's pairs: faces and b into registers
+    float f[T::PPT][11], bb[T::PPT][2];
+    int li[T::PPT];
+    bool ok[T::PPT];
+#pragma unroll
+    for (int q = 0; q < T::PPT; ++q) {
+        const int pq = tid + q * T::NT;
+        const int kp = pq % (TK / 2), jl = (pq / (TK / 2)) % TJ, il = pq / ((TK / 2) * TJ);
+        const int i = i0 + il, j = j0 + jl, k = k0 + 2 * kp;
+        ok[q] = i < d.nx && j < d.ny && k < d.nz;
+        li[q] = ((il + 2) * T::PJ + (jl + 2)) * T::PK + (2 * kp + 2);
+        if (ok[q]) {
+            ld_pair(bx, face_index<0>(d, i, j, k), f[q][0], f[q][1]);      // x-low of k, k+1
+            ld_pair(bx, face_index<0>(d, i + 1, j, k), f[q][2], f[q][3]);  // x-high
+            ld_pair(by, face_index<1>(d, i, j, k), f[q][4], f[q][5]);      // y-low
+            ld_pair(by, face_index<1>(d, i, j + 1, k), f[q][6], f[q][7]);  // y-high
+            const int z = face_index<2>(d, i, j, k);
+            f[q][8] = to_f32(bz[z]);
+            f[q][9] = to_f32(bz[z + 1]);
+            f[q][10] = to_f32(bz[z + 2]);
+            ld_pair(b, cell_index(d, i, j, k), bb[q][0], bb[q][1]);
+        }
+    }
+
+    // (3) first-colour cells of the 1-cell halo shell that lie in the grid
+    float sf[T::SHT][6], sb[T::SHT];
+    int sc[T::SHT];
+    bool sok[T::SHT];
+#pragma unroll
+    for (int r = 0; r < T::SHT; ++r) {
+        int e = tid + r * T::NT, i, j, k;
+        if (e < T::S1) {  // i = i0-1 and i0+TI, all j of the shell, rows along k
+            i = e < T::S1 / 2 ? i0 - 1 : i0 + TI;
+            e %= T::S1 / 2;
+            j = j0 - 1 + e / T::RK;
+            const int m = e % T::RK, base = k0 - 1;
+            k = base + 2 * m + ((c1 - (i + j + base)) & 1);
+        } else if ((e -= T::S1) < T::S2) {  // j = j0-1 and j0+TJ, i in the tile
+            j = e < T::S2 / 2 ? j0 - 1 : j0 + TJ;
+            e %= T::S2 / 2;
+            i = i0 + e / T::RK;
+            const int m = e % T::RK, base = k0 - 1;
+            k = base + 2 * m + ((c1 - (i + j + base)) & 1);
+        } else {  // k = k0-1 and k0+TK, i and j in the tile, rows along j
+            e -= T::S2;
+            k = e < T::S3 / 2 ? k0 - 1 : k0 + TK;
+            e %= T::S3 / 2;
+            i = i0 + e / (TJ / 2);
+            const int m = e % (TJ / 2);
+            j = j0 + 2 * m + ((c1 - (i + j0 + k)) & 1);
+        }
+        sok[r] = (tid + r * T::NT) < T::NS && i >= 0 && j >= 0 && k >= 0 && i < d.nx && j < d.ny && k < d.nz;
+        sc[r] = lidx(i, j, k);
+        if (sok[r]) {
+            sf[r][0] = to_f32(bx[face_index<0>(d, i, j, k)]);
+            sf[r][1] = to_f32(bx[face_index<0>(d, i + 1, j, k)]);
+            sf[r][2] = to_f32(by[face_index<1>(d, i, j, k)]);
+            sf[r][3] = to_f32(by[face_index<1>(d, i, j + 1, k)]);
+            sf[r][4] = to_f32(bz[face_index<2>(d, i, j, k)]);
+            sf[r][5] = to_f32(bz[face_index<2>(d, i, j, k + 1)]);
+            sb[r] = b[cell_index(d, i, j, k)];
+        }
+    }
+    const bool skip = done && *done;
+
+    // staged pairs into LDS (row-major pairs: LDS float index = 2 * pair
+    // index); past either end of a row both entries clamp to the end cell
+#pragma unroll
+    for (int r = 0; r < T::SPT; ++r) {
+        const int e = tid + r * T::NT;
+        if (e < T::NSP) {
+            const int kk = k0 - 2 + 2 * (e % T::NPR);
+            float2 v = sv[r];
+            if (kk < 0) v.y = v.x;
+            if (kk > d.nz - 2) v.x = v.y;
+            *reinterpret_cast<float2*>(&P[2 * e]) = v;
+        }
+    }
+    if (ZINIT)
+        for (int e = tid; e < T::NP; e += T::NT) P[e] = 0.0f;
+    __syncthreads();
+
+    // first colour on the tile
+#pragma unroll
+    for (int q = 0; q < T::PPT; ++q) {
+        if (!ok[q]) continue;
+        const int pq = tid + q * T::NT;
+        const int jl = (pq / (TK / 2)) % TJ, il = pq / ((TK / 2) * TJ);
+        const int h = ((i0 + il + j0 + jl) & 1) == c1 ? 0 : 1;  // which cell of the pair has colour c1
+        const int c = li[q] + h;
+        const float fc[6] = {f[q][h], f[q][2 + h], f[q][4 + h], f[q][6 + h], f[q][8 + h], f[q][9 + h]};
+        float v;
+        if (ZINIT) {
+            const float dg = fc[0] + fc[1] + fc[2] + fc[3] + fc[4] + fc[5];
+            v = dg > 0.0f ? rb_first(bb[q][h], dg, omega) : 0.0f;
+        } else {
+            const float pc = P[c];
+            const float pn[6] = {P[c - SI], P[c + SI], P[c - SJ], P[c + SJ], P[c - 1], P[c + 1]};
+            float dg, Ap;
+            stencil_vals(pc, fc, pn, dg, Ap);
+            v = dg > 0.0f ? rb_relax(pc, bb[q][h], Ap, dg, omega) : pc;
+        }
+        P[c] = round_st<ST>(v);
+    }
+    // first colour on the halo shell
+#pragma unroll
+    for (int r = 0; r < T::SHT; ++r) {
+        if (!sok[r]) continue;
+        const int c = sc[r];
+        const float fc[6] = {sf[r][0], sf[r][1], sf[r][2], sf[r][3], sf[r][4], sf[r][5]};
+        float v;
+        if (ZINIT) {
+            const float dg = fc[0] + fc[1] + fc[2] + fc[3] + fc[4] + fc[5];
+            v = dg > 0.0f ? rb_first(sb[r], dg, omega) : 0.0f;
+        } else {
+            const float pc = P[c];
+            const float pn[6] = {P[c - SI], P[c + SI], P[c - SJ], P[c + SJ], P[c - 1], P[c + 1]};
+            float dg, Ap;
+            stencil_vals(pc, fc, pn, dg, Ap);
+            v = dg > 0.0f ? rb_relax(pc, sb[r], Ap, dg, omega) : pc;
+        }
+        P[c] = round_st<ST>(v);
+    }
+    __syncthreads();
+
+    // second colour on the tile, then the pair goes out
+#pragma unroll
+    for (int q = 0; q < T::PPT; ++q) {
+        if (!ok[q]) continue;
+        const int pq = tid + q * T::NT;
+        const int kp = pq % (TK / 2), jl = (pq / (TK / 2)) % TJ, il = pq / ((TK / 2) * TJ);
+        const int h = ((i0 + il + j0 + jl) & 1) == c1 ? 1 : 0;  // the cell of the other colour
+        const int c = li[q] + h;
+        const float fc[6] = {f[q][h], f[q][2 + h], f[q][4 + h], f[q][6 + h], f[q][8 + h], f[q][9 + h]};
+        const float pc = P[c];
+        const float pn[6] = {P[c - SI], P[c + SI], P[c - SJ], P[c + SJ], P[c - 1], P[c + 1]};
+        float dg, Ap;
+        stencil_vals(pc, fc, pn, dg, Ap);
+        const float v = dg > 0.0f ? rb_relax(pc, bb[q][h], Ap, dg, omega) : pc;
+        const float other = P[li[q] + (1 - h)];
+        if (!skip)
+            st_pair(pout, cell_index(d, i0 + il, j0 + jl, k0 + 2 * kp), h ? other : v, h ? v : other);
     }
 }
 
@@ -239,15 +501,16 @@ __global__ __launch_bounds__(RT) void k_resid_restrict(Dims d, Dims dc, const ST
 }
 
 // P: piecewise-constant injection added onto the fine level (poisson.py:178-182)
+// (pin and pout may be the same array)
 template <typename ST, typename CT>
-__global__ __launch_bounds__(RT) void k_prolong(Dims d, Dims dc, ST* p, const CT* __restrict__ pc,
-                                                const int* __restrict__ done) {
+__global__ __launch_bounds__(RT) void k_prolong(Dims d, Dims dc, const ST* pin, ST* pout,
+                                                const CT* __restrict__ pc, const int* __restrict__ done) {
     const bool skip = done && *done;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)d.nx * d.ny * d.nz) return;
     const int k = (int)(t % d.nz), j = (int)((t / d.nz) % d.ny), i = (int)(t / ((int64_t)d.nz * d.ny));
-    const float v = to_f32(p[t]) + to_f32(pc[cell_index(dc, i / 2, j / 2, k / 2)]);
-    if (!skip) p[t] = from_f32<ST>(v);
+    const float v = to_f32(pin[t]) + to_f32(pc[cell_index(dc, i / 2, j / 2, k / 2)]);
+    if (!skip) pout[t] = from_f32<ST>(v);
 }
 
 // ================================================================ reductions
@@ -286,16 +549,25 @@ struct SmallArgs {
     int s, L, pre, post, coarsest;
     float omega;
     Dims d[PF_MAX_LEVELS];
-    const float* bx[PF_MAX_LEVELS];
-    const float* by[PF_MAX_LEVELS];
-    const float* bz[PF_MAX_LEVELS];
     int off[PF_MAX_LEVELS][5];  // LDS float offsets of bx, by, bz, p, b
-    const float* b_top;
+    // global -> LDS copies at entry: segment q copies seg_end[q] - seg_end[q-1]
+    // floats from seg_src[q] to LDS offset seg_dst[q] (the face arrays of every
+    // level, then the top level's b)
+    int nseg;
+    const float* seg_src[3 * PF_MAX_LEVELS + 1];
+    int seg_dst[3 * PF_MAX_LEVELS + 1];
+    int seg_end[3 * PF_MAX_LEVELS + 1];
     void* p_top;
     const int* done;
 };
 
-constexpr int kWaveLevelCells = 512;  // levels this small are smoothed by one wave
+constexpr int kSmallThreads = 1024;
+constexpr int kSlots = 4;  // cells of one colour per thread and level
+
+// Size of the per-colour enumeration of a level (pairs along k); a level
+// fits the single-workgroup V-cycle if it is at most kSmallThreads*kSlots,
+// and is smoothed by one wave if it is at most 64*kSlots.
+__host__ __device__ inline int colour_space(const Dims& d) { return d.nx * d.ny * ((d.nz + 1) >> 1); }
 
 struct LdsLevel {
     Dims d;
@@ -317,71 +589,139 @@ __device__ inline LdsLevel lds_level(const SmallArgs& a, float* lds, int l) {
     return v;
 }
 
-__device__ inline void lds_half_sweep(const LdsLevel& v, int colour, float omega) {
+// The cells of one colour that a thread updates on one level, decoded once
+// per level: LDS offsets of the cell (and of its y / z low faces; the x low
+// face shares the cell's offset), which of the 6 neighbours lie inside the
+// grid (a missing neighbour is the cell itself, as the reference clamps),
+// and the cell's right-hand side.
+struct Slots {
+    int c[kSlots], fy[kSlots], fz[kSlots];
+    unsigned m[kSlots];
+    float b[kSlots];
+};
+
+__device__ inline void make_slots(const LdsLevel& v, int colour, int t0, int nth, Slots& S) {
     const Dims& d = v.d;
-    const int nkp = (d.nz + 1) >> 1;
-    const int n = d.nx * d.ny * nkp;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const int kp = t % nkp, row = t / nkp;
-        const int j = row % d.ny, i = row / d.ny;
-        const int k = 2 * kp + ((i + j + colour) & 1);
-        if (k >= d.nz) continue;
-        float dg, Ap, pc;
-        stencil(d, v.bx, v.by, v.bz, v.p, i, j, k, dg, Ap, pc);
-        if (dg <= 0.0f) continue;
-        const int c = cell_index(d, i, j, k);
-        v.p[c] = pc + omega * (v.b[c] - Ap) / dg;
+    const int nkp = (d.nz + 1) >> 1, n = d.nx * d.ny * nkp;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const int t = t0 + s * nth;
+        S.c[s] = -1;
+        S.fy[s] = S.fz[s] = 0;
+        S.m[s] = 0u;
+        S.b[s] = 0.0f;
+        if (t < n) {
+            const int kp = t % nkp, row = t / nkp;
+            const int j = row % d.ny, i = row / d.ny;
+            const int k = 2 * kp + ((i + j + colour) & 1);
+            if (k < d.nz) {
+                const int c = cell_index(d, i, j, k);
+                S.c[s] = c;
+                S.fy[s] = c + i * d.nz;                     // face_index<1>(d, i, j, k)
+                S.fz[s] = c + (i * d.ny + j);               // face_index<2>(d, i, j, k)
+                S.m[s] = (unsigned)(i > 0) | ((unsigned)(i < d.nx - 1) << 1) | ((unsigned)(j > 0) << 2) |
+                         ((unsigned)(j < d.ny - 1) << 3) | ((unsigned)(k > 0) << 4) |
+                         ((unsigned)(k < d.nz - 1) << 5);
+                S.b[s] = v.b[c];
+            }
+        }
     }
-    __syncthreads();
 }
 
-// Half-sweep run by wave 0 alone (levels of at most a few hundred cells).
-// LDS operations of one wave execute in issue order, so a wave-level
-// barrier separates the colours without a workgroup barrier.
-__device__ inline void wave_half_sweep(const LdsLevel& v, int colour, float omega) {
+// One half-sweep over a thread's slots: every read first, then every write
+// (cells of one colour never neighbour each other).
+__device__ inline void slots_sweep(const LdsLevel& v, const Slots& S, float omega) {
     const Dims& d = v.d;
-    const int nkp = (d.nz + 1) >> 1;
-    const int n = d.nx * d.ny * nkp;
-    for (int t = threadIdx.x; t < n; t += 64) {
-        const int kp = t % nkp, row = t / nkp;
-        const int j = row % d.ny, i = row / d.ny;
-        const int k = 2 * kp + ((i + j + colour) & 1);
-        if (k >= d.nz) continue;
-        float dg, Ap, pc;
-        stencil(d, v.bx, v.by, v.bz, v.p, i, j, k, dg, Ap, pc);
-        if (dg <= 0.0f) continue;
-        const int c = cell_index(d, i, j, k);
-        v.p[c] = pc + omega * (v.b[c] - Ap) / dg;
+    const int SI = d.ny * d.nz, SJ = d.nz;
+    float nv[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        nv[s] = 0.0f;
+        if (S.c[s] < 0) continue;
+        const int c = S.c[s];
+        const unsigned m = S.m[s];
+        const float pc = v.p[c];
+        const float pn[6] = {v.p[c - ((m & 1u) ? SI : 0)],  v.p[c + ((m & 2u) ? SI : 0)],
+                             v.p[c - ((m & 4u) ? SJ : 0)],  v.p[c + ((m & 8u) ? SJ : 0)],
+                             v.p[c - ((m & 16u) ? 1 : 0)], v.p[c + ((m & 32u) ? 1 : 0)]};
+        const float f[6] = {v.bx[c], v.bx[c + SI], v.by[S.fy[s]], v.by[S.fy[s] + SJ], v.bz[S.fz[s]],
+                            v.bz[S.fz[s] + 1]};
+        float dg, Ap;
+        stencil_vals(pc, f, pn, dg, Ap);
+        nv[s] = dg > 0.0f ? rb_relax(pc, S.b[s], Ap, dg, omega) : pc;
     }
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s)
+        if (S.c[s] >= 0) v.p[S.c[s]] = nv[s];
+}
+
+__device__ inline void wave_sync() {
+    // LDS operations of one wave execute in issue order; this only keeps the
+    // compiler from moving LDS accesses across the colour boundary
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ inline void lds_smooth(const LdsLevel& v, int sweeps, bool reverse, float omega) {
-    for (int s = 0; s < sweeps; ++s) {
-        lds_half_sweep(v, reverse ? 1 : 0, omega);
-        lds_half_sweep(v, reverse ? 0 : 1, omega);
+// `sweeps` red-black sweeps of a level, colour order (0,1) or (1,0) if
+// reverse.  Levels whose colour enumeration fits one wave's slots are
+// smoothed by wave 0 alone (no workgroup barriers); the caller synchronises
+// afterwards.
+__device__ inline void small_smooth(const LdsLevel& v, int sweeps, bool reverse, float omega) {
+    const int tid = threadIdx.x;
+    Slots S0, S1;
+    if (colour_space(v.d) <= 64 * kSlots) {
+        if (tid >= 64) return;
+        make_slots(v, reverse ? 1 : 0, tid, 64, S0);
+        make_slots(v, reverse ? 0 : 1, tid, 64, S1);
+        for (int sw = 0; sw < sweeps; ++sw) {
+            slots_sweep(v, S0, omega);
+            wave_sync();
+            slots_sweep(v, S1, omega);
+            wave_sync();
+        }
+        return;
+    }
+    make_slots(v, reverse ? 1 : 0, tid, kSmallThreads, S0);
+    make_slots(v, reverse ? 0 : 1, tid, kSmallThreads, S1);
+    for (int sw = 0; sw < sweeps; ++sw) {
+        slots_sweep(v, S0, omega);
+        __syncthreads();
+        slots_sweep(v, S1, omega);
+        __syncthreads();
     }
 }
 
+// The whole V-cycle below `s` in one workgroup, every level in LDS.
 template <typename OT>
-__global__ __launch_bounds__(1024) void k_small_vcycle(SmallArgs a) {
+__global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
     extern __shared__ float lds[];
     if (a.done && *a.done) return;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int l = a.s; l < a.L; ++l) {
-        const LdsLevel v = lds_level(a, lds, l);
-        const Dims d = v.d;
-        const int nfx = (d.nx + 1) * d.ny * d.nz, nfy = d.nx * (d.ny + 1) * d.nz, nfz = d.nx * d.ny * (d.nz + 1);
-        for (int e = tid; e < nfx; e += nt) v.bx[e] = a.bx[l][e];
-        for (int e = tid; e < nfy; e += nt) v.by[e] = a.by[l][e];
-        for (int e = tid; e < nfz; e += nt) v.bz[e] = a.bz[l][e];
-    }
+    const int tid = threadIdx.x, nt = kSmallThreads;
+    // copy faces and the top b into LDS, eight loads in flight per thread
     {
-        const LdsLevel v = lds_level(a, lds, a.s);
-        const int n = v.d.nx * v.d.ny * v.d.nz;
-        for (int e = tid; e < n; e += nt) v.b[e] = a.b_top[e];
+        constexpr int U = 8;
+        const int total = a.seg_end[a.nseg - 1];
+        for (int base = 0; base < total; base += U * nt) {
+            float v[U];
+            int dst[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = base + u * nt + tid;
+                dst[u] = -1;
+                v[u] = 0.0f;
+                if (e < total) {
+                    int q = 0;
+                    while (e >= a.seg_end[q]) ++q;
+                    const int lo = q ? a.seg_end[q - 1] : 0;
+                    v[u] = a.seg_src[q][e - lo];
+                    dst[u] = a.seg_dst[q] + (e - lo);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (dst[u] >= 0) lds[dst[u]] = v[u];
+        }
     }
     __syncthreads();
     // descend
@@ -391,7 +731,8 @@ __global__ __launch_bounds__(1024) void k_small_vcycle(SmallArgs a) {
         const int n = v.d.nx * v.d.ny * v.d.nz;
         for (int e = tid; e < n; e += nt) v.p[e] = 0.0f;
         __syncthreads();
-        lds_smooth(v, a.pre, false, a.omega);
+        small_smooth(v, a.pre, false, a.omega);
+        __syncthreads();
         const int nc = c.d.nx * c.d.ny * c.d.nz;
         for (int t = tid; t < nc; t += nt) {
             const int K = t % c.d.nz, J = (t / c.d.nz) % c.d.ny, I = t / (c.d.nz * c.d.ny);
@@ -414,31 +755,17 @@ __global__ __launch_bounds__(1024) void k_small_vcycle(SmallArgs a) {
         const int n = v.d.nx * v.d.ny * v.d.nz;
         for (int e = tid; e < n; e += nt) v.p[e] = 0.0f;
         __syncthreads();
-        if (n <= kWaveLevelCells) {
-            if (tid < 64) {
-                for (int sw = 0; sw < a.coarsest; ++sw) {
-                    wave_half_sweep(v, 0, a.omega);
-                    wave_half_sweep(v, 1, a.omega);
-                }
-                double sum = 0.0;
-                for (int e = tid; e < n; e += 64) sum += (double)v.p[e];
-                sum = wave_sum(sum);
-                const float shift = (float)(-(sum / (double)n));
-                for (int e = tid; e < n; e += 64) v.p[e] = v.p[e] + shift;
-            }
-            __syncthreads();
-        } else {
-            lds_smooth(v, a.coarsest, false, a.omega);
-            __shared__ double sh[16];
-            double s[1] = {0.0};
-            for (int e = tid; e < n; e += nt) s[0] += (double)v.p[e];
-            block_sum<1>(s, sh);
-            __shared__ float shift;
-            if (tid == 0) shift = (float)(-(s[0] / (double)n));
-            __syncthreads();
-            for (int e = tid; e < n; e += nt) v.p[e] = v.p[e] + shift;
-            __syncthreads();
-        }
+        small_smooth(v, a.coarsest, false, a.omega);
+        __syncthreads();
+        __shared__ double sh[16];
+        double sm[1] = {0.0};
+        for (int e = tid; e < n; e += nt) sm[0] += (double)v.p[e];
+        block_sum<1>(sm, sh);
+        __shared__ float shift;
+        if (tid == 0) shift = (float)(-(sm[0] / (double)n));
+        __syncthreads();
+        for (int e = tid; e < n; e += nt) v.p[e] = v.p[e] + shift;
+        __syncthreads();
     }
     // ascend
     for (int l = a.L - 2; l >= a.s; --l) {
@@ -450,7 +777,8 @@ __global__ __launch_bounds__(1024) void k_small_vcycle(SmallArgs a) {
             v.p[t] = v.p[t] + c.p[cell_index(c.d, i / 2, j / 2, k / 2)];
         }
         __syncthreads();
-        lds_smooth(v, a.post, true, a.omega);
+        small_smooth(v, a.post, true, a.omega);
+        __syncthreads();
     }
     {
         const LdsLevel v = lds_level(a, lds, a.s);
@@ -472,14 +800,15 @@ struct CGK {
     float* rhs;
     float* x;
     float* r;
-    float* q;
+    float* q;            // search direction written by this iteration
+    const float* q_old;  // the previous one (the two alternate between iterations)
     float* Ap;
     const ZT* z;
     CGState* st;
     double tol;
 };
 
-// rhs = b - mean(b); r = rhs; x = 0; partial max|rhs| (poisson.py:363-372)
+// r = b - mean(b); x = 0; q = 0; partial max|r| (poisson.py:363-372)
 template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_init(CGK<ZT> a) {
     __shared__ float sh[4];
@@ -487,9 +816,9 @@ __global__ __launch_bounds__(RT) void k_cg_init(CGK<ZT> a) {
     float m = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
         const float v = a.b[i] + c;
-        a.rhs[i] = v;
         a.r[i] = v;
         a.x[i] = 0.0f;
+        a.q[i] = 0.0f;  // the first iteration's previous direction (beta = 0)
         m = fmaxf(m, fabsf(v));
     }
     m = block_max(m, sh);
@@ -513,29 +842,34 @@ __global__ __launch_bounds__(RT) void k_cg_first(CGK<ZT> a) {
         for (int q = 0; q < 3; ++q) a.st->part[q * MAXB + blockIdx.x] = v[q];
 }
 
-// q = (z - mu) + beta q  (first call: beta = 0)
-template <typename ZT>
-__global__ __launch_bounds__(RT) void k_cg_q(CGK<ZT> a, int first) {
-    if (a.st->done) return;
-    const float c = (float)(-a.st->mu);
-    const float be = first ? 0.0f : (float)a.st->beta;
-    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
-        const float zz = to_f32(a.z[i]) + c;
-        a.q[i] = first ? zz : zz + be * a.q[i];
-    }
-}
-
-// Ap = A q; partial <q, Ap> (poisson.py:385-386)
+// q = (z - mu) + beta q_old (the search direction, poisson.py:406; beta = 0
+// on the first iteration), evaluated at the cell and its six neighbours;
+// then Ap = A q and the partial <q, Ap> (poisson.py:385-386).  Writes q and Ap
+// of the cell; q_old is a different buffer, so neighbours read old values.
 template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_apply(CGK<ZT> a) {
     __shared__ double sh[16];
     if (a.st->done) return;
+    const float cm = (float)(-a.st->mu);
+    const float be = (float)a.st->beta;
     double v[1] = {0.0};
     const Dims d = a.d;
+    auto qn = [&](int c) {
+        const float zz = to_f32(a.z[c]) + cm;
+        return zz + be * a.q_old[c];
+    };
     for (int64_t c = (int64_t)blockIdx.x * RT + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * RT) {
         const int k = (int)(c % d.nz), j = (int)((c / d.nz) % d.ny), i = (int)(c / ((int64_t)d.nz * d.ny));
-        float dg, Ap, pc;
-        stencil(d, a.bx, a.by, a.bz, a.q, i, j, k, dg, Ap, pc);
+        const float f[6] = {a.bx[face_index<0>(d, i, j, k)], a.bx[face_index<0>(d, i + 1, j, k)],
+                            a.by[face_index<1>(d, i, j, k)], a.by[face_index<1>(d, i, j + 1, k)],
+                            a.bz[face_index<2>(d, i, j, k)], a.bz[face_index<2>(d, i, j, k + 1)]};
+        const float pc = qn((int)c);
+        const float pn[6] = {qn(cell_index(d, max(i - 1, 0), j, k)), qn(cell_index(d, min(i + 1, d.nx - 1), j, k)),
+                             qn(cell_index(d, i, max(j - 1, 0), k)), qn(cell_index(d, i, min(j + 1, d.ny - 1), k)),
+                             qn(cell_index(d, i, j, max(k - 1, 0))), qn(cell_index(d, i, j, min(k + 1, d.nz - 1)))};
+        float dg, Ap;
+        stencil_vals(pc, f, pn, dg, Ap);
+        a.q[c] = pc;
         a.Ap[c] = Ap;
         v[0] += (double)pc * (double)Ap;
     }
@@ -621,6 +955,7 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
             const double mu = t[0] / (double)n;
             st->mu = mu;
             st->rz = t[2] - mu * t[1];
+            st->beta = 0.0;
         }
     } else if (op == C_APPLY) {
         // qAq <= 0 ends the solve before any update (poisson.py:386-389)
@@ -738,6 +1073,38 @@ static int ensure_small_lds() {
     return PF_OK;
 }
 
+// Fused full sweep on one level: tile shape by level size (big tiles cut
+// the redundant halo work, small ones keep enough workgroups in flight).
+template <typename ST, bool ZINIT>
+static int launch_full_sweep(const Dims& d, const ST* fx, const ST* fy, const ST* fz, const ST* pin,
+                             ST* pout, const float* b, int c1, float omega, const int* done,
+                             hipStream_t s) {
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    auto tiles = [&](int ti, int tj, int tk) {
+        return (unsigned)(cdiv(d.nx, ti) * cdiv(d.ny, tj) * cdiv(d.nz, tk));
+    };
+    if (n >= (int64_t)1 << 20)
+        k_rbgs_full<ST, ZINIT, 8, 8, 32><<<tiles(8, 8, 32), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done);
+    else if (n >= (int64_t)1 << 17)
+        k_rbgs_full<ST, ZINIT, 8, 8, 16><<<tiles(8, 8, 16), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done);
+    else
+        k_rbgs_full<ST, ZINIT, 4, 8, 16><<<tiles(4, 8, 16), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+// The two iterate buffers of an HBM level: the level's own (level 0: the
+// output z0) and the fused smoother's second buffer.
+template <typename ST>
+static ST* second_buffer(const pf_mg& mg, int l) {
+    if (sizeof(ST) == 4 && l == 0) return reinterpret_cast<ST*>(mg.lev[0].p);
+    return reinterpret_cast<ST*>(mg.lev[l].r);
+}
+
+static bool fused_smoother(const pf_mg& mg, int l) {
+    return !(mg.options & PF_MG_UNFUSED_SMOOTHER) && (mg.lev[l].nz % 2) == 0;
+}
+
 // One V-cycle: z0 = M^{-1} r0 (no mean removal at level 0).
 template <typename ST>
 static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hipStream_t s) {
@@ -746,26 +1113,42 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
     const int G = sm < L ? sm : L - 1;        // levels 0..G-1 run out of HBM with restriction
     auto P = [&](int l) -> ST* { return l == 0 ? z0 : Lv<ST>::p(mg.lev[l]); };
     auto B = [&](int l) -> const float* { return l == 0 ? r0 : mg.lev[l].b; };
+    ST* cur[PF_MAX_LEVELS];  // buffer holding each HBM level's current iterate
+    auto other = [&](int l, const ST* x) -> ST* { return x == P(l) ? second_buffer<ST>(mg, l) : P(l); };
 
+    // `sweeps` red-black sweeps of level l (colour order (0,1), or (1,0) if
+    // reverse), starting from zero if from_zero, else from cur[l]
     auto smooth = [&](int l, int sweeps, bool reverse, bool from_zero) -> int {
         const pf_mg_level& lv = mg.lev[l];
         const Dims d = ldims(lv);
-        const int64_t nthr = (int64_t)d.nx * d.ny * ((d.nz + 1) / 2);
-        const unsigned grid = cdiv(nthr, RT);
         const ST *fx = Lv<ST>::fx(lv), *fy = Lv<ST>::fy(lv), *fz = Lv<ST>::fz(lv);
-        ST* p = P(l);
         const float* b = B(l);
+        if (from_zero) cur[l] = P(l);
         if (from_zero && sweeps == 0) {
-            k_zero<ST><<<red_blocks(ncell(lv)), RT, 0, s>>>(p, ncell(lv), done);
+            k_zero<ST><<<red_blocks(ncell(lv)), RT, 0, s>>>(cur[l], ncell(lv), done);
             PF_CHECK_LAUNCH();
         }
+        if (fused_smoother(mg, l)) {
+            for (int sw = 0; sw < sweeps; ++sw) {
+                ST* out = other(l, cur[l]);
+                const int c1 = reverse ? 1 : 0;
+                const int rc = (from_zero && sw == 0)
+                                   ? launch_full_sweep<ST, true>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done, s)
+                                   : launch_full_sweep<ST, false>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done, s);
+                if (rc) return rc;
+                cur[l] = out;
+            }
+            return PF_OK;
+        }
+        const int64_t nthr = (int64_t)d.nx * d.ny * ((d.nz + 1) / 2);
+        const unsigned grid = cdiv(nthr, RT);
         for (int sw = 0; sw < sweeps; ++sw) {
             for (int h = 0; h < 2; ++h) {
                 const int colour = reverse ? 1 - h : h;
                 if (from_zero && sw == 0 && h == 0)
-                    k_rbgs<ST, true><<<grid, RT, 0, s>>>(d, fx, fy, fz, p, b, colour, mg.omega, done);
+                    k_rbgs<ST, true><<<grid, RT, 0, s>>>(d, fx, fy, fz, cur[l], b, colour, mg.omega, done);
                 else
-                    k_rbgs<ST, false><<<grid, RT, 0, s>>>(d, fx, fy, fz, p, b, colour, mg.omega, done);
+                    k_rbgs<ST, false><<<grid, RT, 0, s>>>(d, fx, fy, fz, cur[l], b, colour, mg.omega, done);
                 PF_CHECK_LAUNCH();
             }
         }
@@ -778,7 +1161,7 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
         const pf_mg_level& lv = mg.lev[l];
         const pf_mg_level& cv = mg.lev[l + 1];
         k_resid_restrict<ST><<<cdiv(ncell(cv), RT), RT, 0, s>>>(ldims(lv), ldims(cv), Lv<ST>::fx(lv),
-                                                              Lv<ST>::fy(lv), Lv<ST>::fz(lv), P(l), B(l),
+                                                              Lv<ST>::fy(lv), Lv<ST>::fz(lv), cur[l], B(l),
                                                               mg.lev[l + 1].b, done);
         PF_CHECK_LAUNCH();
     }
@@ -792,22 +1175,31 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
         a.post = mg.post;
         a.coarsest = mg.coarsest;
         a.omega = mg.omega;
-        for (int l = sm; l < L; ++l) {
-            a.d[l] = ldims(mg.lev[l]);
-            a.bx[l] = mg.lev[l].bx;
-            a.by[l] = mg.lev[l].by;
-            a.bz[l] = mg.lev[l].bz;
-        }
+        for (int l = sm; l < L; ++l) a.d[l] = ldims(mg.lev[l]);
         const int64_t nf = small_lds_floats(mg, sm, a.off);
-        a.b_top = B(sm);
+        a.nseg = 0;
+        int end = 0;
+        auto seg = [&](const float* src, int dst, int64_t n) {
+            a.seg_src[a.nseg] = src;
+            a.seg_dst[a.nseg] = dst;
+            end += (int)n;
+            a.seg_end[a.nseg++] = end;
+        };
+        for (int l = sm; l < L; ++l) {
+            const Dims d = a.d[l];
+            seg(mg.lev[l].bx, a.off[l][0], face_count(d, 0));
+            seg(mg.lev[l].by, a.off[l][1], face_count(d, 1));
+            seg(mg.lev[l].bz, a.off[l][2], face_count(d, 2));
+        }
+        seg(B(sm), a.off[sm][4], ncell(mg.lev[sm]));
         a.done = done;
         const size_t lds = (size_t)nf * sizeof(float);
         if (sm == 0) {
             a.p_top = z0;
-            k_small_vcycle<ST><<<1, 1024, lds, s>>>(a);
+            k_small_vcycle<ST><<<1, kSmallThreads, lds, s>>>(a);
         } else {
             a.p_top = mg.lev[sm].p;
-            k_small_vcycle<float><<<1, 1024, lds, s>>>(a);
+            k_small_vcycle<float><<<1, kSmallThreads, lds, s>>>(a);
         }
         PF_CHECK_LAUNCH();
     } else {
@@ -818,21 +1210,27 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
         CGState* st = static_cast<CGState*>(mg.scalars);
         const int64_t n = ncell(mg.lev[l]);
         // p is ST; the mean is taken in FP64 and the shifted value stored back
-        k_part_sum<ST><<<red_blocks(n), RT, 0, s>>>(P(l), n, st->part);
+        k_part_sum<ST><<<red_blocks(n), RT, 0, s>>>(cur[l], n, st->part);
         PF_CHECK_LAUNCH();
         k_comb<<<1, RT, 0, s>>>(st, C_MEAN, (int)red_blocks(n), n, &st->mu_x, 0.0);
         PF_CHECK_LAUNCH();
-        k_shift<ST, ST><<<red_blocks(n), RT, 0, s>>>(P(l), P(l), n, &st->mu_x);
+        k_shift<ST, ST><<<red_blocks(n), RT, 0, s>>>(cur[l], cur[l], n, &st->mu_x);
         PF_CHECK_LAUNCH();
     }
     for (int l = G - 1; l >= 0; --l) {
         const pf_mg_level& lv = mg.lev[l];
         const pf_mg_level& cv = mg.lev[l + 1];
+        // prolongation target: in place, except on level 0 where the last
+        // post-sweep must land in z0
+        ST* dst = cur[l];
+        if (l == 0) dst = (fused_smoother(mg, 0) && (mg.post % 2)) ? second_buffer<ST>(mg, 0) : z0;
+        const unsigned grid = cdiv(ncell(lv), RT);
         if (l + 1 == sm)
-            k_prolong<ST, float><<<cdiv(ncell(lv), RT), RT, 0, s>>>(ldims(lv), ldims(cv), P(l), cv.p, done);
+            k_prolong<ST, float><<<grid, RT, 0, s>>>(ldims(lv), ldims(cv), cur[l], dst, cv.p, done);
         else
-            k_prolong<ST, ST><<<cdiv(ncell(lv), RT), RT, 0, s>>>(ldims(lv), ldims(cv), P(l), P(l + 1), done);
+            k_prolong<ST, ST><<<grid, RT, 0, s>>>(ldims(lv), ldims(cv), cur[l], dst, cur[l + 1], done);
         PF_CHECK_LAUNCH();
+        cur[l] = dst;
         int rc = smooth(l, mg.post, true, false);
         if (rc) return rc;
     }
@@ -880,12 +1278,11 @@ static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int
     PF_CHECK_LAUNCH();
     k_comb<<<1, RT, 0, s>>>(st, C_BETA, (int)nb, a.n, nullptr, 0.0);
     PF_CHECK_LAUNCH();
-    k_cg_q<ST><<<nb, RT, 0, s>>>(a, 0);
-    PF_CHECK_LAUNCH();
     return PF_OK;
 }
 
-// First preconditioner application and search direction (poisson.py:378-380).
+// First preconditioner application (poisson.py:378-380); the first search
+// direction is formed inside the first iteration's k_cg_apply (beta = 0).
 template <typename ST>
 static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, unsigned nb,
                             hipStream_t s) {
@@ -894,8 +1291,6 @@ static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int*
     k_cg_first<ST><<<nb, RT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
     k_comb<<<1, RT, 0, s>>>(a.st, C_FIRST, (int)nb, a.n, nullptr, 0.0);
-    PF_CHECK_LAUNCH();
-    k_cg_q<ST><<<nb, RT, 0, s>>>(a, 1);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }
@@ -933,6 +1328,16 @@ static int capture(hipStream_t cs, F&& enqueue, hipGraphExec_t* out) {
     return PF_OK;
 }
 
+// Iteration `slot` (= iteration number & 1) writes its search direction to
+// mg.q (slot 0) or mg.rhs (slot 1) and reads the other one.
+template <typename ST>
+static CGK<ST> slot_args(const CGK<ST>& a, const pf_mg& mg, int slot) {
+    CGK<ST> b = a;
+    b.q = slot ? mg.rhs : mg.q;
+    b.q_old = slot ? mg.q : mg.rhs;
+    return b;
+}
+
 template <typename ST>
 static int get_graphs(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, HostPoll* hp,
                       unsigned nb, SolveGraphs*& out) {
@@ -958,9 +1363,11 @@ static int get_graphs(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done,
     hipStream_t cs = nullptr;
     PF_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     rc = capture(cs, [&](hipStream_t t) { return enqueue_prologue<ST>(mg, a, z, done, nb, t); }, &g.pro);
-    for (int q = 0; q < 2 && !rc; ++q)
-        rc = capture(cs, [&](hipStream_t t) { return enqueue_iteration<ST>(mg, a, z, done, &hp->flag[q], nb, t); },
+    for (int q = 0; q < 2 && !rc; ++q) {
+        const CGK<ST> aq = slot_args(a, mg, q);
+        rc = capture(cs, [&](hipStream_t t) { return enqueue_iteration<ST>(mg, aq, z, done, &hp->flag[q], nb, t); },
                      &g.it[q]);
+    }
     (void)hipStreamDestroy(cs);
     if (rc) {
         destroy_graphs(g);
@@ -1001,7 +1408,8 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
     a.rhs = mg.rhs;
     a.x = x;
     a.r = mg.res;
-    a.q = mg.q;
+    a.q = mg.q;  // zeroed by k_cg_init: iteration 1 (slot 1) reads it as q_old
+    a.q_old = mg.rhs;
     a.Ap = mg.Ap;
     a.z = z;
     a.st = st;
@@ -1044,7 +1452,7 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
         if (gr) {
             PF_CHECK_HIP(hipGraphLaunch(gr->it[slot], s));
         } else {
-            rc = enqueue_iteration<ST>(mg, a, z, done, &hp->flag[slot], nb, s);
+            rc = enqueue_iteration<ST>(mg, slot_args(a, mg, slot), z, done, &hp->flag[slot], nb, s);
             if (rc) return rc;
         }
         PF_CHECK_HIP(hipEventRecord(hp->ev[slot], s));
@@ -1110,6 +1518,11 @@ static int check_mg(const pf_mg* mg) {
         // level 0's 16-bit iterate holds the preconditioned vector z
         if (mg->precision != 0 && l == 0) PF_REQUIRE(v.p_h, "pf_mg: level 0 missing its 16-bit iterate");
         if (l > 0 && (mg->precision == 0 || l >= mg->small_from)) PF_REQUIRE(v.p, "pf_mg: level %d missing p", l);
+        // the fused smoother's second iterate buffer on the HBM levels
+        if (l < mg->small_from) {
+            PF_REQUIRE(v.r, "pf_mg: level %d missing r (second smoother buffer)", l);
+            if (l == 0 && mg->precision == 0) PF_REQUIRE(v.p, "pf_mg: level 0 missing p (second smoother buffer)");
+        }
     }
     return PF_OK;
 }
@@ -1127,7 +1540,8 @@ PF_API int pf_mg_setup(pf_mg* mg) {
     for (int l = mg->nlev - 1; l >= 0; --l) {
         const int64_t nc = (int64_t)mg->lev[l].nx * mg->lev[l].ny * mg->lev[l].nz;
         const int64_t bytes = small_lds_floats(*mg, l, nullptr) * (int64_t)sizeof(float);
-        if (nc <= kSmallMaxCells && bytes <= kSmallMaxBytes)
+        const Dims d{mg->lev[l].nx, mg->lev[l].ny, mg->lev[l].nz};
+        if (nc <= kSmallMaxCells && bytes <= kSmallMaxBytes && colour_space(d) <= kSmallThreads * kSlots)
             sm = l;
         else
             break;
@@ -1200,6 +1614,53 @@ PF_API int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol,
     }
 }
 
+template <typename ST>
+static int time_smoother_impl(const pf_mg& mg, int level, int reps, float* ms_out, hipStream_t s) {
+    const pf_mg_level& lv = mg.lev[level];
+    const Dims d = ldims(lv);
+    ST* pa = level == 0 ? (sizeof(ST) == 4 ? reinterpret_cast<ST*>(mg.z) : Lv<ST>::p(lv)) : Lv<ST>::p(lv);
+    ST* pb = second_buffer<ST>(mg, level);
+    const float* b = level == 0 ? mg.res : lv.b;
+    const bool fused = fused_smoother(mg, level);
+    const int64_t nthr = (int64_t)d.nx * d.ny * ((d.nz + 1) / 2);
+    const unsigned grid = cdiv(nthr, RT);
+    hipEvent_t e0, e1;
+    PF_CHECK_HIP(hipEventCreate(&e0));
+    PF_CHECK_HIP(hipEventCreate(&e1));
+    auto run = [&]() -> int {
+        for (int r = 0; r < reps; ++r) {
+            if (fused) {
+                const int rc = launch_full_sweep<ST, false>(d, Lv<ST>::fx(lv), Lv<ST>::fy(lv), Lv<ST>::fz(lv),
+                                                            r % 2 ? pb : pa, r % 2 ? pa : pb, b, 0, mg.omega,
+                                                            nullptr, s);
+                if (rc) return rc;
+            } else {
+                for (int c = 0; c < 2; ++c) {
+                    k_rbgs<ST, false><<<grid, RT, 0, s>>>(d, Lv<ST>::fx(lv), Lv<ST>::fy(lv), Lv<ST>::fz(lv), pa,
+                                                          b, c, mg.omega, nullptr);
+                    PF_CHECK_LAUNCH();
+                }
+            }
+        }
+        return PF_OK;
+    };
+    int rc = run();  // warm-up
+    if (!rc) {
+        PF_CHECK_HIP(hipEventRecord(e0, s));
+        rc = run();
+    }
+    if (!rc) {
+        PF_CHECK_HIP(hipEventRecord(e1, s));
+        PF_CHECK_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PF_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *ms_out = ms / (fused ? (float)reps : 2.0f * reps);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 PF_API int pf_mg_time_smoother(const pf_mg* mg, int32_t level, int32_t reps, float* ms_out,
                                pf_stream_t stream) {
     int rc = check_mg(mg);
@@ -1207,47 +1668,12 @@ PF_API int pf_mg_time_smoother(const pf_mg* mg, int32_t level, int32_t reps, flo
     PF_REQUIRE(ms_out && reps > 0, "pf_mg_time_smoother: bad arguments");
     PF_REQUIRE(level >= 0 && level < mg->nlev && level < mg->small_from,
                "pf_mg_time_smoother: level %d is not smoothed out of HBM", level);
-    hipStream_t s = S(stream);
-    const pf_mg_level& lv = mg->lev[level];
-    const Dims d = ldims(lv);
-    const int64_t nthr = (int64_t)d.nx * d.ny * ((d.nz + 1) / 2);
-    const unsigned grid = cdiv(nthr, RT);
-    hipEvent_t e0, e1;
-    PF_CHECK_HIP(hipEventCreate(&e0));
-    PF_CHECK_HIP(hipEventCreate(&e1));
-    auto run = [&]() -> int {
-        for (int r = 0; r < reps; ++r)
-            for (int c = 0; c < 2; ++c) {
-                if (mg->precision == 0) {
-                    float* pp = level == 0 ? mg->z : lv.p;
-                    k_rbgs<float, false><<<grid, RT, 0, s>>>(d, lv.bx, lv.by, lv.bz, pp,
-                                                             level == 0 ? mg->res : lv.b, c, mg->omega, nullptr);
-                } else if (mg->precision == 1) {
-                    k_rbgs<bf16s, false><<<grid, RT, 0, s>>>(d, Lv<bf16s>::fx(lv), Lv<bf16s>::fy(lv),
-                                                             Lv<bf16s>::fz(lv), Lv<bf16s>::p(lv),
-                                                             level == 0 ? mg->res : lv.b, c, mg->omega, nullptr);
-                } else {
-                    k_rbgs<fp16s, false><<<grid, RT, 0, s>>>(d, Lv<fp16s>::fx(lv), Lv<fp16s>::fy(lv),
-                                                             Lv<fp16s>::fz(lv), Lv<fp16s>::p(lv),
-                                                             level == 0 ? mg->res : lv.b, c, mg->omega, nullptr);
-                }
-                PF_CHECK_LAUNCH();
-            }
-        return PF_OK;
-    };
-    rc = run();  // warm-up
-    if (rc) return rc;
-    PF_CHECK_HIP(hipEventRecord(e0, s));
-    rc = run();
-    if (rc) return rc;
-    PF_CHECK_HIP(hipEventRecord(e1, s));
-    PF_CHECK_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    PF_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *ms_out = ms / (2.0f * reps);
-    return PF_OK;
+    PF_REQUIRE(level > 0 || mg->precision != 0 || mg->z, "pf_mg_time_smoother: missing z");
+    switch (mg->precision) {
+        case 0: return time_smoother_impl<float>(*mg, level, reps, ms_out, S(stream));
+        case 1: return time_smoother_impl<bf16s>(*mg, level, reps, ms_out, S(stream));
+        default: return time_smoother_impl<fp16s>(*mg, level, reps, ms_out, S(stream));
+    }
 }
 
 PF_API int pf_apply_A(int nx, int ny, int nz, const float* p, const float* bx, const float* by,

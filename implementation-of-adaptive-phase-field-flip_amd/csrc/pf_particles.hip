@@ -596,22 +596,27 @@ struct G2PArgs {
     float lo[3], hi[3];
 };
 
-// trilinear taps of the face grid of axis A at grid coordinates g = x/dx
-// (kernels.py:52-79: indices clamped per axis)
+// Trilinear interpolation of the face grid of axis A at grid coordinates
+// g = x/dx, indices clamped per axis (kernels.py:52-79).  The two taps along
+// k of each (i, j) row are adjacent in memory, so a sample is 4 row loads of
+// 8 bytes: the pair (kc, kc+1) with kc = clamp(k0, 0, n2-2); where both taps
+// clamp to one end of the row the pair supplies that end value twice.
 template <int A>
-struct Taps {
-    int idx[8];
-    float w[8];
+struct Rows {
+    int base[4];  // (ii * n1 + jj) * n2 + kc for (di, dj) = (0,0), (0,1), (1,0), (1,1)
+    float w[8];   // wi * wj * wk, index di*4 + dj*2 + dk
+    int mode;     // 0: taps (kc, kc+1); 1: both at 0; 2: both at n2-1
 };
 
 template <int A>
-__device__ inline Taps<A> make_taps(float gx, float gy, float gz, const Dims& d) {
-    Taps<A> t;
+__device__ inline void make_rows(float gx, float gy, float gz, const Dims& d, Rows<A>& t) {
     const float ox = A == 0 ? 0.f : 0.5f, oy = A == 1 ? 0.f : 0.5f, oz = A == 2 ? 0.f : 0.5f;
     const int n0 = d.nx + (A == 0), n1 = d.ny + (A == 1), n2 = d.nz + (A == 2);
     const float hx = gx - ox, hy = gy - oy, hz = gz - oz;
     const int i0 = (int)floorf(hx), j0 = (int)floorf(hy), k0 = (int)floorf(hz);
     const float fx = hx - (float)i0, fy = hy - (float)j0, fz = hz - (float)k0;
+    const int kc = min(max(k0, 0), n2 - 2);
+    t.mode = k0 < 0 ? 1 : (k0 > n2 - 2 ? 2 : 0);
 #pragma unroll
     for (int di = 0; di < 2; ++di) {
         const float wi = di == 0 ? 1.0f - fx : fx;
@@ -620,33 +625,48 @@ __device__ inline Taps<A> make_taps(float gx, float gy, float gz, const Dims& d)
         for (int dj = 0; dj < 2; ++dj) {
             const float wj = dj == 0 ? 1.0f - fy : fy;
             const int jj = min(max(j0 + dj, 0), n1 - 1);
+            t.base[di * 2 + dj] = (ii * n1 + jj) * n2 + kc;
 #pragma unroll
             for (int dk = 0; dk < 2; ++dk) {
                 const float wk = dk == 0 ? 1.0f - fz : fz;
-                const int kk = min(max(k0 + dk, 0), n2 - 1);
-                const int q = di * 4 + dj * 2 + dk;
-                t.w[q] = wi * wj * wk;
-                t.idx[q] = (ii * n1 + jj) * n2 + kk;
+                t.w[di * 4 + dj * 2 + dk] = wi * wj * wk;
             }
         }
     }
-    return t;
+}
+
+__device__ inline float2 ld_row_pair(const float* __restrict__ f, int idx) {
+    float2 v;
+    __builtin_memcpy(&v, f + idx, sizeof(v));  // 4-byte aligned 8-byte load
+    return v;
 }
 
 template <int A>
-__device__ inline float gather(const Taps<A>& t, const float* __restrict__ f) {
+__device__ inline float sample_rows(const Rows<A>& t, const float* __restrict__ f) {
     float acc = 0.f;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc += t.w[q] * f[t.idx[q]];
+    for (int r = 0; r < 4; ++r) {
+        const float2 v = ld_row_pair(f, t.base[r]);
+        const float v0 = t.mode == 2 ? v.y : v.x;
+        const float v1 = t.mode == 1 ? v.x : v.y;
+        acc += t.w[2 * r] * v0;
+        acc += t.w[2 * r + 1] * v1;
+    }
     return acc;
 }
 
 __device__ inline void sample_vel(const float* const* f, float x, float y, float z, float dx,
                                   const Dims& d, float& u, float& v, float& w) {
     const float gx = x / dx, gy = y / dx, gz = z / dx;
-    u = gather(make_taps<0>(gx, gy, gz, d), f[0]);
-    v = gather(make_taps<1>(gx, gy, gz, d), f[1]);
-    w = gather(make_taps<2>(gx, gy, gz, d), f[2]);
+    Rows<0> t0;
+    Rows<1> t1;
+    Rows<2> t2;
+    make_rows(gx, gy, gz, d, t0);
+    make_rows(gx, gy, gz, d, t1);
+    make_rows(gx, gy, gz, d, t2);
+    u = sample_rows(t0, f[0]);
+    v = sample_rows(t1, f[1]);
+    w = sample_rows(t2, f[2]);
 }
 
 __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
@@ -669,39 +689,52 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
     float vx = a.pv[3 * p], vy = a.pv[3 * p + 1], vz = a.pv[3 * p + 2];
     const Dims d = a.d;
     const float dx = a.dx;
+    const bool g2p = active && (a.flags & PF_GA_G2P);
+    const bool adv = active && (a.flags & PF_GA_ADVECT);
 
-    float pic[3] = {0.f, 0.f, 0.f};
-    bool have_pic = false;
-    if (active && (a.flags & PF_GA_G2P)) {
-        // eq. (12): v = alpha (v + I(du)) + (1 - alpha) I(u)
+    // samples at x share one set of rows: PIC I(vel), FLIP I(dvel), and the
+    // first RK3 stage (liquid: I(vel_liq); air: I(vel) = the PIC sample)
+    float k1x = 0.f, k1y = 0.f, k1z = 0.f;
+    if (g2p || adv) {
         const float gx = x / dx, gy = y / dx, gz = z / dx;
-        const Taps<0> t0 = make_taps<0>(gx, gy, gz, d);
-        const Taps<1> t1 = make_taps<1>(gx, gy, gz, d);
-        const Taps<2> t2 = make_taps<2>(gx, gy, gz, d);
-        const float dux = gather(t0, a.dvel[0]), duy = gather(t1, a.dvel[1]),
-                    duz = gather(t2, a.dvel[2]);
-        pic[0] = gather(t0, a.vel[0]);
-        pic[1] = gather(t1, a.vel[1]);
-        pic[2] = gather(t2, a.vel[2]);
-        have_pic = true;
-        const float al = liquid ? a.alpha_l : a.alpha_a;
-        vx = al * (vx + dux) + (1.0f - al) * pic[0];
-        vy = al * (vy + duy) + (1.0f - al) * pic[1];
-        vz = al * (vz + duz) + (1.0f - al) * pic[2];
+        Rows<0> t0;
+        Rows<1> t1;
+        Rows<2> t2;
+        make_rows(gx, gy, gz, d, t0);
+        make_rows(gx, gy, gz, d, t1);
+        make_rows(gx, gy, gz, d, t2);
+        float pic[3] = {0.f, 0.f, 0.f};
+        if (g2p || !liquid) {
+            pic[0] = sample_rows(t0, a.vel[0]);
+            pic[1] = sample_rows(t1, a.vel[1]);
+            pic[2] = sample_rows(t2, a.vel[2]);
+        }
+        if (adv) {
+            if (liquid) {
+                k1x = sample_rows(t0, a.vel_liq[0]);
+                k1y = sample_rows(t1, a.vel_liq[1]);
+                k1z = sample_rows(t2, a.vel_liq[2]);
+            } else {
+                k1x = pic[0];
+                k1y = pic[1];
+                k1z = pic[2];
+            }
+        }
+        if (g2p) {
+            // eq. (12): v = alpha (v + I(du)) + (1 - alpha) I(u)
+            const float dux = sample_rows(t0, a.dvel[0]), duy = sample_rows(t1, a.dvel[1]),
+                        duz = sample_rows(t2, a.dvel[2]);
+            const float al = liquid ? a.alpha_l : a.alpha_a;
+            vx = al * (vx + dux) + (1.0f - al) * pic[0];
+            vy = al * (vy + duy) + (1.0f - al) * pic[1];
+            vz = al * (vz + duz) + (1.0f - al) * pic[2];
+        }
     }
-    if (active && (a.flags & PF_GA_ADVECT)) {
+    if (adv) {
         // Ralston RK3 (kernels.py:508-549); liquid samples vel_liq, air vel
         const float* f[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) f[q] = liquid ? a.vel_liq[q] : a.vel[q];
-        float k1x, k1y, k1z;
-        if (!liquid && have_pic) {
-            k1x = pic[0];
-            k1y = pic[1];
-            k1z = pic[2];
-        } else {
-            sample_vel(f, x, y, z, dx, d, k1x, k1y, k1z);
-        }
         const float h1 = 0.5f * a.dt, h2 = 0.75f * a.dt;
         float k2x, k2y, k2z, k3x, k3y, k3z;
         sample_vel(f, x + h1 * k1x, y + h1 * k1y, z + h1 * k1z, dx, d, k2x, k2y, k2z);
@@ -944,6 +977,7 @@ PF_API int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream) {
     PF_REQUIRE(args, "pf_g2p_advect: null args");
     const pf_g2p_args& in = *args;
     PF_REQUIRE(in.nx > 0 && in.ny > 0 && in.nz > 0 && in.n >= 0, "pf_g2p_advect: bad sizes");
+    PF_REQUIRE(in.nz >= 2, "pf_g2p_advect: nz >= 2 is required (row-pair samples)");
     PF_REQUIRE(in.n < (int64_t)1 << 31, "pf_g2p_advect: more than 2^31 particles");
     PF_REQUIRE(in.dx > 0.f, "pf_g2p_advect: dx must be positive");
     if (in.n == 0) return PF_OK;

@@ -14,7 +14,11 @@ Differences from the reference that do not change the algorithm:
   r_prev copy is kept (``r_prev`` exists as an attribute but is unused);
 * ``precision="bf16"`` / ``"fp16"`` stores the V-cycle's face coefficients and
   iterates in 16 bits on the levels that run out of HBM (FP32 arithmetic);
-  the flexible CG absorbs the inexact preconditioner.
+  the flexible CG absorbs the inexact preconditioner;
+* a red-black sweep is one fused kernel that reads the iterate from one
+  buffer and writes it to a second one (``levels[l].r``, and ``levels[0].p``
+  in FP32); ``fused_smoother=False`` runs the two half-sweeps in place, with
+  the same result bit for bit.
 """
 
 from __future__ import annotations
@@ -69,7 +73,7 @@ class MGPCG:
     """Multigrid-preconditioned flexible CG (pfflip/poisson.py:240-409)."""
 
     def __init__(self, res, device, *, omega=6.0 / 7.0, pre=2, post=2, coarsest=60, min_dim=4,
-                 precision="fp32"):
+                 precision="fp32", fused_smoother=True):
         self.device = _require_cuda(device)
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
@@ -91,6 +95,9 @@ class MGPCG:
         mg.pre, mg.post, mg.coarsest = int(pre), int(post), int(coarsest)
         mg.omega = float(omega)
         mg.precision = PRECISIONS[precision]
+        # one fused kernel per red-black sweep (default) or one per half-sweep;
+        # both give the same bits
+        mg.options = 0 if fused_smoother else _lib.MG_UNFUSED_SMOOTHER
         for i, L in enumerate(self.levels):
             mg.lev[i].nx, mg.lev[i].ny, mg.lev[i].nz = L.res
         _lib.check(lib.pf_mg_setup(ctypes.byref(mg)), "pf_mg_setup")
@@ -148,9 +155,10 @@ class MGPCG:
 
 
 def time_smoother(solver, level=0, reps=10):
-    """Average milliseconds of one red-black half-sweep kernel of an HBM
-    level (HIP events on the current stream).  Smooths that level's current
-    iterate in place, so call it after a solve."""
+    """Average milliseconds of one smoother sweep kernel of an HBM level (HIP
+    events on the current stream): a fused full sweep, or one half-sweep if
+    the solver was built with ``fused_smoother=False``.  Overwrites that
+    level's iterate buffers, so call it after a solve."""
     ms = ctypes.c_float(0.0)
     _lib.call("pf_mg_time_smoother", ctypes.byref(solver._mg), int(level), int(reps),
               ctypes.byref(ms), _lib.stream())

@@ -155,6 +155,9 @@ typedef struct pf_mg_level {
     float* diag;
     float* p;
     float* b;
+    /* second iterate buffer of the fused smoother (which reads one buffer and
+     * writes the other) on the levels below small_from; in 16-bit mode it
+     * holds 16-bit values.  Level 0's `p` is the FP32 mode's second buffer. */
     float* r;
     /* reduced-precision copies, used when precision != 0 on the levels below
      * small_from; level 0's p_h (the preconditioned vector) always */
@@ -169,15 +172,20 @@ typedef struct pf_mg {
     float omega;
     int32_t precision;
     int32_t small_from; /* out: first level solved inside one workgroup (set by pf_mg_setup) */
-    int32_t _pad;
+    int32_t options;    /* PF_MG_* bits, 0 = defaults */
     pf_mg_level lev[PF_MAX_LEVELS];
-    float* rhs; /* CG work vectors, level-0 sized */
+    float* rhs; /* CG work vectors, level-0 sized (rhs and q hold the
+                   alternating search directions) */
     float* res;
     float* z;
     float* q;
     float* Ap;
     void* scalars; /* pf_mg_scalar_bytes() of device memory */
 } pf_mg;
+
+/* Smooth with two half-sweep launches per sweep instead of the fused
+ * full-sweep kernel (same results bit for bit; for testing and timing). */
+#define PF_MG_UNFUSED_SMOOTHER 1
 
 size_t pf_mg_scalar_bytes(void);
 /* Validate the level table and choose small_from. */
@@ -195,9 +203,10 @@ int pf_mg_precondition(const pf_mg* mg, const float* r, float* z, pf_stream_t st
 int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol, int32_t max_iter,
                    int32_t* it_out, double* rel_out, pf_stream_t stream);
 /* Measurement hook (no reference counterpart): average milliseconds of one
- * red-black half-sweep kernel of HBM level `level` over `reps` full sweeps,
- * timed with HIP events on `stream`.  Smooths that level's current iterate
- * in place (use after a solve).  Blocks. */
+ * smoother sweep of HBM level `level` over `reps` sweeps, timed with HIP
+ * events on `stream`: one fused full-sweep kernel, or with
+ * PF_MG_UNFUSED_SMOOTHER in mg->options one red-black half-sweep kernel.
+ * Overwrites that level's iterate buffers (use after a solve).  Blocks. */
 int pf_mg_time_smoother(const pf_mg* mg, int32_t level, int32_t reps, float* ms_out,
                         pf_stream_t stream);
 /* apply_A (poisson.py:54-88): out = diag*p - sum_f beta_f p_nbr with
@@ -266,6 +275,51 @@ int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream);
  * out[1] = max m and out[2] = max m*max_a|v_a| (the P2G bounds). */
 int pf_max_speed(const float* pv, const float* pmass, const int32_t* pstate, int64_t n,
                  float* out, pf_stream_t stream);
+
+/* ---------------------------------------------------------------- adjoint
+ * Reverse-mode derivative of one step with dt held fixed (no reference
+ * counterpart: the reference has no adjoint).  Each entry point transposes
+ * one forward stage at the forward state, which the caller rebuilds from a
+ * checkpoint of (x, v).  Order of a backward step:
+ *   pf_adj_g2p_advect -> pf_adj_extrapolate -> pf_adj_project ->
+ *   pf_mgpcg_solve(lam = A^+ p-bar) -> pf_adj_solve_beta -> pf_adj_phase ->
+ *   pf_adj_p2g.
+ * Liquid tags, wall/solid masks and extrapolation masks are held at their
+ * forward values; the eq. (7) clamps use one-sided subgradients. */
+
+/* Transpose of pf_g2p_advect with the same args (px, pv: the step's input
+ * state; vel, dvel, vel_liq: its grid fields; solids push-out and packed
+ * flags unsupported).  Writes gx_in, gv_in ((n,3) adjoints of x, v) from
+ * gx_out, gv_out (adjoints of x', v'); adds the grid adjoints into g_vel,
+ * g_dvel, g_vel_liq with float atomics (the caller zeroes them). */
+int pf_adj_g2p_advect(const pf_g2p_args* fwd, const float* gx_out, const float* gv_out, float* gx_in,
+                      float* gv_in, float* const g_vel[3], float* const g_dvel[3],
+                      float* const g_vel_liq[3], pf_stream_t stream);
+/* Transpose of pf_extrapolate: adds E^T g_vel_liq into g_vel; g_vel_liq is
+ * used as scratch; level (3 uint8 face arrays) is scratch. */
+int pf_adj_extrapolate(const int32_t* is_liquid, float* const g_vel_liq[3], float* const g_vel[3],
+                       uint8_t* const level[3], int nx, int ny, int nz, int sweeps, pf_stream_t stream);
+/* Transpose of pf_project + dvel: in g_vel (adjoint of the projected
+ * velocity) and g_dvel; out g_vel = adjoint of the pre-projection velocity,
+ * g_dvel = adjoint of vel_star, g_beta and g_p (both overwritten). */
+int pf_adj_project(const float* p, float* const beta[3], float* const g_vel[3], float* const g_dvel[3],
+                   float* const g_beta[3], float* g_p, int nx, int ny, int nz, float dx, float dt,
+                   pf_stream_t stream);
+/* beta-bar of the solve: g_beta_f -= (lam_i - lam_j)(p_i - p_j) over the
+ * interior faces, lam = A^+ p-bar (mean removed). */
+int pf_adj_solve_beta(const float* p, const float* lam, float* const g_beta[3], int nx, int ny, int nz,
+                      pf_stream_t stream);
+/* Transpose of pf_phase_beta_forces (PF_PH_VEL_FROM_MOM | PHASE | ZERO |
+ * STAR | GRAVITY | RHS, k_div = 0) with the same args at the forward state:
+ * g_rhs = lam; writes g_mass, g_mom (face arrays). */
+int pf_adj_phase(const pf_phase_args* fwd, const float* const g_vel[3], const float* const g_vel_star[3],
+                 const float* const g_beta[3], const float* g_rhs, float* const g_mass[3],
+                 float* const g_mom[3], pf_stream_t stream);
+/* Transpose of pf_p2g's face sums (the cell mass carries no gradient):
+ * adds into gx, gv ((n,3)). */
+int pf_adj_p2g(const float* px, const float* pv, const float* pmass, const int32_t* pstate, int64_t n,
+               const float* const g_mass[3], const float* const g_mom[3], int nx, int ny, int nz, float dx,
+               float rp, float* gx, float* gv, pf_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -61,3 +61,25 @@ def test_mixed_precision_requires_level0_iterate():
     assert rc == -1 and b"16-bit iterate" in lib.pf_last_error()
     rc = lib.pf_mg_precondition(ctypes.byref(mg), FAKE, FAKE, None)
     assert rc == -1
+
+
+def test_setup_respects_colour_capacity():
+    """A level joins the single-workgroup V-cycle only if one colour's cells
+    fit 1024 threads x 4 slots: 35 x 40 x 5 has 7000 cells and fits the LDS
+    (147 KB) but enumerates 35*40*3 = 4200 pairs along k; 34 x 40 x 5 gives
+    4080 and fits."""
+    load_pkg()
+    L = import_module("implementation-of-adaptive-phase-field-flip_amd._lib")
+    assert _mg(L, (35, 40, 5), 0, True).small_from == 1
+    assert _mg(L, (34, 40, 5), 0, True).small_from == 0
+
+
+def test_fused_smoother_requires_second_buffer():
+    """HBM levels need their second iterate buffer (r) for the fused sweep."""
+    load_pkg()
+    L = import_module("implementation-of-adaptive-phase-field-flip_amd._lib")
+    lib = L.load()
+    mg = _mg(L, (64, 64, 64), 0, True)
+    mg.lev[1].r = None
+    rc = lib.pf_mg_precondition(ctypes.byref(mg), FAKE, FAKE, None)
+    assert rc == -1 and b"second smoother buffer" in lib.pf_last_error()

@@ -1,0 +1,57 @@
+"""Host-side pieces of bench.py that its numbers rest on (CPU only)."""
+
+import importlib.util
+import os
+
+import numpy as np
+import torch
+
+from oracle import ref_np as O
+
+from test_oracle import assemble, random_beta
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_fp64_operator_matches_assembled_matrix():
+    """The FP64 operator used for the reference pressure equals the
+    scipy-assembled matrix (symmetric, zero row sums) on random data."""
+    bench = _bench()
+    rng = np.random.default_rng(3)
+    res = (6, 5, 7)
+    bx, by, bz = random_beta(res, rng, contrast=1000.0)
+    p = rng.standard_normal(res)
+    diag = O.compute_diag(bx, by, bz)
+    got = bench.fp64_apply_A(*(torch.from_numpy(a) for a in (bx, by, bz)), torch.from_numpy(p))
+    A = assemble(res, bx, by, bz)
+    np.testing.assert_allclose(got.numpy().ravel(), A @ p.ravel(), rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(got.numpy(), O.apply_A(p, bx, by, bz, diag), rtol=1e-13, atol=1e-13)
+    # constants are in the null space (zero row sums)
+    one = bench.fp64_apply_A(*(torch.from_numpy(a) for a in (bx, by, bz)), torch.ones(res, dtype=torch.float64))
+    assert float(one.abs().max()) < 1e-14   # FP64 rounding only
+
+
+def test_roofline_bytes():
+    bench = _bench()
+    n = 128 ** 3
+    assert bench.smoother_bytes(n) == 20 * n          # FP32: 40 B/cell per full sweep
+    assert bench.smoother_bytes(n, 2, 2) == 11 * n    # BF16 p, beta; FP32 b
+    assert bench.fused_sweep_bytes(n) == 24 * n
+    assert bench.fused_sweep_bytes(n, 2, 2) == 14 * n
+    assert bench.p2g_bytes(10, 1) == 25 * 10 + 28
+    assert bench.g2p_bytes(10, 1) == 49 * 10 + 36
+
+
+def test_multi_gpu_request_exits():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "single-GPU" in r.stdout

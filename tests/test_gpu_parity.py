@@ -285,7 +285,7 @@ def test_apply_A_matches_assembled(pf, contrast):
             assert rel_err(np_(getattr(solver.levels[lv], name)), getattr(o.levels[lv], name)) < 1e-6
 
 
-@pytest.mark.parametrize("res", [(16, 16, 16), (32, 32, 32), (32, 16, 24)])
+@pytest.mark.parametrize("res", [(16, 16, 16), (32, 32, 32), (32, 16, 24), (40, 40, 20), (12, 10, 14)])
 def test_precondition_matches_oracle_vcycle(pf, res):
     """One FP32 V-cycle (HBM levels + the LDS coarse solve) against the FP64
     restatement of poisson.py:312-352."""
@@ -348,6 +348,51 @@ def test_mgpcg_matches_oracle_iterations(pf):
     assert abs(it - oit) <= 1, (it, oit)
     assert rel < 1e-5
     assert rel_err(np_(p), x) < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("res,pre,post", [((32, 32, 32), 3, 3), ((48, 40, 24), 3, 3),
+                                          ((64, 64, 64), 2, 1), ((128, 64, 32), 1, 2),
+                                          ((128, 128, 128), 3, 3)])
+def test_fused_smoother_is_bitwise_split(pf, res, pre, post, precision):
+    """The fused full-sweep smoother (LDS tile, halo recomputation, ping-pong
+    buffers) gives exactly the bits of two in-place half-sweeps, including
+    partial tiles and odd sweep counts."""
+    say(f"fused smoother {res} pre={pre} post={post} {precision}")
+    rng = np.random.default_rng(11)
+    bx, by, bz = random_beta(res, rng, 1000.0)
+    r = rng.standard_normal(res).astype(np.float32)
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
+    out = []
+    for fused in (True, False):
+        solver = pf.MGPCG(res, "cuda:0", pre=pre, post=post, precision=precision,
+                          fused_smoother=fused)
+        assert solver.small_from >= 1
+        solver.build_hierarchy(g(bx), g(by), g(bz))
+        z = torch.zeros(res, device="cuda")
+        solver.precondition(g(r), z)
+        out.append(np_(z))
+        del solver
+    assert np.array_equal(out[0], out[1])
+
+
+def test_fused_smoother_solve_is_bitwise_split(pf):
+    say("fused smoother solve 64^3")
+    res = (64, 64, 64)
+    rng = np.random.default_rng(12)
+    bx, by, bz = random_beta(res, rng, 1000.0)
+    b = rng.standard_normal(res).astype(np.float32)
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
+    out = []
+    for fused in (True, False):
+        solver = pf.MGPCG(res, "cuda:0", pre=3, post=3, fused_smoother=fused)
+        solver.build_hierarchy(g(bx), g(by), g(bz))
+        p = torch.zeros(res, device="cuda")
+        it, rel = solver.solve(g(b), p, tol=1e-6, max_iter=200)
+        out.append((it, rel, np_(p)))
+        del solver
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
 
 
 def test_mgpcg_zero_rhs_and_max_iter(pf):
