@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libpfflip_hip.so")
 MAX_LEVELS = 16
 TILE = 8
 
-PH_VEL_FROM_MOM, PH_PHASE, PH_ZERO, PH_STAR, PH_GRAVITY, PH_RHS = 1, 2, 4, 8, 16, 32
+PH_VEL_FROM_MOM, PH_PHASE, PH_ZERO, PH_STAR, PH_GRAVITY, PH_RHS, PH_TAGS_IN = 1, 2, 4, 8, 16, 32, 64
 GA_G2P, GA_ADVECT, GA_REFLECT, GA_SOLIDS = 1, 2, 4, 8
 
 P = ctypes.c_void_p

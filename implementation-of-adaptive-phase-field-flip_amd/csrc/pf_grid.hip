@@ -36,6 +36,7 @@ __device__ inline float phase_field(float rho, const PhaseK& a) {
 }
 
 __device__ inline int cell_tag(const PhaseK& a, int c) {
+    if (a.flags & PF_PH_TAGS_IN) return a.is_liq[c];
     return phase_field(a.cmass[c], a) >= 0.5f ? 1 : 0;
 }
 
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(256) void k_phase(PhaseK a) {
         // cell_phase, kernels.py:242-255
         const float ph = phase_field(a.cmass[c], a);
         a.phi_cell[c] = ph;
-        a.is_liq[c] = ph >= 0.5f ? 1 : 0;
+        if (!(a.flags & PF_PH_TAGS_IN)) a.is_liq[c] = ph >= 0.5f ? 1 : 0;
     }
     if (a.flags & PF_PH_RHS) {
         // build_rhs, kernels.py:338-363
@@ -327,7 +328,9 @@ PF_API int pf_phase_beta_forces(const pf_phase_args* args, pf_stream_t stream) {
     PF_REQUIRE((int64_t)(in.nx + 1) * (in.ny + 1) * (in.nz + 1) < (int64_t)1 << 31,
                "pf_phase_beta_forces: grid too large");
     const int fl = in.flags;
-    PF_REQUIRE(fl != 0 && (fl & ~63) == 0, "pf_phase_beta_forces: bad flags %d", fl);
+    PF_REQUIRE(fl != 0 && (fl & ~127) == 0, "pf_phase_beta_forces: bad flags %d", fl);
+    PF_REQUIRE(!(fl & PF_PH_TAGS_IN) || ((fl & PF_PH_PHASE) && in.is_liquid),
+               "pf_phase_beta_forces: PF_PH_TAGS_IN goes with PF_PH_PHASE and an is_liquid input");
     for (int q = 0; q < 3; ++q) {
         if (fl & PF_PH_VEL_FROM_MOM)
             PF_REQUIRE(in.mass[q] && in.mom[q], "pf_phase_beta_forces: need mass/mom");

@@ -277,8 +277,13 @@ class Simulator:
         vmax = max(vmax, math.sqrt(2.0 * g * cfg.dx) if g > 0 else 0.0, 1e-8)
         return float(np.clip(cfg.cfl * cfg.dx / vmax, cfg.dt_min, cfg.dt_max))
 
-    def step(self, dt: float | None = None, verbose: bool = False) -> dict:
-        """One time step (pfflip/sim.py:168-202); droplet stage not included."""
+    def step(self, dt: float | None = None, verbose: bool = False, tags=None) -> dict:
+        """One time step (pfflip/sim.py:168-202); droplet stage not included.
+
+        ``tags`` (optional, an int32 cell array) replays the liquid tags of
+        another run instead of deriving them from this step's cell mass,
+        e.g. to difference a rollout along the branch its adjoint follows
+        (the adjoint holds the tags fixed)."""
         cfg = self.cfg
         if cfg.droplets and not self._warned_droplets:
             warnings.warn("Config.droplets=True: the escaped-particle / droplet stage is not "
@@ -294,8 +299,14 @@ class Simulator:
         self._mark("bin_sort")
         self._splat()
         self._mark("p2g")
-        self._phase(_lib.PH_VEL_FROM_MOM | _lib.PH_PHASE | _lib.PH_ZERO | _lib.PH_STAR
-                    | _lib.PH_GRAVITY | _lib.PH_RHS, dt)
+        flags = (_lib.PH_VEL_FROM_MOM | _lib.PH_PHASE | _lib.PH_ZERO | _lib.PH_STAR
+                 | _lib.PH_GRAVITY | _lib.PH_RHS)
+        if tags is not None:
+            if tuple(tags.shape) != self.res:
+                raise ValueError("tags must be a cell array of the grid's shape")
+            self.is_liquid.copy_(tags)
+            flags |= _lib.PH_TAGS_IN
+        self._phase(flags, dt)
         self._mark("phase_beta_rhs")
         self.solver.build_hierarchy(*self.beta)
         self._mark("mg_build")

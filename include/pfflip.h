@@ -110,6 +110,10 @@ int pf_p2g(const float* px, const float* pv, const float* pmass,
  *   PF_PH_STAR          vel_star = vel after zeroing (sim.py:176-177)
  *   PF_PH_GRAVITY       vel += g_a*dt, then re-zero (sim.py:270-285)
  *   PF_PH_RHS           rhs from the final vel (build_rhs, kernels.py:338-363)
+ *   PF_PH_TAGS_IN       with PF_PH_PHASE: is_liquid is an input (liquid tags
+ *                       replayed from another run, e.g. to difference a
+ *                       simulation along the branch its adjoint follows);
+ *                       phi_cell is still written
  * `solid` may be NULL (no obstacles). */
 #define PF_PH_VEL_FROM_MOM 1
 #define PF_PH_PHASE 2
@@ -117,6 +121,7 @@ int pf_p2g(const float* px, const float* pv, const float* pmass,
 #define PF_PH_STAR 8
 #define PF_PH_GRAVITY 16
 #define PF_PH_RHS 32
+#define PF_PH_TAGS_IN 64
 
 typedef struct pf_phase_args {
     int32_t nx, ny, nz, flags;
