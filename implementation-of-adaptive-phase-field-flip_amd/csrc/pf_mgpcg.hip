@@ -292,8 +292,10 @@ struct RbTile {
     static_assert(PAIRS % NT == 0 && TK % 2 == 0 && TJ % 2 == 0 && PPT >= 1, "tile shape");
 };
 
+// (4 waves per SIMD: at 128^3 all 1024 workgroups of 8x8x32 tiles are
+// resident at once, so the sweep costs one memory round trip, not two)
 template <typename ST, bool ZINIT, int TI, int TJ, int TK>
-__global__ __launch_bounds__(256) void k_rbgs_full(Dims d, const ST* __restrict__ bx,
+__global__ __launch_bounds__(256, 4) void k_rbgs_full(Dims d, const ST* __restrict__ bx,
                                                    const ST* __restrict__ by, const ST* __restrict__ bz,
                                                    const ST* __restrict__ pin, ST* __restrict__ pout,
                                                    const float* __restrict__ b, int c1, float omega,
@@ -307,22 +309,39 @@ __global__ __launch_bounds__(256) void k_rbgs_full(Dims d, const ST* __restrict_
     constexpr int SI = T::PJ * T::PK, SJ = T::PK;
     auto lidx = [&](int i, int j, int k) { return ((i - i0 + 2) * T::PJ + (j - j0 + 2)) * T::PK + (k - k0 + 2); };
 
-    // (1) staging loads: pairs along k of the tile + 2-cell halo
-    float2 sv[T::SPT];
+    // (1) staging: the iterate of the tile + 2-cell halo goes straight from
+    // memory into LDS (global_load_lds: lane l of a wave writes its dword to
+    // the wave's LDS base + 4 l, no registers).  FP32: one element per lane
+    // from clamped coordinates.  16-bit: raw pairs (k, k+1) from the nearest
+    // in-grid pair into R, converted into P after the barrier.
+    constexpr bool F32 = sizeof(ST) == 4;
+    __shared__ uint32_t R[F32 ? 1 : T::NSP];
     if (!ZINIT) {
-#pragma unroll
-        for (int r = 0; r < T::SPT; ++r) {
-            const int e = tid + r * T::NT;
-            sv[r] = make_float2(0.0f, 0.0f);
-            if (e < T::NSP) {
-                const int q = e % T::NPR, row = e / T::NPR;
-                const int gi = min(max(i0 - 2 + row / T::PJ, 0), d.nx - 1);
-                const int gj = min(max(j0 - 2 + row % T::PJ, 0), d.ny - 1);
-                // pair (kk, kk+1) with kk even, loaded from the nearest
-                // in-grid pair (fixed up at the LDS write)
-                const int kk = k0 - 2 + 2 * q;
-                const int kc = min(max(kk, 0), d.nz - 2);
-                ld_pair(pin, cell_index(d, gi, gj, kc), sv[r].x, sv[r].y);
+        if constexpr (F32) {
+            for (int e0 = 0; e0 < T::NP; e0 += T::NT) {
+                const int e = e0 + tid;
+                if (e < T::NP) {
+                    const int kk = e % T::PK, jj = (e / T::PK) % T::PJ, ii = e / SI;
+                    const int gi = min(max(i0 - 2 + ii, 0), d.nx - 1);
+                    const int gj = min(max(j0 - 2 + jj, 0), d.ny - 1);
+                    const int gk = min(max(k0 - 2 + kk, 0), d.nz - 1);
+                    __builtin_amdgcn_global_load_lds(
+                        (const void*)(pin + cell_index(d, gi, gj, gk)),
+                        (__attribute__((address_space(3))) void*)(P + e0 + (tid & ~63)), 4, 0, 0);
+                }
+            }
+        } else {
+            for (int e0 = 0; e0 < T::NSP; e0 += T::NT) {
+                const int e = e0 + tid;
+                if (e < T::NSP) {
+                    const int q = e % T::NPR, row = e / T::NPR;
+                    const int gi = min(max(i0 - 2 + row / T::PJ, 0), d.nx - 1);
+                    const int gj = min(max(j0 - 2 + row % T::PJ, 0), d.ny - 1);
+                    const int kc = min(max(k0 - 2 + 2 * q, 0), d.nz - 2);
+                    __builtin_amdgcn_global_load_lds(
+                        (const void*)(pin + cell_index(d, gi, gj, kc)),
+                        (__attribute__((address_space(3))) void*)(R + e0 + (tid & ~63)), 4, 0, 0);
+                }
             }
         }
     }
@@ -392,22 +411,28 @@ __global__ __launch_bounds__(256) void k_rbgs_full(Dims d, const ST* __restrict_
     }
     const bool skip = done && *done;
 
-    // staged pairs into LDS (row-major pairs: LDS float index = 2 * pair
-    // index); past either end of a row both entries clamp to the end cell
-#pragma unroll
-    for (int r = 0; r < T::SPT; ++r) {
-        const int e = tid + r * T::NT;
-        if (e < T::NSP) {
-            const int kk = k0 - 2 + 2 * (e % T::NPR);
-            float2 v = sv[r];
-            if (kk < 0) v.y = v.x;
-            if (kk > d.nz - 2) v.x = v.y;
-            *reinterpret_cast<float2*>(&P[2 * e]) = v;
+    if (ZINIT) {
+        for (int e = tid; e < T::NP; e += T::NT) P[e] = 0.0f;
+    } else {
+        __builtin_amdgcn_s_waitcnt(0);  // the LDS-destined loads have landed
+    }
+    __syncthreads();
+    if constexpr (!F32) {
+        if (!ZINIT) {
+            // raw pairs -> floats (LDS index 2 * pair index); past either end
+            // of a row both entries clamp to the end cell
+            for (int e = tid; e < T::NSP; e += T::NT) {
+                const uint32_t w = R[e];
+                float x = to_f32(ST{(uint16_t)(w & 0xFFFFu)}), y = to_f32(ST{(uint16_t)(w >> 16)});
+                const int kk = k0 - 2 + 2 * (e % T::NPR);
+                if (kk < 0) y = x;
+                if (kk > d.nz - 2) x = y;
+                P[2 * e] = x;
+                P[2 * e + 1] = y;
+            }
+            __syncthreads();
         }
     }
-    if (ZINIT)
-        for (int e = tid; e < T::NP; e += T::NT) P[e] = 0.0f;
-    __syncthreads();
 
     // first colour on the tile
 #pragma unroll
@@ -550,24 +575,30 @@ struct SmallArgs {
     float omega;
     Dims d[PF_MAX_LEVELS];
     int off[PF_MAX_LEVELS][5];  // LDS float offsets of bx, by, bz, p, b
-    // global -> LDS copies at entry: segment q copies seg_end[q] - seg_end[q-1]
-    // floats from seg_src[q] to LDS offset seg_dst[q] (the face arrays of every
-    // level, then the top level's b)
+    int scratch;                // LDS float offset of the residual scratch (n(s) floats)
+    // global -> LDS copies at entry: segment q copies seg_len[q] floats from
+    // seg_src[q] to LDS offset seg_dst[q] (the face arrays of every level,
+    // then the top level's b)
     int nseg;
     const float* seg_src[3 * PF_MAX_LEVELS + 1];
     int seg_dst[3 * PF_MAX_LEVELS + 1];
-    int seg_end[3 * PF_MAX_LEVELS + 1];
+    int seg_len[3 * PF_MAX_LEVELS + 1];
     void* p_top;
     const int* done;
 };
 
 constexpr int kSmallThreads = 1024;
-constexpr int kSlots = 4;  // cells of one colour per thread and level
+constexpr int kSlots = 2;  // cells of one colour per thread and level
 
-// Size of the per-colour enumeration of a level (pairs along k); a level
-// fits the single-workgroup V-cycle if it is at most kSmallThreads*kSlots,
-// and is smoothed by one wave if it is at most 64*kSlots.
+// Size of the per-colour enumeration of a level (pairs along k).  A level
+// fits the single-workgroup V-cycle if it is at most kSmallThreads*kSlots;
+// levels of at most 64 are smoothed by one wave.
 __host__ __device__ inline int colour_space(const Dims& d) { return d.nx * d.ny * ((d.nz + 1) >> 1); }
+
+// floor(t / d) for 0 <= t < 2^22 and 1 <= d, with inv = 1.0f / d: t + 1/2
+// stays at least 1/(2d) away from a multiple of d, far more than the float
+// rounding of the product.
+__device__ inline int fdiv(int t, float inv) { return (int)(((float)t + 0.5f) * inv); }
 
 struct LdsLevel {
     Dims d;
@@ -589,70 +620,68 @@ __device__ inline LdsLevel lds_level(const SmallArgs& a, float* lds, int l) {
     return v;
 }
 
-// The cells of one colour that a thread updates on one level, decoded once
-// per level: LDS offsets of the cell (and of its y / z low faces; the x low
-// face shares the cell's offset), which of the 6 neighbours lie inside the
-// grid (a missing neighbour is the cell itself, as the reference clamps),
-// and the cell's right-hand side.
-struct Slots {
-    int c[kSlots], fy[kSlots], fz[kSlots];
-    unsigned m[kSlots];
-    float b[kSlots];
+// One cell that a thread updates on one level, decoded once per level visit:
+// its index, the indices of its six neighbours with the reference's clamp at
+// the walls (a missing neighbour is the cell itself), its y / z low faces
+// (the x low face shares the cell index), its parent on the next coarser
+// level, its right-hand side and omega / diag (0 where diag <= 0: no update).
+struct Slot {
+    int c, n[6], fy, fz, par;
+    float b, w;
 };
 
-__device__ inline void make_slots(const LdsLevel& v, int colour, int t0, int nth, Slots& S) {
+__device__ inline void make_slot(const LdsLevel& v, int colour, int t, float omega, const Dims* dc, Slot& S) {
     const Dims& d = v.d;
-    const int nkp = (d.nz + 1) >> 1, n = d.nx * d.ny * nkp;
-#pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-        const int t = t0 + s * nth;
-        S.c[s] = -1;
-        S.fy[s] = S.fz[s] = 0;
-        S.m[s] = 0u;
-        S.b[s] = 0.0f;
-        if (t < n) {
-            const int kp = t % nkp, row = t / nkp;
-            const int j = row % d.ny, i = row / d.ny;
-            const int k = 2 * kp + ((i + j + colour) & 1);
-            if (k < d.nz) {
-                const int c = cell_index(d, i, j, k);
-                S.c[s] = c;
-                S.fy[s] = c + i * d.nz;                     // face_index<1>(d, i, j, k)
-                S.fz[s] = c + (i * d.ny + j);               // face_index<2>(d, i, j, k)
-                S.m[s] = (unsigned)(i > 0) | ((unsigned)(i < d.nx - 1) << 1) | ((unsigned)(j > 0) << 2) |
-                         ((unsigned)(j < d.ny - 1) << 3) | ((unsigned)(k > 0) << 4) |
-                         ((unsigned)(k < d.nz - 1) << 5);
-                S.b[s] = v.b[c];
-            }
-        }
-    }
+    const int nkp = (d.nz + 1) >> 1;
+    S.c = -1;
+    if (t >= d.nx * d.ny * nkp) return;
+    const int row = fdiv(t, 1.0f / (float)nkp), kp = t - row * nkp;
+    const int i = fdiv(row, 1.0f / (float)d.ny), j = row - i * d.ny;
+    const int k = 2 * kp + ((i + j + colour) & 1);
+    if (k >= d.nz) return;
+    const int SI = d.ny * d.nz, SJ = d.nz;
+    const int c = (i * d.ny + j) * d.nz + k;
+    S.c = c;
+    S.n[0] = i > 0 ? c - SI : c;
+    S.n[1] = i < d.nx - 1 ? c + SI : c;
+    S.n[2] = j > 0 ? c - SJ : c;
+    S.n[3] = j < d.ny - 1 ? c + SJ : c;
+    S.n[4] = k > 0 ? c - 1 : c;
+    S.n[5] = k < d.nz - 1 ? c + 1 : c;
+    S.fy = c + i * d.nz;        // face_index<1>(d, i, j, k)
+    S.fz = c + (i * d.ny + j);  // face_index<2>(d, i, j, k)
+    S.par = dc ? cell_index(*dc, i >> 1, j >> 1, k >> 1) : 0;
+    const float dg = v.bx[c] + v.bx[c + SI] + v.by[S.fy] + v.by[S.fy + SJ] + v.bz[S.fz] + v.bz[S.fz + 1];
+    S.b = v.b[c];
+    S.w = dg > 0.0f ? omega / dg : 0.0f;
+}
+
+// A p at a slot's cell in the difference form (same term order as stencil_vals)
+__device__ inline float slot_Ap(const LdsLevel& v, const Slot& S, float pc) {
+    const int SI = v.d.ny * v.d.nz, SJ = v.d.nz;
+    float s = v.bx[S.c] * (pc - v.p[S.n[0]]);
+    s += v.bx[S.c + SI] * (pc - v.p[S.n[1]]);
+    s += v.by[S.fy] * (pc - v.p[S.n[2]]);
+    s += v.by[S.fy + SJ] * (pc - v.p[S.n[3]]);
+    s += v.bz[S.fz] * (pc - v.p[S.n[4]]);
+    s += v.bz[S.fz + 1] * (pc - v.p[S.n[5]]);
+    return s;
 }
 
 // One half-sweep over a thread's slots: every read first, then every write
 // (cells of one colour never neighbour each other).
-__device__ inline void slots_sweep(const LdsLevel& v, const Slots& S, float omega) {
-    const Dims& d = v.d;
-    const int SI = d.ny * d.nz, SJ = d.nz;
+__device__ inline void slots_sweep(const LdsLevel& v, const Slot (&S)[kSlots]) {
     float nv[kSlots];
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-        nv[s] = 0.0f;
-        if (S.c[s] < 0) continue;
-        const int c = S.c[s];
-        const unsigned m = S.m[s];
-        const float pc = v.p[c];
-        const float pn[6] = {v.p[c - ((m & 1u) ? SI : 0)],  v.p[c + ((m & 2u) ? SI : 0)],
-                             v.p[c - ((m & 4u) ? SJ : 0)],  v.p[c + ((m & 8u) ? SJ : 0)],
-                             v.p[c - ((m & 16u) ? 1 : 0)], v.p[c + ((m & 32u) ? 1 : 0)]};
-        const float f[6] = {v.bx[c], v.bx[c + SI], v.by[S.fy[s]], v.by[S.fy[s] + SJ], v.bz[S.fz[s]],
-                            v.bz[S.fz[s] + 1]};
-        float dg, Ap;
-        stencil_vals(pc, f, pn, dg, Ap);
-        nv[s] = dg > 0.0f ? rb_relax(pc, S.b[s], Ap, dg, omega) : pc;
+    for (int q = 0; q < kSlots; ++q) {
+        nv[q] = 0.0f;
+        if (S[q].c < 0) continue;
+        const float pc = v.p[S[q].c];
+        nv[q] = pc + (S[q].b - slot_Ap(v, S[q], pc)) * S[q].w;
     }
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s)
-        if (S.c[s] >= 0) v.p[S.c[s]] = nv[s];
+    for (int q = 0; q < kSlots; ++q)
+        if (S[q].c >= 0) v.p[S[q].c] = nv[q];
 }
 
 __device__ inline void wave_sync() {
@@ -663,88 +692,105 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// `sweeps` red-black sweeps of a level, colour order (0,1) or (1,0) if
-// reverse.  Levels whose colour enumeration fits one wave's slots are
-// smoothed by wave 0 alone (no workgroup barriers); the caller synchronises
-// afterwards.
-__device__ inline void small_smooth(const LdsLevel& v, int sweeps, bool reverse, float omega) {
-    const int tid = threadIdx.x;
-    Slots S0, S1;
-    if (colour_space(v.d) <= 64 * kSlots) {
-        if (tid >= 64) return;
-        make_slots(v, reverse ? 1 : 0, tid, 64, S0);
-        make_slots(v, reverse ? 0 : 1, tid, 64, S1);
-        for (int sw = 0; sw < sweeps; ++sw) {
-            slots_sweep(v, S0, omega);
-            wave_sync();
-            slots_sweep(v, S1, omega);
-            wave_sync();
+// The slots of both colours of one level for this thread (colour c0 first).
+struct LevelSlots {
+    Slot s0[kSlots], s1[kSlots];
+    bool wave;  // smoothed by wave 0 alone
+};
+
+__device__ inline void make_level_slots(const LdsLevel& v, int c0, float omega, const Dims* dc, LevelSlots& L) {
+    L.wave = colour_space(v.d) <= 64;
+    const int nth = L.wave ? 64 : kSmallThreads;
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+        const int t = threadIdx.x + q * nth;
+        const bool mine = !L.wave || threadIdx.x < 64;
+        L.s0[q].c = L.s1[q].c = -1;
+        if (mine) {
+            make_slot(v, c0, t, omega, dc, L.s0[q]);
+            make_slot(v, 1 - c0, t, omega, dc, L.s1[q]);
         }
-        return;
     }
-    make_slots(v, reverse ? 1 : 0, tid, kSmallThreads, S0);
-    make_slots(v, reverse ? 0 : 1, tid, kSmallThreads, S1);
-    for (int sw = 0; sw < sweeps; ++sw) {
-        slots_sweep(v, S0, omega);
-        __syncthreads();
-        slots_sweep(v, S1, omega);
-        __syncthreads();
+}
+
+// `sweeps` red-black sweeps (colour c0 first); ends with a workgroup barrier
+__device__ inline void small_smooth(const LdsLevel& v, const LevelSlots& L, int sweeps) {
+    if (L.wave) {
+        if (threadIdx.x < 64)
+            for (int sw = 0; sw < sweeps; ++sw) {
+                slots_sweep(v, L.s0);
+                wave_sync();
+                slots_sweep(v, L.s1);
+                wave_sync();
+            }
+    } else {
+        for (int sw = 0; sw < sweeps; ++sw) {
+            slots_sweep(v, L.s0);
+            __syncthreads();
+            slots_sweep(v, L.s1);
+            __syncthreads();
+        }
     }
+    __syncthreads();
 }
 
 // The whole V-cycle below `s` in one workgroup, every level in LDS.
 template <typename OT>
 __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
     extern __shared__ float lds[];
-    if (a.done && *a.done) return;
     const int tid = threadIdx.x, nt = kSmallThreads;
-    // copy faces and the top b into LDS, eight loads in flight per thread
-    {
-        constexpr int U = 8;
-        const int total = a.seg_end[a.nseg - 1];
-        for (int base = 0; base < total; base += U * nt) {
-            float v[U];
-            int dst[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int e = base + u * nt + tid;
-                dst[u] = -1;
-                v[u] = 0.0f;
-                if (e < total) {
-                    int q = 0;
-                    while (e >= a.seg_end[q]) ++q;
-                    const int lo = q ? a.seg_end[q - 1] : 0;
-                    v[u] = a.seg_src[q][e - lo];
-                    dst[u] = a.seg_dst[q] + (e - lo);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (dst[u] >= 0) lds[dst[u]] = v[u];
-        }
+    const int dn = a.done ? *a.done : 0;
+    // faces of every level and the top b straight into LDS (lane l of a wave
+    // writes its dword to the wave's LDS base + 4 l)
+    for (int q = 0; q < a.nseg; ++q) {
+        const float* src = a.seg_src[q];
+        const int len = a.seg_len[q];
+        float* dst = lds + a.seg_dst[q];
+        for (int e0 = 0; e0 < len; e0 += nt)
+            if (e0 + tid < len)
+                __builtin_amdgcn_global_load_lds((const void*)(src + e0 + tid),
+                                                 (__attribute__((address_space(3))) void*)(dst + e0 + (tid & ~63)),
+                                                 4, 0, 0);
     }
+    __builtin_amdgcn_s_waitcnt(0);
+    if (dn) return;  // uniform: no workgroup barrier has been reached yet
     __syncthreads();
-    // descend
+    float* scr = lds + a.scratch;
+    LevelSlots S;
+    // descend: smooth from zero, restrict the residual (poisson.py:323-336)
     for (int l = a.s; l < a.L - 1; ++l) {
         const LdsLevel v = lds_level(a, lds, l);
         const LdsLevel c = lds_level(a, lds, l + 1);
         const int n = v.d.nx * v.d.ny * v.d.nz;
         for (int e = tid; e < n; e += nt) v.p[e] = 0.0f;
+        make_level_slots(v, 0, a.omega, &c.d, S);
         __syncthreads();
-        small_smooth(v, a.pre, false, a.omega);
+        small_smooth(v, S, a.pre);
+        // residual of every fine cell into the scratch, then R = P^T
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            if (S.s0[q].c >= 0) {
+                const float pc = v.p[S.s0[q].c];
+                scr[S.s0[q].c] = S.s0[q].b - slot_Ap(v, S.s0[q], pc);
+            }
+            if (S.s1[q].c >= 0) {
+                const float pc = v.p[S.s1[q].c];
+                scr[S.s1[q].c] = S.s1[q].b - slot_Ap(v, S.s1[q], pc);
+            }
+        }
         __syncthreads();
         const int nc = c.d.nx * c.d.ny * c.d.nz;
+        const float inz = 1.0f / (float)c.d.nz, iny = 1.0f / (float)c.d.ny;
         for (int t = tid; t < nc; t += nt) {
-            const int K = t % c.d.nz, J = (t / c.d.nz) % c.d.ny, I = t / (c.d.nz * c.d.ny);
+            const int row = fdiv(t, inz), K = t - row * c.d.nz;
+            const int I = fdiv(row, iny), J = row - I * c.d.ny;
             float acc = 0.0f;
+#pragma unroll
             for (int x = 0; x < 2; ++x)
+#pragma unroll
                 for (int y = 0; y < 2; ++y)
-                    for (int z = 0; z < 2; ++z) {
-                        const int i = 2 * I + x, j = 2 * J + y, k = 2 * K + z;
-                        float dg, Ap, pc;
-                        stencil(v.d, v.bx, v.by, v.bz, v.p, i, j, k, dg, Ap, pc);
-                        acc += v.b[cell_index(v.d, i, j, k)] - Ap;
-                    }
+#pragma unroll
+                    for (int z = 0; z < 2; ++z) acc += scr[cell_index(v.d, 2 * I + x, 2 * J + y, 2 * K + z)];
             c.b[t] = acc;
         }
         __syncthreads();
@@ -754,9 +800,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
         const LdsLevel v = lds_level(a, lds, a.L - 1);
         const int n = v.d.nx * v.d.ny * v.d.nz;
         for (int e = tid; e < n; e += nt) v.p[e] = 0.0f;
+        make_level_slots(v, 0, a.omega, nullptr, S);
         __syncthreads();
-        small_smooth(v, a.coarsest, false, a.omega);
-        __syncthreads();
+        small_smooth(v, S, a.coarsest);
         __shared__ double sh[16];
         double sm[1] = {0.0};
         for (int e = tid; e < n; e += nt) sm[0] += (double)v.p[e];
@@ -767,18 +813,18 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
         for (int e = tid; e < n; e += nt) v.p[e] = v.p[e] + shift;
         __syncthreads();
     }
-    // ascend
+    // ascend: prolong (poisson.py:178-182), smooth (colours (1,0))
     for (int l = a.L - 2; l >= a.s; --l) {
         const LdsLevel v = lds_level(a, lds, l);
         const LdsLevel c = lds_level(a, lds, l + 1);
-        const int n = v.d.nx * v.d.ny * v.d.nz;
-        for (int t = tid; t < n; t += nt) {
-            const int k = t % v.d.nz, j = (t / v.d.nz) % v.d.ny, i = t / (v.d.nz * v.d.ny);
-            v.p[t] = v.p[t] + c.p[cell_index(c.d, i / 2, j / 2, k / 2)];
+        make_level_slots(v, 1, a.omega, &c.d, S);
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            if (S.s0[q].c >= 0) v.p[S.s0[q].c] += c.p[S.s0[q].par];
+            if (S.s1[q].c >= 0) v.p[S.s1[q].c] += c.p[S.s1[q].par];
         }
         __syncthreads();
-        small_smooth(v, a.post, true, a.omega);
-        __syncthreads();
+        small_smooth(v, S, a.post);
     }
     {
         const LdsLevel v = lds_level(a, lds, a.s);
@@ -1013,27 +1059,29 @@ __global__ __launch_bounds__(RT) void k_apply_A(Dims d, const float* __restrict_
 static Dims ldims(const pf_mg_level& l) { return Dims{l.nx, l.ny, l.nz}; }
 static int64_t ncell(const pf_mg_level& l) { return (int64_t)l.nx * l.ny * l.nz; }
 
-static int64_t small_lds_floats(const pf_mg& mg, int s, int (*off)[5]) {
+// LDS layout of the single-workgroup V-cycle from level s down: the face
+// arrays of every level back to back (one contiguous copy), then p and b of
+// every level, then a residual scratch of level s's size.
+static int64_t small_lds_floats(const pf_mg& mg, int s, int (*off)[5], int* scratch) {
     int64_t o = 0;
     for (int l = s; l < mg.nlev; ++l) {
         const Dims d = ldims(mg.lev[l]);
-        const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
-        if (off) {
-            off[l][0] = (int)o;
-            o += face_count(d, 0);
-            off[l][1] = (int)o;
-            o += face_count(d, 1);
-            off[l][2] = (int)o;
-            o += face_count(d, 2);
-            off[l][3] = (int)o;
-            o += nc;
-            off[l][4] = (int)o;
-            o += nc;
-        } else {
-            o += face_count(d, 0) + face_count(d, 1) + face_count(d, 2) + 2 * nc;
-        }
+        if (off) off[l][0] = (int)o;
+        o += face_count(d, 0);
+        if (off) off[l][1] = (int)o;
+        o += face_count(d, 1);
+        if (off) off[l][2] = (int)o;
+        o += face_count(d, 2);
     }
-    return o;
+    for (int l = s; l < mg.nlev; ++l) {
+        const int64_t nc = ncell(mg.lev[l]);
+        if (off) off[l][3] = (int)o;
+        o += nc;
+        if (off) off[l][4] = (int)o;
+        o += nc;
+    }
+    if (scratch) *scratch = (int)o;
+    return o + ncell(mg.lev[s]);
 }
 
 constexpr int64_t kSmallMaxCells = 8192;
@@ -1176,14 +1224,12 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
         a.coarsest = mg.coarsest;
         a.omega = mg.omega;
         for (int l = sm; l < L; ++l) a.d[l] = ldims(mg.lev[l]);
-        const int64_t nf = small_lds_floats(mg, sm, a.off);
+        const int64_t nf = small_lds_floats(mg, sm, a.off, &a.scratch);
         a.nseg = 0;
-        int end = 0;
         auto seg = [&](const float* src, int dst, int64_t n) {
             a.seg_src[a.nseg] = src;
             a.seg_dst[a.nseg] = dst;
-            end += (int)n;
-            a.seg_end[a.nseg++] = end;
+            a.seg_len[a.nseg++] = (int)n;
         };
         for (int l = sm; l < L; ++l) {
             const Dims d = a.d[l];
@@ -1539,7 +1585,7 @@ PF_API int pf_mg_setup(pf_mg* mg) {
     int sm = mg->nlev;
     for (int l = mg->nlev - 1; l >= 0; --l) {
         const int64_t nc = (int64_t)mg->lev[l].nx * mg->lev[l].ny * mg->lev[l].nz;
-        const int64_t bytes = small_lds_floats(*mg, l, nullptr) * (int64_t)sizeof(float);
+        const int64_t bytes = small_lds_floats(*mg, l, nullptr, nullptr) * (int64_t)sizeof(float);
         const Dims d{mg->lev[l].nx, mg->lev[l].ny, mg->lev[l].nz};
         if (nc <= kSmallMaxCells && bytes <= kSmallMaxBytes && colour_space(d) <= kSmallThreads * kSlots)
             sm = l;

@@ -65,13 +65,13 @@ def test_mixed_precision_requires_level0_iterate():
 
 def test_setup_respects_colour_capacity():
     """A level joins the single-workgroup V-cycle only if one colour's cells
-    fit 1024 threads x 4 slots: 35 x 40 x 5 has 7000 cells and fits the LDS
-    (147 KB) but enumerates 35*40*3 = 4200 pairs along k; 34 x 40 x 5 gives
-    4080 and fits."""
+    fit 1024 threads x 2 slots: 20 x 30 x 7 has 4200 cells and fits the LDS
+    (105 KB) but enumerates 20*30*4 = 2400 pairs along k; 20 x 30 x 6 gives
+    1800 and fits."""
     load_pkg()
     L = import_module("implementation-of-adaptive-phase-field-flip_amd._lib")
-    assert _mg(L, (35, 40, 5), 0, True).small_from == 1
-    assert _mg(L, (34, 40, 5), 0, True).small_from == 0
+    assert _mg(L, (20, 30, 7), 0, True).small_from == 1
+    assert _mg(L, (20, 30, 6), 0, True).small_from == 0
 
 
 def test_fused_smoother_requires_second_buffer():
