@@ -126,13 +126,14 @@ def fp64_reference_pressure(solver, rhs, torch, tol=1e-12, max_refine=10):
 
 
 def solve_compare(pf, torch, sim, cfg, tol):
-    """Time FP32 and BF16 MG-PCG on the same captured system (beta, rhs) and
-    give each one's pressure error against an FP64 reference solution."""
+    """Time FP32, BF16 and FP16 MG-PCG on the same captured system (beta,
+    rhs) and give each one's pressure error against an FP64 reference
+    solution."""
     out = {}
     beta = [b.clone() for b in sim.beta]
     rhs = sim.rhs.clone()
     p_ref = None
-    for prec in ("fp32", "bf16"):
+    for prec in ("fp32", "bf16", "fp16"):
         s = pf.MGPCG(sim.res, cfg.device, omega=cfg.mg_omega, pre=cfg.mg_pre_sweeps,
                      post=cfg.mg_post_sweeps, coarsest=cfg.mg_coarsest_sweeps,
                      min_dim=cfg.mg_min_dim, precision=prec)

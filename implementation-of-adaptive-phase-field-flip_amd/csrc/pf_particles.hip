@@ -865,7 +865,7 @@ PF_API size_t pf_bin_scan_tmp_bytes(int nx, int ny, int nz) {
 PF_API int pf_bin_sort(const float* px, const int32_t* pstate, int64_t n, int nx, int ny, int nz,
                        float dx, const uint32_t* order, uint32_t* keys, uint32_t* rank,
                        uint32_t* perm, int32_t* cell_start, void* scan_tmp, int32_t* n_active,
-                       pf_stream_t stream) {
+                       int32_t stable, pf_stream_t stream) {
     PF_REQUIRE(nx > 0 && ny > 0 && nz > 0 && n >= 0, "pf_bin_sort: bad sizes");
     PF_REQUIRE(n < (int64_t)1 << 31, "pf_bin_sort: more than 2^31 particles");
     PF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 31, "pf_bin_sort: grid too large");
@@ -896,8 +896,10 @@ PF_API int pf_bin_sort(const float* px, const int32_t* pstate, int64_t n, int nx
     if (n > 0) {
         k_scatter<<<cdiv(n, 256), 256, 0, s>>>(keys, rank, n, order, (uint32_t)nkeys, cell_start, perm);
         PF_CHECK_LAUNCH();
-        k_key_sort<<<(unsigned)(nkeys / SLAB_KEYS), 64, 0, s>>>(cell_start, perm);
-        PF_CHECK_LAUNCH();
+        if (stable) {
+            k_key_sort<<<(unsigned)(nkeys / SLAB_KEYS), 64, 0, s>>>(cell_start, perm);
+            PF_CHECK_LAUNCH();
+        }
     }
     return PF_OK;
 }

@@ -150,14 +150,10 @@ class Simulator:
         self.prad = torch.full((n,), cfg.r_p, dtype=torch.float32, device=dev)
         self.pstate = torch.zeros(n, dtype=torch.int32, device=dev)
         self.pdrop_r = torch.zeros(n, dtype=torch.float32, device=dev)
-        # binning buffers: keys and slot ranks in visiting order, and two
-        # permutations (this step's sorted order; the previous one, which is
-        # the visiting order of the next binning)
+        # binning buffers: keys and slot ranks per particle, cell-sorted order
         self._keys = torch.empty(n, dtype=torch.int32, device=dev)
         self._rank = torch.empty(n, dtype=torch.int32, device=dev)
         self.perm = torch.empty(n, dtype=torch.int32, device=dev)
-        self._perm_alt = torch.empty(n, dtype=torch.int32, device=dev)
-        self._perm_valid = False
 
     # ------------------------------------------------------------------ helpers
     def _read(self, t, count=1):
@@ -225,18 +221,16 @@ class Simulator:
         a.hi = (ctypes.c_float * 3)(*hi)
         return a
 
-    def _bin(self):
-        """Cell-sort the FLIP particles (pf_bin_sort), visiting them in the
-        previous step's sorted order."""
+    def _bin(self, stable: bool = False):
+        """Cell-sort the FLIP particles (pf_bin_sort) in particle-index order.
+        The P2G sums do not depend on the order within a cell, so ties are
+        left in atomic order unless ``stable`` asks for index order."""
         nx, ny, nz = self.res
-        order = self.perm if self._perm_valid else None
-        out = self._perm_alt
         _lib.call("pf_bin_sort", _lib.ptr(self.px), _lib.ptr(self.pstate), self.n_particles,
-                  nx, ny, nz, self.cfg.dx, _lib.ptr(order), _lib.ptr(self._keys),
-                  _lib.ptr(self._rank), _lib.ptr(out), _lib.ptr(self.cell_start),
-                  _lib.ptr(self._scan_tmp), _lib.ptr(self._n_active), _lib.stream())
-        self._perm_alt, self.perm = self.perm, out
-        self._perm_valid = True
+                  nx, ny, nz, self.cfg.dx, None, _lib.ptr(self._keys),
+                  _lib.ptr(self._rank), _lib.ptr(self.perm), _lib.ptr(self.cell_start),
+                  _lib.ptr(self._scan_tmp), _lib.ptr(self._n_active), 1 if stable else 0,
+                  _lib.stream())
 
     def _splat(self):
         nx, ny, nz = self.res

@@ -62,8 +62,9 @@ int pf_read_scalars(const void* dev_src, void* host_dst, size_t bytes, pf_stream
  * previous step's `perm`; NULL = index order); keys[i] and rank[i] belong
  * to the i-th visited particle (key 0xFFFFFFFF = not a FLIP particle).
  * On return `perm` is a permutation of all n particles: the FLIP particles
- * sorted by key with ties by particle index (a unique order), then the
- * others.  cell_start (nkeys + 2 entries, nkeys = pf_bin_nkeys()) gives
+ * sorted by key, then the others.  With `stable` != 0 ties are broken by
+ * particle index (a unique order); otherwise the order within a key depends
+ * on atomic timing (pf_p2g's sums do not depend on it).  cell_start (nkeys + 2 entries, nkeys = pf_bin_nkeys()) gives
  * key k the range cell_start[k] .. cell_start[k+1]; cell_start[nkeys] is
  * the number of FLIP particles and cell_start[nkeys+1] = n.
  * Buffers: keys, rank, perm: n entries; scan_tmp: pf_bin_scan_tmp_bytes().
@@ -73,7 +74,7 @@ size_t pf_bin_scan_tmp_bytes(int nx, int ny, int nz);
 int pf_bin_sort(const float* px, const int32_t* pstate, int64_t n, int nx, int ny, int nz,
                 float dx, const uint32_t* order, uint32_t* keys, uint32_t* rank,
                 uint32_t* perm, int32_t* cell_start, void* scan_tmp, int32_t* n_active,
-                pf_stream_t stream);
+                int32_t stable, pf_stream_t stream);
 
 /* ---------------------------------------------------------------- P2G
  * Replaces Simulator._p2g's zeroing + p2g_face x3 + p2g_cell
