@@ -63,6 +63,19 @@ def roofline(name, nbytes, ms):
             "frac": gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": nbytes, "avg_ms": ms}
 
 
+def measured_traffic(workload, precision):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    (copy-calibrated) for this workload, from profiles/traffic.json."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("solver_precision") != precision:
+        return None
+    return t
+
+
 def cpu_baseline(seconds_budget=30.0):
     """FP64 numpy restatement of the reference step (oracle/ref_np.py), one
     thread, on a bounded sample: the same dam break at 32^3 x 27 ppc."""
@@ -272,6 +285,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve-compare", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 512 MiB device-to-device copy three times (the known-size "
+                         "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "only single-GPU runs are supported: the step runs on one "
@@ -281,6 +297,16 @@ def main():
 
     import numpy as np
     import torch
+    if args.copy_calibration:
+        nbytes = 512 << 20
+        x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": nbytes, "bytes_written": nbytes,
+                                               "launches": 3}}), flush=True)
+        return
     pf = importlib.import_module("implementation-of-adaptive-phase-field-flip_amd")
 
     n = args.res
@@ -322,6 +348,14 @@ def main():
         roofline("p2g stage (k_max_count + k_p2g + k_p2g_gather)", p2g_bytes(n_p, n_c), stages["p2g"]),
         roofline("k_g2p_advect (G2P + RK3 + reflect)", g2p_bytes(n_p, n_c), stages["g2p_advect"]),
     ]
+    workload = f"dam break z-up {n}^3 x {args.ppc} ppc, 1000:1, MG-PCG tol 1e-6"
+    tr = measured_traffic(workload, args.precision)
+    if tr is not None:
+        for k in kernels:
+            m = tr["kernels"].get(k["kernel"])
+            if m is not None:
+                k["traffic"] = m["bytes"]
+                k["traffic_source"] = tr["source"]
     result = {
         "metric": "PF-FLIP step particle-steps/s at 128^3/27ppc; MG-PCG ms to 1e-6, mixed vs FP32",
         "value": n_p * args.steps / elapsed,
@@ -335,7 +369,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded jittered lattice)",
-        "config": {"workload": f"dam break z-up {n}^3 x {args.ppc} ppc, 1000:1, MG-PCG tol 1e-6",
+        "config": {"workload": workload,
                    "grid": n, "ppc": args.ppc, "particles": n_p, "pcg_tol": 1e-6,
                    "solver_precision": args.precision, "parallelism": "single GPU"},
         "roofline": kernels[0],
