@@ -274,6 +274,121 @@ def adjoint_check(pf, torch, n=64, ppc=8, n_steps=16, n_dirs=8, eps=1e-3, tol=1e
             "directions": rows}
 
 
+def stage_rooflines(pf, sim, stages, n_p, n_c, s16):
+    """Per-kernel GB/s (algorithmic bytes / measured time) of the step's
+    dominant kernels: the level-0 sweep (HIP events via pf_mg_time_smoother),
+    P2G and G2P (per-stage HIP events averaged over the timed steps)."""
+    out = []
+    if sim.solver.small_from > 0:
+        ms = pf.poisson.time_smoother(sim.solver, 0, reps=10)
+        out.append(roofline(f"k_rbgs_full level-0 sweep ({sim.res[0]}^3)",
+                            fused_sweep_bytes(n_c, s16, s16), ms))
+    out.append(roofline("p2g stage (k_max_count + k_p2g + k_p2g_gather)", p2g_bytes(n_p, n_c), stages["p2g"]))
+    out.append(roofline("k_g2p_advect (G2P + RK3 + reflect)", g2p_bytes(n_p, n_c), stages["g2p_advect"]))
+    return out
+
+
+def run_rt128(pf, torch, args):
+    """Rayleigh-Taylor instability at 128^3 x 27 ppc, 1000:1, heavy over light
+    at half height with a seeded 8-mode interface perturbation (amplitude
+    0.02, wavenumbers 1-4), 60 steps, mixed-precision MG-PCG (BF16) to 1e-6.
+    alpha_phi = 0.5 and CFL = 0.5 as the reference's own Rayleigh-Taylor
+    run uses (scripts/run.py:42-43).  Reports per-kernel GB/s and the
+    V-cycles per step (one per FPCG iteration plus the first)."""
+    n, steps = 128, args.rt_steps
+    cfg = pf.Config(res=(n, n, n), dx=1.0 / n, ppc=27, gravity=(0.0, 0.0, -9.81), droplets=False,
+                    pcg_tol=1e-6, pcg_max_iter=400, solver_precision="bf16", alpha_phi=0.5, cfl=0.5)
+    pos, vel, pt = pf.scenes.rayleigh_taylor_modes(cfg)
+    sim = pf.Simulator(cfg)
+    sim.set_particles(pos, vel, pt)
+    n_p, n_c = len(pos), n ** 3
+    del pos, vel, pt
+    sim.stage_ms = {}
+    recs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        recs.append(sim.step())
+        if k % 10 == 9:
+            print(f"[bench] rt step {k + 1}: pcg_iter={recs[-1]['pcg_iter']}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stages = {k: v / steps for k, v in sim.stage_ms.items()}
+    sim.stage_ms = None
+    kernels = stage_rooflines(pf, sim, stages, n_p, n_c, 2)
+    return {"metric": "Rayleigh-Taylor 128^3 x 27 ppc particle-steps/s, 60 steps, BF16 MG-PCG",
+            "value": n_p * steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+            "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic (seeded jittered lattice)",
+            "config": {"workload": f"rayleigh-taylor z-up {n}^3 x 27 ppc, 8 modes amplitude 0.02, "
+                                   f"1000:1, alpha_phi 0.5, CFL 0.5, BF16 MG-PCG tol 1e-6",
+                       "particles": n_p},
+            "vcycles_per_step": [r["pcg_iter"] + 1 for r in recs],
+            "roofline_kernels": kernels, "stages_ms": stages}
+
+
+def run_surge256(pf, torch, args):
+    """Surge on a column at 256^3 x 27 ppc: liquid slab of depth 0.3 moving at
+    +2.0 along x in its top 0.1, a solid column of width 0.1 at x = 0.7 cut
+    from the grid (particles inside dropped), mixed-precision MG-PCG (BF16)
+    to 1e-6; timed forward steps, then one adjoint step from a checkpoint.
+    Reports the HBM footprint, particle-steps/s and roofline fractions."""
+    n = 256
+    cfg = pf.Config(res=(n, n, n), dx=1.0 / n, ppc=27, gravity=(0.0, 0.0, -9.81), droplets=False,
+                    pcg_tol=1e-6, pcg_max_iter=400, solver_precision="bf16")
+    t_seed = time.perf_counter()
+    pos, vel, pt, solid = pf.scenes.surge_column(cfg)
+    print(f"[bench] seeded {len(pos)} particles in {time.perf_counter() - t_seed:.1f} s",
+          file=sys.stderr, flush=True)
+    torch.cuda.reset_peak_memory_stats()
+    sim = pf.Simulator(cfg)
+    sim.set_solids(solid)
+    sim.set_particles(pos, vel, pt)
+    n_p, n_c = len(pos), n ** 3
+    del pos, vel, pt
+    for _ in range(args.warmup):
+        sim.step()
+    sim.stage_ms = {}
+    recs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        recs.append(sim.step())
+        print(f"[bench] surge step {k}: pcg_iter={recs[-1]['pcg_iter']} wall={recs[-1]['wall_s'] * 1e3:.1f} ms",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stages = {k: v / args.steps for k, v in sim.stage_ms.items()}
+    sim.stage_ms = None
+    kernels = stage_rooflines(pf, sim, stages, n_p, n_c, 2)
+    fwd_peak = torch.cuda.max_memory_allocated()
+    # one adjoint step with checkpointing: loss = liquid centre of mass
+    x0, v0 = sim.px.clone(), sim.pv.clone().requires_grad_(True)
+    lmask = sim.ptype == 1
+    adj = pf.StepAdjoint(sim)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    xT, _ = pf.rollout(sim, x0, v0, 1, adjoint=adj)
+    loss = (xT[lmask].double().mean(0) ** 2).sum()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    grad_norm = float(v0.grad.double().norm())
+    return {"metric": "surge 256^3 x 27 ppc particle-steps/s, BF16 MG-PCG, + one adjoint step",
+            "value": n_p * args.steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded jittered lattice)",
+            "config": {"workload": f"surge slab 0.3 (+2.0 in top 0.1) on a solid column, {n}^3 x 27 ppc, "
+                                   f"1000:1, BF16 MG-PCG tol 1e-6", "particles": n_p,
+                       "solid_cells": int(solid.sum())},
+            "hbm_bytes_forward_peak": fwd_peak, "hbm_bytes_with_adjoint_peak": torch.cuda.max_memory_allocated(),
+            "pcg_iters": [r["pcg_iter"] for r in recs], "roofline_kernels": kernels, "stages_ms": stages,
+            "adjoint_step": {"forward_ms": 1e3 * (t2 - t1), "backward_ms": 1e3 * (t3 - t2),
+                             "adjoint_pcg": list(adj.last_solve), "grad_norm": grad_norm}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,6 +400,10 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve-compare", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--workload", default="dambreak128", choices=["dambreak128", "rt128", "surge256"],
+                    help="dambreak128: the headline benchmark (default); rt128 / surge256: the "
+                         "Rayleigh-Taylor and large surge configurations (their own JSON line)")
+    ap.add_argument("--rt-steps", type=int, default=60)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy three times (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -308,6 +427,12 @@ def main():
                                                "launches": 3}}), flush=True)
         return
     pf = importlib.import_module("implementation-of-adaptive-phase-field-flip_amd")
+    if args.workload == "rt128":
+        print(json.dumps(run_rt128(pf, torch, args)), flush=True)
+        return
+    if args.workload == "surge256":
+        print(json.dumps(run_surge256(pf, torch, args)), flush=True)
+        return
 
     n = args.res
     cfg = pf.Config(res=(n, n, n), dx=1.0 / n, ppc=args.ppc, gravity=(0.0, 0.0, -9.81),

@@ -18,7 +18,8 @@ applies the transposed stages in libpfflip_hip.so:
     P2G                   pf_adj_p2g          (per particle; gather from the grid)
 
 dt is held at its forward value (the CFL step size is not differentiated),
-and so are the liquid tags, wall / solid masks and extrapolation masks.
+and so are the liquid tags, wall / solid masks and extrapolation masks; the
+solid push-out passes x-bar and zeroes the v-bar component it zeroes.
 """
 
 from __future__ import annotations
@@ -38,8 +39,6 @@ class StepAdjoint:
     """Adjoint buffers of one simulator and the backward pass of one step."""
 
     def __init__(self, sim: Simulator, tol: float = 1e-7, max_iter: int = 400):
-        if sim.has_solids:
-            raise NotImplementedError("the adjoint does not cover the solid push-out")
         if sim.cfg.k_div > 0.0:
             raise NotImplementedError("the adjoint does not cover k_div > 0")
         self.sim = sim

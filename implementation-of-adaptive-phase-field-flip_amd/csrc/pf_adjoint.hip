@@ -136,6 +136,7 @@ struct AdjG2PArgs {
     const float* vel[3];
     const float* dvel[3];
     const float* vel_liq[3];
+    const int* solid;
     float dx, dt, alpha_l, alpha_a;
     float lo[3], hi[3];
     const float* gx_out;  // adjoint of x'
@@ -208,6 +209,50 @@ __global__ __launch_bounds__(256) void k_adj_g2p_advect(AdjG2PArgs a) {
     for (int q = 0; q < 3; ++q) {
         xb[q] = a.gx_out[3 * p + q];
         vb[q] = a.gv_out[3 * p + q];
+    }
+    if ((a.flags & PF_GA_SOLIDS) && a.solid) {
+        // push_out_of_solids (kernels.py:630-679) after the reflection: the
+        // position moves by a constant, and an inward velocity component is
+        // zeroed (its adjoint with it)
+        float xr[3], vr[3];
+        for (int q = 0; q < 3; ++q) {
+            xr[q] = xc[q];
+            vr[q] = v1[q];
+            if (a.flags & PF_GA_REFLECT) {
+                if (xr[q] <= a.lo[q]) {
+                    xr[q] = a.lo[q];
+                    vr[q] = fmaxf(vr[q], 0.f);
+                } else if (xr[q] >= a.hi[q]) {
+                    xr[q] = a.hi[q];
+                    vr[q] = fminf(vr[q], 0.f);
+                }
+            }
+        }
+        const int i = min(max((int)(xr[0] / dx), 0), d.nx - 1);
+        const int j = min(max((int)(xr[1] / dx), 0), d.ny - 1);
+        const int k = min(max((int)(xr[2] / dx), 0), d.nz - 1);
+        if (a.solid[cell_index(d, i, j, k)] != 0) {
+            float best = 1.0e9f;
+            int bax = 0, bsg = 1;
+            for (int dd = 0; dd < 6; ++dd) {
+                const int ax = dd / 2, sg = (dd % 2 == 0) ? -1 : 1;
+                for (int t = 1; t < 8; ++t) {
+                    const int ii = i + (ax == 0 ? sg * t : 0);
+                    const int jj = j + (ax == 1 ? sg * t : 0);
+                    const int kk = k + (ax == 2 ? sg * t : 0);
+                    if (ii < 0 || jj < 0 || kk < 0 || ii >= d.nx || jj >= d.ny || kk >= d.nz) break;
+                    if (a.solid[cell_index(d, ii, jj, kk)] == 0) {
+                        if ((float)t < best) {
+                            best = (float)t;
+                            bax = ax;
+                            bsg = sg;
+                        }
+                        break;
+                    }
+                }
+            }
+            if (best <= 1.0e8f && (float)bsg * vr[bax] < 0.f) vb[bax] = 0.0f;
+        }
     }
     if (a.flags & PF_GA_REFLECT) {
         // kernels.py:567-595: a particle on a wall plane keeps that coordinate
@@ -637,7 +682,7 @@ PF_API int pf_adj_g2p_advect(const pf_g2p_args* fwd, const float* gx_out, const 
     const pf_g2p_args& in = *fwd;
     PF_REQUIRE(in.px && in.pv && in.ptype && in.pstate, "pf_adj_g2p_advect: need px, pv, ptype, pstate");
     PF_REQUIRE(!in.pflag, "pf_adj_g2p_advect: packed phase flags are not supported");
-    PF_REQUIRE(!(in.flags & PF_GA_SOLIDS), "pf_adj_g2p_advect: solids push-out has no adjoint here");
+    PF_REQUIRE(!(in.flags & PF_GA_SOLIDS) || in.solid, "pf_adj_g2p_advect: solids flag without mask");
     PF_REQUIRE(gx_out && gv_out && gx_in && gv_in && g_vel && g_dvel && g_vel_liq, "pf_adj_g2p_advect: null adjoint");
     PF_REQUIRE(in.nx > 0 && in.ny > 0 && in.nz > 0 && in.n >= 0 && in.dx > 0.0f, "pf_adj_g2p_advect: bad sizes");
     AdjG2PArgs a;
@@ -661,6 +706,7 @@ PF_API int pf_adj_g2p_advect(const pf_g2p_args* fwd, const float* gx_out, const 
         a.lo[q] = in.lo[q];
         a.hi[q] = in.hi[q];
     }
+    a.solid = in.solid;
     a.dx = in.dx;
     a.dt = in.dt;
     a.alpha_l = in.alpha_liquid;

@@ -294,8 +294,9 @@ int pf_max_speed(const float* pv, const float* pmass, const int32_t* pstate, int
  * forward values; the eq. (7) clamps use one-sided subgradients. */
 
 /* Transpose of pf_g2p_advect with the same args (px, pv: the step's input
- * state; vel, dvel, vel_liq: its grid fields; solids push-out and packed
- * flags unsupported).  Writes gx_in, gv_in ((n,3) adjoints of x, v) from
+ * state; vel, dvel, vel_liq: its grid fields; packed flags unsupported; the
+ * solids push-out moves x by a constant and zeroes an inward velocity
+ * component, so it passes x-bar and zeroes that component of v-bar).  Writes gx_in, gv_in ((n,3) adjoints of x, v) from
  * gx_out, gv_out (adjoints of x', v'); adds the grid adjoints into g_vel,
  * g_dvel, g_vel_liq with float atomics (the caller zeroes them). */
 int pf_adj_g2p_advect(const pf_g2p_args* fwd, const float* gx_out, const float* gv_out, float* gx_in,
