@@ -63,6 +63,23 @@ def roofline(name, nbytes, ms):
             "frac": gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": nbytes, "avg_ms": ms}
 
 
+def stream_copy_gbs(torch, nbytes=1 << 30, reps=10):
+    """Achievable HBM bandwidth on this GPU: a device-to-device copy of 1 GiB
+    (read + write bytes / time, HIP events)."""
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del x, y
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
 def measured_traffic(workload, precision):
     """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
     (copy-calibrated) for this workload, from profiles/traffic.json."""
@@ -82,7 +99,7 @@ def cpu_baseline(seconds_budget=30.0):
     from oracle import ref_np as O
     n = 32
     cfg = O.OracleConfig(res=(n, n, n), dx=1.0 / n, ppc=27, gravity=(0.0, 0.0, -9.81),
-                         pcg_tol=1e-6, pcg_max_iter=400)
+                         pcg_tol=1e-6, pcg_max_iter=400, droplets=False)
     sx, sy, sz = cfg.domain_size
     pos, vel, pt = O.seed_particles(cfg, lambda p: (p[:, 0] < 0.4 * sx) & (p[:, 2] < 0.6 * sz),
                                     jitter=0.5)
@@ -474,6 +491,9 @@ def main():
         roofline("k_g2p_advect (G2P + RK3 + reflect)", g2p_bytes(n_p, n_c), stages["g2p_advect"]),
     ]
     workload = f"dam break z-up {n}^3 x {args.ppc} ppc, 1000:1, MG-PCG tol 1e-6"
+    copy_gbs = stream_copy_gbs(torch)
+    for k in kernels:
+        k["frac_of_copy"] = k["achieved"] / copy_gbs
     tr = measured_traffic(workload, args.precision)
     if tr is not None:
         for k in kernels:
@@ -499,6 +519,7 @@ def main():
                    "solver_precision": args.precision, "parallelism": "single GPU"},
         "roofline": kernels[0],
         "roofline_kernels": kernels[1:],
+        "stream_copy_gbs": copy_gbs,
         "pcg_iters": [r["pcg_iter"] for r in recs],
         "stages_ms": stages,
     }

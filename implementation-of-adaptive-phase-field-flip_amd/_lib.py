@@ -76,6 +76,26 @@ class G2PArgs(ctypes.Structure):
     ]
 
 
+class DropletArgs(ctypes.Structure):
+    _fields_ = [
+        ("px", P), ("pv", P), ("ptype", P), ("pstate", P), ("pdrop_r", P),
+        ("n", ctypes.c_int64),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("seed", ctypes.c_int32),
+        ("phi", P), ("shear", P), ("vel", F3),
+        ("dx", ctypes.c_float), ("dt", ctypes.c_float), ("delta_phi", ctypes.c_float),
+        ("r_mu", ctypes.c_float), ("r_sigma", ctypes.c_float),
+        ("r_min", ctypes.c_float), ("r_max", ctypes.c_float),
+        ("spray_threshold", ctypes.c_float), ("spray_rate", ctypes.c_float),
+        ("gravity", ctypes.c_float * 3),
+        ("rho_g", ctypes.c_float), ("rho_l", ctypes.c_float), ("cd", ctypes.c_float),
+        ("evap_k", ctypes.c_float),
+        ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
+        ("max_substeps", ctypes.c_int32),
+        ("counters", P),
+    ]
+
+
 # name -> (restype, argtypes); mirrors include/pfflip.h
 I32, I64, F32, F64, SZ = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
 SIGNATURES = {
@@ -105,6 +125,10 @@ SIGNATURES = {
                                       ctypes.c_int, ctypes.c_int, P]),
     "pf_g2p_advect": (ctypes.c_int, [ctypes.POINTER(G2PArgs), P]),
     "pf_max_speed": (ctypes.c_int, [P, P, P, I64, P, P]),
+    "pf_droplets": (ctypes.c_int, [ctypes.POINTER(DropletArgs), P]),
+    "pf_velocity_gradient": (ctypes.c_int, [PP, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32, P]),
+    "pf_rasterize_spray": (ctypes.c_int, [P, P, P, I64, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          F32, P]),
     "pf_adj_g2p_advect": (ctypes.c_int, [ctypes.POINTER(G2PArgs), P, P, P, P, PP, PP, PP, P]),
     "pf_adj_extrapolate": (ctypes.c_int, [P, PP, PP, PP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, P]),

@@ -273,9 +273,9 @@ __device__ inline void st_pair(ST* p, int idx, float a, float b) {
 // and b, the shell cells' faces and b) is issued before the first wait, so
 // a workgroup is exposed to memory latency once.  Needs nz even.
 // ZINIT: the input is zero (no staging loads).
-template <int TI, int TJ, int TK>
+template <int TI, int TJ, int TK, int NTH>
 struct RbTile {
-    static constexpr int NT = 256;
+    static constexpr int NT = NTH;
     static constexpr int PI = TI + 4, PJ = TJ + 4, PK = TK + 4;
     static constexpr int NP = PI * PJ * PK;
     static constexpr int NPR = PK / 2;                    // staged pairs per row
@@ -292,15 +292,15 @@ struct RbTile {
     static_assert(PAIRS % NT == 0 && TK % 2 == 0 && TJ % 2 == 0 && PPT >= 1, "tile shape");
 };
 
-// (4 waves per SIMD: at 128^3 all 1024 workgroups of 8x8x32 tiles are
+// (4 waves per SIMD: a 128^3 level's 512 workgroups of 8x8x64 tiles are all
 // resident at once, so the sweep costs one memory round trip, not two)
-template <typename ST, bool ZINIT, int TI, int TJ, int TK>
-__global__ __launch_bounds__(256, 4) void k_rbgs_full(Dims d, const ST* __restrict__ bx,
+template <typename ST, bool ZINIT, int TI, int TJ, int TK, int NTH = 256>
+__global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restrict__ bx,
                                                    const ST* __restrict__ by, const ST* __restrict__ bz,
                                                    const ST* __restrict__ pin, ST* __restrict__ pout,
                                                    const float* __restrict__ b, int c1, float omega,
                                                    const int* __restrict__ done) {
-    using T = RbTile<TI, TJ, TK>;
+    using T = RbTile<TI, TJ, TK, NTH>;
     __shared__ float P[T::NP];
     const int tid = threadIdx.x;
     const int ntk = (d.nz + TK - 1) / TK, ntj = (d.ny + TJ - 1) / TJ;
@@ -1132,7 +1132,7 @@ static int launch_full_sweep(const Dims& d, const ST* fx, const ST* fy, const ST
         return (unsigned)(cdiv(d.nx, ti) * cdiv(d.ny, tj) * cdiv(d.nz, tk));
     };
     if (n >= (int64_t)1 << 20)
-        k_rbgs_full<ST, ZINIT, 8, 8, 32><<<tiles(8, 8, 32), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done);
+        k_rbgs_full<ST, ZINIT, 8, 8, 64, 512><<<tiles(8, 8, 64), 512, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done);
     else if (n >= (int64_t)1 << 17)
         k_rbgs_full<ST, ZINIT, 8, 8, 16><<<tiles(8, 8, 16), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done);
     else

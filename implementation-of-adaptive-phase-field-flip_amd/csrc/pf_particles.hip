@@ -592,7 +592,7 @@ struct G2PArgs {
     const float* dvel[3];
     const float* vel_liq[3];
     const int* solid;
-    float dx, dt, alpha_l, alpha_a;
+    float dx, inv_dx, dt, alpha_l, alpha_a;
     float lo[3], hi[3];
 };
 
@@ -608,13 +608,28 @@ struct Rows {
     int mode;     // 0: taps (kc, kc+1); 1: both at 0; 2: both at n2-1
 };
 
+// Per coordinate the two cell splits a sample needs: the face grid of the
+// same axis (offset 0) and the two others (offset 1/2).
+struct Coord {
+    int in, it;     // floor(g), floor(g - 1/2)
+    float fn, ft;   // the fractions
+};
+
+__device__ inline Coord coord(float g) {
+    Coord c;
+    c.in = (int)floorf(g);
+    c.fn = g - (float)c.in;
+    const float h = g - 0.5f;
+    c.it = (int)floorf(h);
+    c.ft = h - (float)c.it;
+    return c;
+}
+
 template <int A>
-__device__ inline void make_rows(float gx, float gy, float gz, const Dims& d, Rows<A>& t) {
-    const float ox = A == 0 ? 0.f : 0.5f, oy = A == 1 ? 0.f : 0.5f, oz = A == 2 ? 0.f : 0.5f;
+__device__ inline void make_rows(const Coord& cx, const Coord& cy, const Coord& cz, const Dims& d, Rows<A>& t) {
     const int n0 = d.nx + (A == 0), n1 = d.ny + (A == 1), n2 = d.nz + (A == 2);
-    const float hx = gx - ox, hy = gy - oy, hz = gz - oz;
-    const int i0 = (int)floorf(hx), j0 = (int)floorf(hy), k0 = (int)floorf(hz);
-    const float fx = hx - (float)i0, fy = hy - (float)j0, fz = hz - (float)k0;
+    const int i0 = A == 0 ? cx.in : cx.it, j0 = A == 1 ? cy.in : cy.it, k0 = A == 2 ? cz.in : cz.it;
+    const float fx = A == 0 ? cx.fn : cx.ft, fy = A == 1 ? cy.fn : cy.ft, fz = A == 2 ? cz.fn : cz.ft;
     const int kc = min(max(k0, 0), n2 - 2);
     t.mode = k0 < 0 ? 1 : (k0 > n2 - 2 ? 2 : 0);
 #pragma unroll
@@ -655,21 +670,40 @@ __device__ inline float sample_rows(const Rows<A>& t, const float* __restrict__ 
     return acc;
 }
 
-__device__ inline void sample_vel(const float* const* f, float x, float y, float z, float dx,
-                                  const Dims& d, float& u, float& v, float& w) {
-    const float gx = x / dx, gy = y / dx, gz = z / dx;
+struct Rows3 {
     Rows<0> t0;
     Rows<1> t1;
     Rows<2> t2;
-    make_rows(gx, gy, gz, d, t0);
-    make_rows(gx, gy, gz, d, t1);
-    make_rows(gx, gy, gz, d, t2);
-    u = sample_rows(t0, f[0]);
-    v = sample_rows(t1, f[1]);
-    w = sample_rows(t2, f[2]);
+};
+
+__device__ inline void make_rows3(float x, float y, float z, float inv_dx, const Dims& d, Rows3& r) {
+    const Coord cx = coord(x * inv_dx), cy = coord(y * inv_dx), cz = coord(z * inv_dx);
+    make_rows(cx, cy, cz, d, r.t0);
+    make_rows(cx, cy, cz, d, r.t1);
+    make_rows(cx, cy, cz, d, r.t2);
 }
 
-__global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
+// Two RK3 stages through the face fields f (uniform pointers: the caller
+// branches on the particle's phase), from k1 at x (kernels.py:508-549).
+__device__ inline void rk3_rest(const float* __restrict__ f0, const float* __restrict__ f1,
+                                const float* __restrict__ f2, float& x, float& y, float& z, float k1x, float k1y,
+                                float k1z, float dt, float inv_dx, const Dims& d) {
+    const float h1 = 0.5f * dt, h2 = 0.75f * dt;
+    Rows3 r;
+    make_rows3(x + h1 * k1x, y + h1 * k1y, z + h1 * k1z, inv_dx, d, r);
+    const float k2x = sample_rows(r.t0, f0), k2y = sample_rows(r.t1, f1), k2z = sample_rows(r.t2, f2);
+    make_rows3(x + h2 * k2x, y + h2 * k2y, z + h2 * k2z, inv_dx, d, r);
+    const float k3x = sample_rows(r.t0, f0), k3y = sample_rows(r.t1, f1), k3z = sample_rows(r.t2, f2);
+    const float c1 = 2.0f / 9.0f, c2 = 3.0f / 9.0f, c3 = 4.0f / 9.0f;
+    x = x + dt * (c1 * k1x + c2 * k2x + c3 * k3x);
+    y = y + dt * (c1 * k1y + c2 * k2y + c3 * k3y);
+    z = z + dt * (c1 * k1z + c2 * k2z + c3 * k3z);
+}
+
+// At most 5 waves per SIMD: the kernel is bound by its 60 row-pair gathers
+// per particle, and more resident waves evict each other's rows from L1/L2
+// (measured at 128^3 x 27 ppc: 7 waves/SIMD is 10 % slower than 5).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) void k_g2p_advect(G2PArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
     bool liquid, active, dead;
@@ -688,7 +722,6 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
     float x = a.px[3 * p], y = a.px[3 * p + 1], z = a.px[3 * p + 2];
     float vx = a.pv[3 * p], vy = a.pv[3 * p + 1], vz = a.pv[3 * p + 2];
     const Dims d = a.d;
-    const float dx = a.dx;
     const bool g2p = active && (a.flags & PF_GA_G2P);
     const bool adv = active && (a.flags & PF_GA_ADVECT);
 
@@ -696,24 +729,19 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
     // first RK3 stage (liquid: I(vel_liq); air: I(vel) = the PIC sample)
     float k1x = 0.f, k1y = 0.f, k1z = 0.f;
     if (g2p || adv) {
-        const float gx = x / dx, gy = y / dx, gz = z / dx;
-        Rows<0> t0;
-        Rows<1> t1;
-        Rows<2> t2;
-        make_rows(gx, gy, gz, d, t0);
-        make_rows(gx, gy, gz, d, t1);
-        make_rows(gx, gy, gz, d, t2);
+        Rows3 r;
+        make_rows3(x, y, z, a.inv_dx, d, r);
         float pic[3] = {0.f, 0.f, 0.f};
         if (g2p || !liquid) {
-            pic[0] = sample_rows(t0, a.vel[0]);
-            pic[1] = sample_rows(t1, a.vel[1]);
-            pic[2] = sample_rows(t2, a.vel[2]);
+            pic[0] = sample_rows(r.t0, a.vel[0]);
+            pic[1] = sample_rows(r.t1, a.vel[1]);
+            pic[2] = sample_rows(r.t2, a.vel[2]);
         }
         if (adv) {
             if (liquid) {
-                k1x = sample_rows(t0, a.vel_liq[0]);
-                k1y = sample_rows(t1, a.vel_liq[1]);
-                k1z = sample_rows(t2, a.vel_liq[2]);
+                k1x = sample_rows(r.t0, a.vel_liq[0]);
+                k1y = sample_rows(r.t1, a.vel_liq[1]);
+                k1z = sample_rows(r.t2, a.vel_liq[2]);
             } else {
                 k1x = pic[0];
                 k1y = pic[1];
@@ -722,8 +750,8 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
         }
         if (g2p) {
             // eq. (12): v = alpha (v + I(du)) + (1 - alpha) I(u)
-            const float dux = sample_rows(t0, a.dvel[0]), duy = sample_rows(t1, a.dvel[1]),
-                        duz = sample_rows(t2, a.dvel[2]);
+            const float dux = sample_rows(r.t0, a.dvel[0]), duy = sample_rows(r.t1, a.dvel[1]),
+                        duz = sample_rows(r.t2, a.dvel[2]);
             const float al = liquid ? a.alpha_l : a.alpha_a;
             vx = al * (vx + dux) + (1.0f - al) * pic[0];
             vy = al * (vy + duy) + (1.0f - al) * pic[1];
@@ -731,18 +759,11 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
         }
     }
     if (adv) {
-        // Ralston RK3 (kernels.py:508-549); liquid samples vel_liq, air vel
-        const float* f[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) f[q] = liquid ? a.vel_liq[q] : a.vel[q];
-        const float h1 = 0.5f * a.dt, h2 = 0.75f * a.dt;
-        float k2x, k2y, k2z, k3x, k3y, k3z;
-        sample_vel(f, x + h1 * k1x, y + h1 * k1y, z + h1 * k1z, dx, d, k2x, k2y, k2z);
-        sample_vel(f, x + h2 * k2x, y + h2 * k2y, z + h2 * k2z, dx, d, k3x, k3y, k3z);
-        const float c1 = 2.0f / 9.0f, c2 = 3.0f / 9.0f, c3 = 4.0f / 9.0f;
-        x = x + a.dt * (c1 * k1x + c2 * k2x + c3 * k3x);
-        y = y + a.dt * (c1 * k1y + c2 * k2y + c3 * k3y);
-        z = z + a.dt * (c1 * k1z + c2 * k2z + c3 * k3z);
+        // Ralston RK3; liquid samples vel_liq, air vel
+        if (liquid)
+            rk3_rest(a.vel_liq[0], a.vel_liq[1], a.vel_liq[2], x, y, z, k1x, k1y, k1z, a.dt, a.inv_dx, d);
+        else
+            rk3_rest(a.vel[0], a.vel[1], a.vel[2], x, y, z, k1x, k1y, k1z, a.dt, a.inv_dx, d);
         x = fminf(fmaxf(x, a.lo[0]), a.hi[0]);
         y = fminf(fmaxf(y, a.lo[1]), a.hi[1]);
         z = fminf(fmaxf(z, a.lo[2]), a.hi[2]);
@@ -769,9 +790,9 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
     }
     if ((a.flags & PF_GA_SOLIDS) && a.solid) {
         // kernels.py:630-679
-        const int i = min(max((int)(x / dx), 0), d.nx - 1);
-        const int j = min(max((int)(y / dx), 0), d.ny - 1);
-        const int k = min(max((int)(z / dx), 0), d.nz - 1);
+        const int i = min(max((int)(x / a.dx), 0), d.nx - 1);
+        const int j = min(max((int)(y / a.dx), 0), d.ny - 1);
+        const int k = min(max((int)(z / a.dx), 0), d.nz - 1);
         if (a.solid[cell_index(d, i, j, k)] != 0) {
             float best = 1.0e9f;
             int bax = 0, bsg = 1;
@@ -794,7 +815,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
                 }
             }
             if (best <= 1.0e8f) {
-                const float off = (best + 0.5f) * dx * (float)bsg;
+                const float off = (best + 0.5f) * a.dx * (float)bsg;
                 float xs[3] = {x, y, z}, vs[3] = {vx, vy, vz};
                 xs[bax] = xs[bax] + off;
                 if ((float)bsg * vs[bax] < 0.f) vs[bax] = 0.f;
@@ -1010,6 +1031,9 @@ PF_API int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream) {
     }
     a.solid = in.solid;
     a.dx = in.dx;
+    // grid coordinates are x * (1/dx): the same bits as x / dx when dx is a
+    // power of two (every grid here), within an ulp otherwise
+    a.inv_dx = 1.0f / in.dx;
     a.dt = in.dt;
     a.alpha_l = in.alpha_liquid;
     a.alpha_a = in.alpha_air;

@@ -11,15 +11,19 @@ reference's shapes and dtypes; every stage runs in libpfflip_hip.so.
 ``step`` fuses stages that the reference launches separately (it produces
 the same fields): binning + tiled P2G; eq. (7) / beta / walls / vel_star /
 gravity / RHS in one grid pass; projection + FLIP delta; G2P + RK3 +
-reflection in one particle pass.  The stage methods (``_p2g``,
+reflection in one particle pass (split around the droplet stage when
+``cfg.droplets`` is on: the stage runs between G2P and advection).  The stage methods (``_p2g``,
 ``_phase_field_and_coefficients``, ``_add_forces``, ``_pressure_solve``,
 ``_project``, ``_max_divergence``, ``_g2p``, ``_extrapolate_liquid_velocity``,
 ``_advect``) keep their individual reference semantics.
 
-Not implemented: the escaped-particle / droplet stage (pfflip/sim.py:343-401)
-and ``rasterize_spray``.  With ``cfg.droplets`` True the stage is skipped
-with a warning.  Kernel radius: ``prad`` must stay ``cfg.r_p`` (= dx), which
-is what the reference's uniform path always sets (sim.py:153).
+The escaped-particle / droplet stage (``_droplets``, pfflip/sim.py:343-390)
+and ``rasterize_spray`` (sim.py:392-401) run in ``pf_droplets`` /
+``pf_velocity_gradient`` / ``pf_rasterize_spray``.  Droplet radii come from
+a counter-based PCG-hash stream keyed by (cfg.seed + step_index, particle)
+like the reference's Warp RNG; the streams are not the same numbers.
+Kernel radius: ``prad`` must stay ``cfg.r_p`` (= dx), which is what the
+reference's uniform path always sets (sim.py:153).
 """
 
 from __future__ import annotations
@@ -27,7 +31,6 @@ from __future__ import annotations
 import ctypes
 import math
 import time
-import warnings
 
 import numpy as np
 import torch
@@ -92,7 +95,21 @@ class Simulator:
         self.solid = zi(self.res)
         self.pressure = z(self.res)
         self.rhs = z(self.res)
-        self._scalar = z((4,))
+        # [max |div u| (float bits), 3 spare, droplet counters x5, spare]:
+        # read back with one copy at the end of a step
+        self._scalar = zi((10,))
+
+        # spray density at spray_resolution_factor x the grid, FP16
+        # (sim.py:55-60), and the cell |grad u| used for spray emission
+        f = max(1, int(cfg.spray_resolution_factor))
+        self.spray_res = tuple(n * f for n in self.res)
+        self.dx_spray = cfg.dx / f
+        ns = math.prod(self.spray_res)
+        # FP16 atomics update aligned 4-byte words: an odd cell count gets
+        # one FP16 of padding so the last word stays inside the allocation
+        self._spray_buf = torch.zeros(ns + (ns & 1), dtype=torch.float16, device=dev)
+        self.spray_density = self._spray_buf[:ns].view(self.spray_res)
+        self.shear = z(self.res)
 
         self.px = self.pv = self.ptype = self.pmass = self.prad = self.pstate = self.pdrop_r = None
         self.n_particles = 0
@@ -111,7 +128,6 @@ class Simulator:
         self.time = 0.0
         self.step_index = 0
         self.stats: list[dict] = []
-        self._warned_droplets = False
 
         lib = _lib.load()
         self._nkeys = int(lib.pf_bin_nkeys(nx, ny, nz))
@@ -161,6 +177,14 @@ class Simulator:
         _lib.call("pf_read_scalars", _lib.ptr(t), ctypes.cast(out, ctypes.c_void_p),
                   4 * count, _lib.stream())
         return list(out)
+
+    def _read_scalar_block(self):
+        """(max |div u|, [emitted, bubbles, merged, spray, alive droplets])."""
+        out = (ctypes.c_int32 * 10)()
+        _lib.call("pf_read_scalars", _lib.ptr(self._scalar), ctypes.cast(out, ctypes.c_void_p),
+                  40, _lib.stream())
+        div = ctypes.c_float.from_buffer_copy(ctypes.c_int32(out[0])).value
+        return div, list(out[4:9])
 
     def _lohi(self):
         eps = 1e-4 * self.cfg.dx
@@ -272,17 +296,13 @@ class Simulator:
         return float(np.clip(cfg.cfl * cfg.dx / vmax, cfg.dt_min, cfg.dt_max))
 
     def step(self, dt: float | None = None, verbose: bool = False, tags=None) -> dict:
-        """One time step (pfflip/sim.py:168-202); droplet stage not included.
+        """One time step (pfflip/sim.py:168-202).
 
         ``tags`` (optional, an int32 cell array) replays the liquid tags of
         another run instead of deriving them from this step's cell mass,
         e.g. to difference a rollout along the branch its adjoint follows
         (the adjoint holds the tags fixed)."""
         cfg = self.cfg
-        if cfg.droplets and not self._warned_droplets:
-            warnings.warn("Config.droplets=True: the escaped-particle / droplet stage is not "
-                          "implemented and is skipped", RuntimeWarning, stacklevel=2)
-            self._warned_droplets = True
         if dt is None:
             dt = self.dt_from_cfl()     # also leaves the P2G bounds on the device
         else:
@@ -314,16 +334,28 @@ class Simulator:
         self._mark("project")
         self._extrapolate_liquid_velocity()
         self._mark("extrapolate")
-        a = self._g2p_args(_lib.GA_G2P | _lib.GA_ADVECT | _lib.GA_REFLECT, dt)
-        _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
+        if cfg.droplets:
+            # G2P, then the droplet stage, then advection (sim.py:180-185)
+            a = self._g2p_args(_lib.GA_G2P, dt)
+            _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
+            self._launch_droplets(dt)
+            a = self._g2p_args(_lib.GA_ADVECT | _lib.GA_REFLECT, dt)
+            _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
+        else:
+            a = self._g2p_args(_lib.GA_G2P | _lib.GA_ADVECT | _lib.GA_REFLECT, dt)
+            _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
         self._mark("g2p_advect")
-        div_after = float(self._read(self._scalar)[0])   # synchronises
+        div_after, counters = self._read_scalar_block()   # synchronises
+        n_drop = 0
+        if cfg.droplets:
+            self._add_droplet_stats(counters)
+            n_drop = counters[4]
         wall = time.perf_counter() - t0
         self._collect_marks()
         self.time += dt
         self.step_index += 1
         rec = {"step": self.step_index, "t": self.time, "dt": dt, "pcg_iter": pcg_it,
-               "pcg_res": pcg_res, "max_div": div_after, "droplets": 0, "wall_s": wall}
+               "pcg_res": pcg_res, "max_div": div_after, "droplets": n_drop, "wall_s": wall}
         self.stats.append(rec)
         return rec
 
@@ -378,6 +410,65 @@ class Simulator:
         _lib.call("pf_extrapolate", _lib.ptr3(self.vel), _lib.ptr(self.is_liquid),
                   _lib.ptr3(self.vel_liq), _lib.ptr3(self._tmp_face), _lib.ptr3(self._known),
                   _lib.ptr3(self._known2), nx, ny, nz, int(self.cfg.extrap_cells), _lib.stream())
+
+    def _droplet_args(self, dt: float):
+        cfg = self.cfg
+        sigma = float(cfg.droplet_radius_spread)
+        mu = math.log(cfg.droplet_radius_mean * cfg.dx) - 0.5 * sigma * sigma   # droplets.py:355-360
+        a = _lib.DropletArgs()
+        a.px, a.pv = self.px.data_ptr(), self.pv.data_ptr()
+        a.ptype, a.pstate, a.pdrop_r = self.ptype.data_ptr(), self.pstate.data_ptr(), self.pdrop_r.data_ptr()
+        a.n = self.n_particles
+        a.nx, a.ny, a.nz = self.res
+        a.seed = int(cfg.seed + self.step_index)
+        a.phi = self.phi_cell.data_ptr()
+        a.shear = self.shear.data_ptr() if cfg.spray_shear_threshold > 0.0 else None
+        a.vel = _lib.ptr3(self.vel)
+        a.dx, a.dt, a.delta_phi = cfg.dx, float(dt), cfg.delta_phi
+        a.r_mu, a.r_sigma = mu, sigma
+        a.r_min, a.r_max = 0.02 * cfg.dx, 2.0 * cfg.dx
+        a.spray_threshold, a.spray_rate = cfg.spray_shear_threshold, cfg.spray_emission_rate
+        a.gravity = (ctypes.c_float * 3)(*[float(g) for g in cfg.gravity])
+        a.rho_g, a.rho_l, a.cd = cfg.rho_g, cfg.rho_l, cfg.droplet_drag_cd
+        a.evap_k = cfg.evaporation_k * cfg.dx * cfg.dx
+        lo, hi = self._lohi()
+        a.lo = (ctypes.c_float * 3)(*lo)
+        a.hi = (ctypes.c_float * 3)(*hi)
+        a.max_substeps = int(cfg.droplet_max_substeps)
+        a.counters = self._scalar.data_ptr() + 16
+        return a
+
+    def _launch_droplets(self, dt: float):
+        """Shear field (if spray emission is on), then one pf_droplets pass;
+        the counters land in the scalar block."""
+        cfg = self.cfg
+        self._scalar[4:9].zero_()
+        if cfg.spray_shear_threshold > 0.0:
+            nx, ny, nz = self.res
+            _lib.call("pf_velocity_gradient", _lib.ptr3(self.vel), _lib.ptr(self.shear), nx, ny, nz,
+                      cfg.dx, _lib.stream())
+        a = self._droplet_args(dt)
+        _lib.call("pf_droplets", ctypes.byref(a), _lib.stream())
+
+    def _add_droplet_stats(self, counters):
+        for k, i in (("emitted", 0), ("bubbles", 1), ("merged", 2), ("spray", 3)):
+            self.droplet_stats[k] += int(counters[i])
+
+    def _droplets(self, dt: float):
+        """Escaped particles -> droplets, droplet dynamics (pfflip/sim.py:343-390)."""
+        if not self.cfg.droplets:
+            return
+        self._launch_droplets(dt)
+        self._add_droplet_stats(self._read_scalar_block()[1])
+
+    def rasterize_spray(self):
+        """Droplet volume into the FP16 spray field (pfflip/sim.py:392-401)."""
+        self.spray_density.zero_()
+        sx, sy, sz = self.spray_res
+        _lib.call("pf_rasterize_spray", _lib.ptr(self.px), _lib.ptr(self.pstate), _lib.ptr(self.pdrop_r),
+                  self.n_particles, _lib.ptr(self.spray_density), sx, sy, sz, float(self.dx_spray),
+                  _lib.stream())
+        return self.spray_density
 
     def _advect(self, dt: float):
         """RK3, wall reflection, solid push-out (pfflip/sim.py:408-424)."""

@@ -282,6 +282,54 @@ int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream);
 int pf_max_speed(const float* pv, const float* pmass, const int32_t* pstate, int64_t n,
                  float* out, pf_stream_t stream);
 
+/* ---------------------------------------------------------------- droplets
+ * Replaces _droplets (pfflip/sim.py:343-390, droplets.py:67-278).  Per
+ * particle, in this order: classify_escaped (liquid FLIP particle where the
+ * interpolated cell phi < 1/2 - delta_phi -> droplet with a log-normal radius;
+ * air FLIP particle where phi > 1/2 + delta_phi -> dead; droplet where
+ * phi > 1/2 + delta_phi -> FLIP), emit_spray_in_shear (only if shear is
+ * non-NULL), advance_droplets (gravity + sphere drag against vel, per-droplet
+ * sub-steps, optional evaporation, death on leaving [lo, hi]).  seed is the
+ * reference's cfg.seed + step_index.  counters: device int32[5], added to
+ * (not cleared): emitted, bubbles, merged, spray, droplets alive after the
+ * stage. */
+typedef struct pf_droplet_args {
+    float* px;
+    float* pv;
+    const int32_t* ptype;
+    int32_t* pstate;
+    float* pdrop_r;
+    int64_t n;
+    int32_t nx, ny, nz;
+    int32_t seed;
+    const float* phi;
+    const float* shear;
+    const float* vel[3];
+    float dx, dt, delta_phi;
+    float r_mu, r_sigma, r_min, r_max;
+    float spray_threshold, spray_rate;
+    float gravity[3];
+    float rho_g, rho_l, cd, evap_k;
+    float lo[3], hi[3];
+    int32_t max_substeps;
+    int32_t* counters;
+} pf_droplet_args;
+
+int pf_droplets(const pf_droplet_args* args, pf_stream_t stream);
+
+/* velocity_gradient_magnitude (droplets.py:156-196): out (cells) = |grad u|
+ * from the MAC faces with the reference's seven terms. */
+int pf_velocity_gradient(const float* const vel[3], float* out, int nx, int ny, int nz, float dx,
+                         pf_stream_t stream);
+
+/* rasterize_spray (droplets.py:308-351, sim.py:392-401): adds droplet volume
+ * trilinearly into density (FP16 bits, sx*sy*sz cells of size dx_spray,
+ * 4-byte aligned), each contribution rounded to FP16 and added atomically.
+ * The atomics update whole 4-byte words, so when sx*sy*sz is odd the
+ * allocation must hold one more FP16 after the last cell (it is added 0). */
+int pf_rasterize_spray(const float* px, const int32_t* pstate, const float* pdrop_r, int64_t n,
+                       uint16_t* density, int sx, int sy, int sz, float dx_spray, pf_stream_t stream);
+
 /* ---------------------------------------------------------------- adjoint
  * Reverse-mode derivative of one step with dt held fixed (no reference
  * counterpart: the reference has no adjoint).  Each entry point transposes

@@ -72,6 +72,17 @@ class OracleConfig:
         self.mg_min_dim = 4
         self.extrap_cells = 3
         self.seed = 42
+        # escaped particles / spray (pfflip/config.py:82-110)
+        self.droplets = True
+        self.delta_phi = 0.2
+        self.droplet_radius_mean = 0.25
+        self.droplet_radius_spread = 0.5
+        self.droplet_drag_cd = 0.5
+        self.evaporation_k = 0.0
+        self.spray_shear_threshold = 0.0
+        self.spray_emission_rate = 0.02
+        self.spray_resolution_factor = 2
+        self.droplet_max_substeps = 8
         for k, v in kw.items():
             setattr(self, k, v)
 
@@ -739,13 +750,243 @@ class MGPCGOracle:
 
 
 # ---------------------------------------------------------------------------
-# the whole step (pfflip/sim.py:168-342, 408-424), droplets off
+# escaped particles, droplets, spray (pfflip/droplets.py, sim.py:343-401)
+# ---------------------------------------------------------------------------
+# Random stream.  The reference draws radii from Warp's rand_init/randn
+# (warp-lang, version not pinned by the reference), an absent dependency.
+# Restated here: the PCG hash of Jarzynski & Olano (JCGT 2020), a stream
+# state = hash(seed + hash(offset)), uniforms from the top 24 bits of each
+# new state, normals by Box-Muller.  The product uses the same stream, so
+# radii agree to the ulps of exp/log/cos; whether these are Warp's numbers
+# is parity unpinned (only their distribution is checked).
+_M32 = np.uint64(0xFFFFFFFF)
+
+
+def pcg_hash(v):
+    """PCG RXS-M-XS hash of uint32 values (Jarzynski & Olano 2020)."""
+    v = np.asarray(v, np.uint64) & _M32
+    st = (v * np.uint64(747796405) + np.uint64(2891336453)) & _M32
+    w = (((st >> ((st >> np.uint64(28)) + np.uint64(4))) ^ st) * np.uint64(277803737)) & _M32
+    return ((w >> np.uint64(22)) ^ w) & _M32
+
+
+def rng_init(seed, offset):
+    """Stream state of particle ``offset`` (droplets.py:91, 149):
+    hash(seed + hash(offset))."""
+    seed_u = np.asarray(seed, np.int64).astype(np.uint64) & _M32
+    return pcg_hash((seed_u + pcg_hash(np.asarray(offset, np.int64).astype(np.uint64) & _M32)) & _M32)
+
+
+def rng_uniform(state):
+    """-> (new state, float32 uniform in [0, 1) from the top 24 bits)."""
+    state = pcg_hash(state)
+    return state, ((state >> np.uint64(8)).astype(np.float64) / 16777216.0).astype(np.float32)
+
+
+def rng_normal(state):
+    """-> (new state, standard normal), Box-Muller; u1 is floored at 2^-25."""
+    state, u1 = rng_uniform(state)
+    state, u2 = rng_uniform(state)
+    u1 = np.maximum(u1.astype(np.float64), 2.0 ** -25)
+    return state, np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2.astype(np.float64))
+
+
+def lognormal_params(mean_radius, spread=0.5):
+    """droplets.py:355-360: (mu, sigma) of a log-normal with the given mean."""
+    sigma = spread
+    return math.log(mean_radius) - 0.5 * sigma * sigma, sigma
+
+
+def sample_cell(f, x, dx):
+    """droplets.py:42-61: trilinear at cell centres, indices clamped."""
+    g = x / dx - 0.5
+    i0 = np.floor(g).astype(np.int64)
+    t = g - i0
+    acc = np.zeros(len(x))
+    n = f.shape
+    for a in range(2):
+        wa = 1.0 - t[:, 0] if a == 0 else t[:, 0]
+        ii = np.clip(i0[:, 0] + a, 0, n[0] - 1)
+        for b in range(2):
+            wb = 1.0 - t[:, 1] if b == 0 else t[:, 1]
+            jj = np.clip(i0[:, 1] + b, 0, n[1] - 1)
+            for c in range(2):
+                wc = 1.0 - t[:, 2] if c == 0 else t[:, 2]
+                kk = np.clip(i0[:, 2] + c, 0, n[2] - 1)
+                acc += wa * wb * wc * f[ii, jj, kk]
+    return acc
+
+
+def _draw_radius(state, mu, sigma, r_min, r_max):
+    state, z = rng_normal(state)
+    return state, np.clip(np.exp(mu + sigma * z), r_min, r_max)
+
+
+def classify_escaped(px, ptype, pstate, pdrop_r, phi, dx, delta_phi, mu, sigma, r_min, r_max, seed):
+    """droplets.py:67-106 -> (pstate, pdrop_r, [emitted, bubbles, merged], f)."""
+    st = np.asarray(pstate, np.int32).copy()
+    r = np.asarray(pdrop_r, np.float64).copy()
+    f = sample_cell(phi, np.asarray(px, np.float64), dx)
+    idx = np.arange(len(st))
+    flip, drop = st == STATE_FLIP, st == STATE_DROPLET
+    emit = flip & (ptype == LIQUID) & (f < 0.5 - delta_phi)
+    bubble = flip & (ptype != LIQUID) & (f > 0.5 + delta_phi)
+    merge = drop & (f > 0.5 + delta_phi)
+    if emit.any():
+        _, r[emit] = _draw_radius(rng_init(seed, idx[emit]), mu, sigma, r_min, r_max)
+    st[emit] = STATE_DROPLET
+    st[bubble] = STATE_DEAD
+    st[merge] = STATE_FLIP
+    return st, r, [int(emit.sum()), int(bubble.sum()), int(merge.sum())], f
+
+
+def velocity_gradient_magnitude(vel, dx):
+    """droplets.py:156-196: |grad u| at cells from the seven terms the
+    reference sums (dv/dz and dw/dy are not among them)."""
+    u, v, w = (np.asarray(a, np.float64) for a in vel)
+    nx, ny, nz = w.shape[0], u.shape[1], u.shape[2]
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    im, ip = np.maximum(i - 1, 0), np.minimum(i + 1, nx - 1)
+    jm, jp = np.maximum(j - 1, 0), np.minimum(j + 1, ny - 1)
+    km, kp = np.maximum(k - 1, 0), np.minimum(k + 1, nz - 1)
+    dudx = (u[i + 1, j, k] - u[i, j, k]) / dx
+    dvdy = (v[i, j + 1, k] - v[i, j, k]) / dx
+    dwdz = (w[i, j, k + 1] - w[i, j, k]) / dx
+    dudy = (0.5 * (u[i, jp, k] + u[i + 1, jp, k]) - 0.5 * (u[i, jm, k] + u[i + 1, jm, k])) / (2 * dx)
+    dudz = (0.5 * (u[i, j, kp] + u[i + 1, j, kp]) - 0.5 * (u[i, j, km] + u[i + 1, j, km])) / (2 * dx)
+    dvdx = (0.5 * (v[ip, j, k] + v[ip, j + 1, k]) - 0.5 * (v[im, j, k] + v[im, j + 1, k])) / (2 * dx)
+    dwdx = (0.5 * (w[ip, j, k] + w[ip, j, k + 1]) - 0.5 * (w[im, j, k] + w[im, j, k + 1])) / (2 * dx)
+    return np.sqrt(dudx ** 2 + dvdy ** 2 + dwdz ** 2 + dudy ** 2 + dudz ** 2 + dvdx ** 2 + dwdx ** 2)
+
+
+def emit_spray_in_shear(px, ptype, pstate, pdrop_r, phi, shear, dx, threshold, rate, mu, sigma,
+                        r_min, r_max, seed):
+    """droplets.py:112-153 -> (pstate, pdrop_r, n_spray)."""
+    st = np.asarray(pstate, np.int32).copy()
+    r = np.asarray(pdrop_r, np.float64).copy()
+    x = np.asarray(px, np.float64)
+    f = sample_cell(phi, x, dx)
+    sh = sample_cell(shear, x, dx)
+    cand = (st == STATE_FLIP) & (ptype == LIQUID) & ~((f > 0.9) | (f < 0.1)) & ~(sh < threshold)
+    idx = np.flatnonzero(cand)
+    state, u = rng_uniform(rng_init(seed + 104729, idx))
+    go = ~(u.astype(np.float64) > rate)
+    idx, state = idx[go], state[go]
+    if len(idx):
+        _, r[idx] = _draw_radius(state, mu, sigma, r_min, r_max)
+    st[idx] = STATE_DROPLET
+    return st, r, len(idx)
+
+
+def drag_acceleration(vrel, r, rho_g, rho_l, cd):
+    """droplets.py:200-208: 3 rho_g Cd |w| w / (8 rho_l r), 0 if |w| or r is 0."""
+    s = np.linalg.norm(vrel, axis=-1)
+    ok = (s > 0.0) & (r > 0.0)
+    c = np.where(ok, 3.0 * rho_g * cd * s / (8.0 * rho_l * np.where(ok, r, 1.0)), 0.0)
+    return c[:, None] * vrel
+
+
+def advance_droplets(px, pv, pstate, pdrop_r, vel, dt, dx, gravity, rho_g, rho_l, cd, evap_k,
+                     r_min, lo, hi, max_substeps):
+    """droplets.py:211-278 -> (px, pv, pstate, pdrop_r).  Semi-implicit
+    Euler sub-steps, count from the droplet's own CFL; evaporation
+    dr/dt = -k/r by midpoint RK2; a droplet that leaves [lo, hi] dies (and,
+    like one that evaporates, keeps its pre-advance x, v and r)."""
+    x = np.asarray(px, np.float64).copy()
+    v = np.asarray(pv, np.float64).copy()
+    st = np.asarray(pstate, np.int32).copy()
+    r = np.asarray(pdrop_r, np.float64).copy()
+    d = np.flatnonzero(st == STATE_DROPLET)
+    if len(d) == 0:
+        return x, v, st, r
+    xd, vd, rd = x[d].copy(), v[d].copy(), r[d].copy()
+    speed = np.linalg.norm(vd, axis=1) + 1e-8
+    nsub = np.clip(np.floor(np.minimum(speed * dt / (0.5 * dx), 1e9)).astype(np.int64) + 1, 1,
+                   max_substeps)
+    h = dt / nsub
+    alive = np.ones(len(d), bool)
+    g = np.asarray(gravity, np.float64)
+    for s in range(int(nsub.max())):
+        m = alive & (s < nsub)
+        if not m.any():
+            break
+        ua = sample_velocity(vel, xd[m], dx)
+        a = g[None, :] + drag_acceleration(ua - vd[m], rd[m], rho_g, rho_l, cd)
+        vd[m] = vd[m] + h[m][:, None] * a
+        xd[m] = xd[m] + h[m][:, None] * vd[m]
+        if evap_k > 0.0:
+            e = np.flatnonzero(m)
+            e = e[rd[e] > 0.0]
+            k1 = -evap_k / rd[e]
+            rm = np.maximum(rd[e] + 0.5 * h[e] * k1, 1e-9)
+            rd[e] = rd[e] + h[e] * (-evap_k / rm)
+            alive[e[rd[e] <= r_min]] = False
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    out = alive & ((xd <= lo).any(axis=1) | (xd >= hi).any(axis=1))
+    keep = alive & ~out
+    st[d[~keep]] = STATE_DEAD
+    x[d[keep]], v[d[keep]], r[d[keep]] = xd[keep], vd[keep], rd[keep]
+    return x, v, st, r
+
+
+def rasterize_spray(px, pstate, pdrop_r, shape, dx_spray):
+    """droplets.py:308-351 in FP64 (the reference adds FP16 atomics)."""
+    dens = np.zeros(shape)
+    d = np.flatnonzero(pstate == STATE_DROPLET)
+    x = np.asarray(px, np.float64)[d]
+    r = np.asarray(pdrop_r, np.float64)[d]
+    vol = (4.0 / 3.0) * math.pi * r ** 3
+    g = x / dx_spray - 0.5
+    i0 = np.floor(g).astype(np.int64)
+    t = g - i0
+    cell_vol = dx_spray ** 3
+    for a in range(2):
+        wa = 1.0 - t[:, 0] if a == 0 else t[:, 0]
+        ii = i0[:, 0] + a
+        for b in range(2):
+            wb = 1.0 - t[:, 1] if b == 0 else t[:, 1]
+            jj = i0[:, 1] + b
+            for c in range(2):
+                wc = 1.0 - t[:, 2] if c == 0 else t[:, 2]
+                kk = i0[:, 2] + c
+                ok = ((ii >= 0) & (ii < shape[0]) & (jj >= 0) & (jj < shape[1])
+                      & (kk >= 0) & (kk < shape[2]))
+                np.add.at(dens, (ii[ok], jj[ok], kk[ok]), (wa * wb * wc * vol / cell_vol)[ok])
+    return dens
+
+
+def droplet_stage(px, pv, ptype, pstate, pdrop_r, phi, vel, cfg, dt, seed):
+    """pfflip/sim.py:343-390 in the reference's order: classify, [shear +
+    spray], advance.  -> (px, pv, pstate, pdrop_r, counters[4])."""
+    dx = cfg.dx
+    mu, sigma = lognormal_params(cfg.droplet_radius_mean * dx, cfg.droplet_radius_spread)
+    r_min, r_max = 0.02 * dx, 2.0 * dx
+    st, r, c, _ = classify_escaped(px, ptype, pstate, pdrop_r, phi, dx, cfg.delta_phi, mu, sigma,
+                                   r_min, r_max, seed)
+    n_spray = 0
+    if cfg.spray_shear_threshold > 0.0:
+        shear = velocity_gradient_magnitude(vel, dx)
+        st, r, n_spray = emit_spray_in_shear(px, ptype, st, r, phi, shear, dx,
+                                             cfg.spray_shear_threshold, cfg.spray_emission_rate,
+                                             mu, sigma, r_min, r_max, seed)
+    eps = 1e-4 * dx
+    lo = np.full(3, eps)
+    hi = np.array([s - eps for s in cfg.domain_size])
+    x, v, st, r = advance_droplets(px, pv, st, r, vel, dt, dx, cfg.gravity, cfg.rho_g, cfg.rho_l,
+                                   cfg.droplet_drag_cd, cfg.evaporation_k * dx * dx, r_min, lo, hi,
+                                   cfg.droplet_max_substeps)
+    return x, v, st, r, c + [n_spray]
+
+
+# ---------------------------------------------------------------------------
+# the whole step (pfflip/sim.py:168-424)
 # ---------------------------------------------------------------------------
 class OracleSim:
     """Stage-by-stage restatement of ``Simulator`` (pfflip/sim.py:37-424).
 
-    The droplet stage (sim.py:343-390) is out of this restatement; configs
-    compared against it run with ``droplets=False``.
+    With ``cfg.droplets`` the droplet stage (sim.py:343-390) runs between
+    extrapolation and advection, as in the reference.
     """
 
     def __init__(self, cfg):
@@ -773,6 +1014,8 @@ class OracleSim:
         self.pmass = np.where(self.ptype == 1, cfg.rho_l, cfg.rho_g).astype(np.float32)
         self.prad = np.full(n, cfg.r_p, np.float32)
         self.pstate = np.zeros(n, np.int32)
+        self.pdrop_r = np.zeros(n)
+        self.droplet_stats = {"emitted": 0, "bubbles": 0, "merged": 0, "spray": 0}
 
     def set_solids(self, mask):
         """pfflip/sim.py:113-116."""
@@ -861,8 +1104,18 @@ class OracleSim:
             self.px, self.pv = push_out_of_solids(self.px, self.pv, self.pstate,
                                                   self.solid, cfg.dx)
 
+    def _droplets(self, dt):
+        """pfflip/sim.py:343-390."""
+        if not self.cfg.droplets:
+            return
+        self.px, self.pv, self.pstate, self.pdrop_r, c = droplet_stage(
+            self.px, self.pv, self.ptype, self.pstate, self.pdrop_r, self.phi_cell, self.vel,
+            self.cfg, dt, self.cfg.seed + self.step_index)
+        for k, i in (("emitted", 0), ("bubbles", 1), ("merged", 2), ("spray", 3)):
+            self.droplet_stats[k] += c[i]
+
     def step(self, dt=None):
-        """pfflip/sim.py:168-202 with the droplet stage off."""
+        """pfflip/sim.py:168-202."""
         if dt is None:
             dt = self.dt_from_cfl()
         self._p2g()
@@ -874,10 +1127,12 @@ class OracleSim:
         div_after = self._max_divergence()
         self._g2p()
         self._extrapolate_liquid_velocity()
+        self._droplets(dt)
         self._advect(dt)
         self.time += dt
         self.step_index += 1
         rec = {"step": self.step_index, "t": self.time, "dt": dt, "pcg_iter": it,
-               "pcg_res": rel, "max_div": div_after}
+               "pcg_res": rel, "max_div": div_after,
+               "droplets": int((self.pstate == STATE_DROPLET).sum()) if self.cfg.droplets else 0}
         self.stats.append(rec)
         return rec

@@ -59,6 +59,9 @@ int main(void) {
   printf("pf_mg_level %zu\n", sizeof(pf_mg_level));
   printf("pf_mg %zu\n", sizeof(pf_mg));
   printf("pf_g2p_args %zu\n", sizeof(pf_g2p_args));
+  printf("pf_droplet_args %zu\n", sizeof(pf_droplet_args));
+  P(pf_droplet_args, seed); P(pf_droplet_args, vel); P(pf_droplet_args, gravity);
+  P(pf_droplet_args, lo); P(pf_droplet_args, max_substeps); P(pf_droplet_args, counters);
   P(pf_phase_args, rhs); P(pf_phase_args, gravity); P(pf_phase_args, k_div);
   P(pf_mg, lev); P(pf_mg, rhs); P(pf_mg, scalars);
   P(pf_mg_level, p_h);
@@ -91,6 +94,9 @@ def test_struct_layouts_match_c(tmp_path):
         "pf_mg.rhs": L.MG.rhs.offset, "pf_mg.scalars": L.MG.scalars.offset,
         "pf_mg_level.p_h": L.MGLevel.p_h.offset, "pf_g2p_args.n": L.G2PArgs.n.offset,
         "pf_g2p_args.solid": L.G2PArgs.solid.offset, "pf_g2p_args.hi": L.G2PArgs.hi.offset,
+        "pf_droplet_args": ctypes.sizeof(L.DropletArgs),
+        **{f"pf_droplet_args.{f}": getattr(L.DropletArgs, f).offset
+           for f in ("seed", "vel", "gravity", "lo", "max_substeps", "counters")},
     }
     assert {k: int(v) for k, v in out.items()} == py
 

@@ -49,7 +49,9 @@ def ocfg_of(cfg):
     keys = ["res", "dx", "ppc", "rho_l", "density_ratio", "gravity", "alpha_phi",
             "alpha_flip_liquid", "alpha_flip_air", "cfl", "dt_max", "dt_min", "k_div", "pcg_tol",
             "pcg_max_iter", "mg_pre_sweeps", "mg_post_sweeps", "mg_coarsest_sweeps", "mg_omega",
-            "mg_min_dim", "extrap_cells", "seed"]
+            "mg_min_dim", "extrap_cells", "seed", "droplets", "delta_phi", "droplet_radius_mean",
+            "droplet_radius_spread", "droplet_drag_cd", "evaporation_k", "spray_shear_threshold",
+            "spray_emission_rate", "spray_resolution_factor", "droplet_max_substeps"]
     return O.OracleConfig(**{k: getattr(cfg, k) for k in keys})
 
 

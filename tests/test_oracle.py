@@ -492,3 +492,218 @@ def test_projection_reduces_divergence_on_dam_break():
     sim._pressure_solve(dt)
     sim._project(dt)
     assert sim._max_divergence() < 0.05 * before
+
+
+# --------------------------------------------------------------- droplets
+def test_rng_uniform_and_normal_statistics():
+    """The stream is uniform on [0, 1) in 24-bit steps and its Box-Muller
+    normals have the moments of N(0, 1); neighbouring particles' streams
+    are uncorrelated."""
+    n = 1 << 18
+    st = O.rng_init(7, np.arange(n))
+    st1, u = O.rng_uniform(st)
+    assert u.min() >= 0.0 and u.max() < 1.0
+    assert np.all(u * 16777216.0 == np.floor(u * 16777216.0))
+    assert abs(u.mean() - 0.5) < 4 * math.sqrt(1 / 12 / n)
+    assert abs(u.var() - 1 / 12) < 2e-3
+    hist = np.bincount((u * 64).astype(int), minlength=64)
+    chi2 = float(((hist - n / 64) ** 2 / (n / 64)).sum())
+    assert chi2 < 63 + 6 * math.sqrt(2 * 63)
+    assert abs(np.corrcoef(u[:-1], u[1:])[0, 1]) < 0.01
+    _, u2 = O.rng_uniform(st1)
+    assert abs(np.corrcoef(u, u2)[0, 1]) < 0.01
+    _, z = O.rng_normal(O.rng_init(11, np.arange(n)))
+    assert abs(z.mean()) < 4 / math.sqrt(n)
+    assert abs(z.std() - 1.0) < 0.01
+    assert abs((z ** 4).mean() - 3.0) < 0.1
+    # distinct seeds give distinct streams
+    _, ua = O.rng_uniform(O.rng_init(1, np.arange(1000)))
+    _, ub = O.rng_uniform(O.rng_init(2, np.arange(1000)))
+    assert not np.array_equal(ua, ub)
+
+
+def test_lognormal_radii_have_the_requested_mean():
+    """E[exp(mu + sigma Z)] = exp(mu + sigma^2 / 2) = mean_radius."""
+    mean_r = 0.25 / 64
+    mu, sigma = O.lognormal_params(mean_r, 0.5)
+    n = 1 << 18
+    _, r = O._draw_radius(O.rng_init(3, np.arange(n)), mu, sigma, 0.0, 1.0)
+    assert abs(r.mean() / mean_r - 1.0) < 4 * math.sqrt(math.exp(sigma ** 2) - 1) / math.sqrt(n)
+    assert abs(np.log(r).std() - sigma) < 0.01
+
+
+def test_sample_cell_linear_field_and_clamped_edges():
+    """Trilinear sampling reproduces a linear field at any interior point and
+    holds the edge cell's value within half a cell of the walls."""
+    res, dx = (6, 5, 7), 0.1
+    c = np.stack(np.meshgrid(*[(np.arange(n) + 0.5) * dx for n in res], indexing="ij"), -1)
+    f = 1.0 + 2.0 * c[..., 0] - 3.0 * c[..., 1] + 0.5 * c[..., 2]
+    rng = np.random.default_rng(0)
+    x = 0.5 * dx + rng.random((500, 3)) * (np.array(res) - 1) * dx
+    got = O.sample_cell(f, x, dx)
+    assert np.abs(got - (1.0 + 2.0 * x[:, 0] - 3.0 * x[:, 1] + 0.5 * x[:, 2])).max() < 1e-12
+    corner = np.array([[0.2 * dx, 0.3 * dx, 0.1 * dx]])
+    assert O.sample_cell(f, corner, dx)[0] == f[0, 0, 0]
+
+
+def _lone_particle_phi(ppc, ptype_lone, ptype_bulk):
+    """Eq. (7) phi at a particle placed on a cell centre of a perfect lattice
+    of another phase, through the oracle's P2G, phase field and sampling."""
+    cfg = O.OracleConfig(res=(8, 8, 8), dx=1 / 8, ppc=ppc)
+    rt0, rho_0l, rho_min = O.derived_densities(cfg)
+    pos = O.lattice(cfg.res, cfg.dx, ppc, np.random.default_rng(0), jitter=0.0)
+    centre = np.array([[4.5, 4.5, 4.5]]) * cfg.dx
+    pos = np.concatenate([pos, centre])
+    pt = np.full(len(pos), ptype_bulk, np.int32)
+    pt[-1] = ptype_lone
+    mass = np.where(pt == 1, cfg.rho_l, cfg.rho_g).astype(np.float32)
+    cm, _ = O.p2g_cell(pos, mass, np.full(len(pos), cfg.dx, np.float32), np.zeros(len(pos), np.int32),
+                       cfg.res, cfg.dx)
+    phi, _ = O.cell_phase(cm, rho_min, cfg.alpha_phi, rho_0l)
+    return cfg, pos, pt, phi, rt0, rho_0l, rho_min
+
+
+@pytest.mark.parametrize("ppc,converts", [(8, False), (27, True)])
+def test_lone_liquid_particle_converts_only_above_17_ppc(ppc, converts):
+    """Reference tests/test_pfflip.py:140-167 and the eq. (7) closed form: a
+    liquid particle alone in air sees phi = sqrt((rho_l + rho~_0 rho_g -
+    rho_min) / rho_0l) at its own cell centre, which is below 1/2 - delta_phi
+    at 27 ppc (0.228) and above it at 8 ppc (0.440)."""
+    cfg, pos, pt, phi, rt0, rho_0l, rho_min = _lone_particle_phi(ppc, 1, 0)
+    expect = math.sqrt((cfg.rho_l + rt0 * cfg.rho_g - rho_min) / rho_0l)
+    # 1e-5: the 27 ppc lattice sites (thirds of a cell) are rounded to FP32
+    assert abs(phi[4, 4, 4] - expect) < 1e-5
+    mu, sigma = O.lognormal_params(0.25 * cfg.dx)
+    st, r, c, f = O.classify_escaped(pos, pt, np.zeros(len(pos), np.int32), np.zeros(len(pos)), phi,
+                                     cfg.dx, 0.2, mu, sigma, 0.02 * cfg.dx, 2 * cfg.dx, 42)
+    assert abs(f[-1] - expect) < 1e-5
+    assert (st[-1] == O.STATE_DROPLET) == converts
+    assert c[0] == int(converts) and c[1] == 0 and (st[:-1] == 0).all()
+    if converts:
+        assert 0.02 * cfg.dx <= r[-1] <= 2 * cfg.dx
+
+
+def test_air_in_liquid_is_deleted_and_droplet_merges_back():
+    """An air particle inside the liquid body sees phi =
+    sqrt((rho~_0 rho_l + rho_g - rho_min) / rho_0l) = 0.9966 (> 0.7): a
+    bubble, deleted.  A droplet there merges back into FLIP."""
+    cfg, pos, pt, phi, rt0, rho_0l, rho_min = _lone_particle_phi(8, 0, 1)
+    expect = math.sqrt((rt0 * cfg.rho_l + cfg.rho_g - rho_min) / rho_0l)
+    st0 = np.zeros(len(pos), np.int32)
+    st0[0] = O.STATE_DROPLET   # a liquid droplet at a lattice site in the bulk
+    st0[1] = O.STATE_DEAD
+    st, _, c, f = O.classify_escaped(pos, pt, st0, np.zeros(len(pos)), phi, cfg.dx, 0.2, -5.0, 0.5,
+                                     0.0, 1.0, 42)
+    assert abs(f[-1] - expect) < 1e-12 and st[-1] == O.STATE_DEAD
+    assert st[0] == O.STATE_FLIP and st[1] == O.STATE_DEAD
+    assert c == [0, 1, 1]
+
+
+def test_velocity_gradient_of_a_linear_field():
+    """u_a = sum_b G_ab x_b: interior cells see sqrt of the sum of G_ab^2 over
+    the seven terms the reference keeps (all but dv/dz and dw/dy)."""
+    res, dx = (6, 7, 5), 0.1
+    G = np.array([[0.3, -1.2, 0.7], [2.0, -0.4, 1.5], [-0.9, 0.8, 0.1]])
+    vel = []
+    for a, shape in enumerate(O.face_shapes(res)):
+        o = [0.5, 0.5, 0.5]
+        o[a] = 0.0
+        x = np.stack(np.meshgrid(*[(np.arange(n) + o[b]) * dx for b, n in enumerate(shape)],
+                                 indexing="ij"), -1)
+        vel.append(x @ G[a])
+    got = O.velocity_gradient_magnitude(vel, dx)
+    keep = np.ones((3, 3), bool)
+    keep[1, 2] = keep[2, 1] = False
+    expect = math.sqrt(float((G[keep] ** 2).sum()))
+    assert np.abs(got[1:-1, 1:-1, 1:-1] - expect).max() < 1e-12
+    # at a wall cell the clamped centred difference halves the cross term
+    assert got[0, 3, 2] != pytest.approx(expect)
+
+
+def _one_droplet(x, v, r=0.004):
+    return (np.array([x], np.float64), np.array([v], np.float64), np.array([1], np.int32),
+            np.array([r]))
+
+
+def test_ballistic_droplet_matches_the_discrete_closed_form():
+    """No drag: N semi-implicit Euler sub-steps of h = dt/N give v = v0 + N h g
+    and x = x0 + N h v0 + h^2 g N (N+1)/2, with N = floor(|v0| dt / (dx/2)) + 1."""
+    res, dx, dt = (16, 16, 16), 1 / 16, 0.02
+    vel = [np.zeros(s) for s in O.face_shapes(res)]
+    g = np.array([0.0, 0.0, -9.81])
+    for v0, n_expect in (((0.5, 0.0, 0.0), 1), ((2.0, 0.0, 1.0), 2), ((3.5, 3.5, 0.0), 4)):
+        x0 = np.array([0.5, 0.5, 0.5])
+        v0 = np.array(v0)
+        x, v, st, r = O.advance_droplets(*_one_droplet(x0, v0), vel, dt, dx, g, 1.0, 1000.0, 0.0, 0.0,
+                                         0.0, np.full(3, 1e-4 * dx), np.full(3, 1 - 1e-4 * dx), 8)
+        n = n_expect
+        h = dt / n
+        assert np.allclose(v[0], v0 + n * h * g, atol=1e-14)
+        assert np.allclose(x[0], x0 + n * h * v0 + h * h * g * n * (n + 1) / 2, atol=1e-14)
+        assert st[0] == O.STATE_DROPLET
+
+
+def test_droplet_at_terminal_velocity_stays_there():
+    """Gravity balances sphere drag, 1/2 rho_g Cd pi r^2 v^2 = rho_l 4/3 pi r^3 g,
+    at v_t = sqrt(8 rho_l r g / (3 rho_g Cd)): a droplet falling through still
+    air at v_t keeps its velocity."""
+    res, dx, dt = (16, 16, 16), 1 / 16, 1e-3
+    vel = [np.zeros(s) for s in O.face_shapes(res)]
+    r, rho_l, rho_g, cd, g = 0.004, 1000.0, 1.0, 0.5, 9.81
+    vt = math.sqrt(8 * rho_l * r * g / (3 * rho_g * cd))
+    x, v, st, _ = O.advance_droplets(*_one_droplet([0.5, 0.5, 0.9], [0.0, 0.0, -vt], r), vel, dt, dx,
+                                     (0.0, 0.0, -g), rho_g, rho_l, cd, 0.0, 0.0,
+                                     np.full(3, 1e-6), np.full(3, 1 - 1e-6), 8)
+    assert abs(v[0, 2] + vt) < 1e-9 * vt and st[0] == O.STATE_DROPLET
+    # and half the terminal speed accelerates downwards
+    _, v2, _, _ = O.advance_droplets(*_one_droplet([0.5, 0.5, 0.9], [0.0, 0.0, -0.5 * vt], r), vel, dt,
+                                     dx, (0.0, 0.0, -g), rho_g, rho_l, cd, 0.0, 0.0,
+                                     np.full(3, 1e-6), np.full(3, 1 - 1e-6), 8)
+    assert v2[0, 2] < -0.5 * vt
+
+
+def test_evaporation_follows_r_squared_law_and_kills_small_droplets():
+    """dr/dt = -k/r integrates to r^2 = r0^2 - 2 k t; midpoint RK2 over a
+    step stays within O(h^3) of it.  A droplet that shrinks below r_min dies
+    and keeps its pre-advance state."""
+    res, dx = (16, 16, 16), 1 / 16
+    vel = [np.zeros(s) for s in O.face_shapes(res)]
+    r0, k, dt = 0.004, 1e-7, 0.01
+    x, v, st, r = O.advance_droplets(*_one_droplet([0.5, 0.5, 0.5], [0.0, 0.0, 0.0], r0), vel, dt, dx,
+                                     (0.0, 0.0, 0.0), 1.0, 1000.0, 0.5, k, 1e-4, np.full(3, 1e-6),
+                                     np.full(3, 1 - 1e-6), 8)
+    exact = math.sqrt(r0 * r0 - 2 * k * dt)
+    assert abs(r[0] - exact) < 1e-3 * (r0 - exact) and r[0] < r0
+    x0 = np.array([[0.5, 0.5, 0.5]])
+    x, v, st, r = O.advance_droplets(x0.copy(), np.array([[1.0, 0.0, 0.0]]), np.array([1], np.int32),
+                                     np.array([0.001]), vel, dt, dx, (0.0, 0.0, 0.0), 1.0, 1000.0, 0.5,
+                                     1e-4, 5e-4, np.full(3, 1e-6), np.full(3, 1 - 1e-6), 8)
+    assert st[0] == O.STATE_DEAD and np.array_equal(x, x0) and r[0] == 0.001
+
+
+def test_droplet_leaving_the_domain_dies():
+    res, dx, dt = (16, 16, 16), 1 / 16, 0.02
+    vel = [np.zeros(s) for s in O.face_shapes(res)]
+    x0 = np.array([[0.99, 0.5, 0.5]])
+    x, v, st, _ = O.advance_droplets(x0.copy(), np.array([[2.0, 0.0, 0.0]]), np.array([1], np.int32),
+                                     np.array([0.004]), vel, dt, dx, (0.0, 0.0, 0.0), 1.0, 1000.0, 0.5,
+                                     0.0, 0.0, np.full(3, 1e-4 * dx), np.full(3, 1 - 1e-4 * dx), 8)
+    assert st[0] == O.STATE_DEAD and np.array_equal(x, x0)
+
+
+def test_rasterize_spray_conserves_volume():
+    """Trilinear weights sum to one: sum(density) * cell volume equals the
+    droplets' total volume when no weight falls outside the field; a droplet
+    on a cell centre puts all of it in that cell."""
+    shape, dxs = (12, 12, 12), 0.05
+    rng = np.random.default_rng(1)
+    x = (1.0 + rng.random((200, 3)) * 10.0) * dxs
+    r = rng.random(200) * 0.01
+    st = np.ones(200, np.int32)
+    st[::7] = 0
+    dens = O.rasterize_spray(x, st, r, shape, dxs)
+    vol = (4 / 3) * math.pi * (r[st == 1] ** 3).sum()
+    assert abs(dens.sum() * dxs ** 3 - vol) < 1e-12 * vol
+    one = O.rasterize_spray(np.array([[3.5, 4.5, 5.5]]) * dxs, np.array([1]), np.array([0.01]), shape, dxs)
+    assert one[3, 4, 5] == pytest.approx((4 / 3) * math.pi * 1e-6 / dxs ** 3, rel=1e-12)
+    assert np.count_nonzero(one) == 1
