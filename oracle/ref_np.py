@@ -31,6 +31,17 @@ Pinning (what checks each function independently of this file) lives in
 * ``extrapolate_sweep``     hand-derived single-cell cases
 * whole step                mass / liquid-count conservation, phi == 0 in air,
                             hydrostatic stability (reference tests)
+* droplet stage             ``sample_cell`` linear fields + clamped edges;
+                            ``classify_escaped`` eq. (7) closed form of a lone
+                            particle (converts at 27 ppc, not at 8) and of an
+                            air particle in liquid; ``advance_droplets`` discrete
+                            ballistic closed form, terminal velocity, r^2 law of
+                            evaporation, death on exit; ``velocity_gradient_
+                            magnitude`` linear fields; ``rasterize_spray``
+                            volume conservation; the random stream by its
+                            uniform / normal / log-normal statistics.  Whether
+                            the stream reproduces Warp's numbers is parity
+                            unpinned (warp-lang is absent).
 """
 
 from __future__ import annotations
