@@ -25,6 +25,10 @@ constexpr int RT = 256;     // threads of element-wise / reduction kernels
 struct CGState {
     double rz, alpha, beta, mu, b_norm, rel, qAq, mu_x, tol;
     int it, done, pad0, pad1;
+    // 16-bit V-cycles: the level-0 right-hand side is read times vscale (a
+    // power of two putting max|r| in [1/2, 1)) and z is read back times
+    // vscale_inv, so the FP16 iterates stay in range as r shrinks
+    float vscale, vscale_inv;
     unsigned ticket[8];
     double part[6 * MAXB];
     float fpart[MAXB];
@@ -195,10 +199,12 @@ template <typename ST, bool ZINIT>
 __global__ __launch_bounds__(RT) void k_rbgs(Dims d, const ST* __restrict__ bx,
                                              const ST* __restrict__ by, const ST* __restrict__ bz,
                                              ST* p, const float* __restrict__ b, int colour,
-                                             float omega, const int* __restrict__ done) {
+                                             float omega, const int* __restrict__ done,
+                                             const float* __restrict__ bscale) {
     // the flag is only consulted at the stores, so its load does not delay
     // the stencil loads (a converged solve's extra iteration stores nothing)
     const bool skip = done && *done;
+    const float bs = bscale ? *bscale : 1.0f;
     const int nkp = (d.nz + 1) >> 1;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)d.nx * d.ny * nkp) return;
@@ -215,7 +221,7 @@ __global__ __launch_bounds__(RT) void k_rbgs(Dims d, const ST* __restrict__ bx,
                 const float dg = to_f32(bx[face_index<0>(d, i, j, k)]) + to_f32(bx[face_index<0>(d, i + 1, j, k)]) +
                                  to_f32(by[face_index<1>(d, i, j, k)]) + to_f32(by[face_index<1>(d, i, j + 1, k)]) +
                                  to_f32(bz[face_index<2>(d, i, j, k)]) + to_f32(bz[face_index<2>(d, i, j, k + 1)]);
-                if (dg > 0.0f) v = rb_first(b[cell_index(d, i, j, k)], dg, omega);
+                if (dg > 0.0f) v = rb_first(b[cell_index(d, i, j, k)] * bs, dg, omega);
             }
             if (!skip) p[cell_index(d, i, j, k)] = from_f32<ST>(v);
         }
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(RT) void k_rbgs(Dims d, const ST* __restrict__ bx,
         stencil(d, bx, by, bz, p, i, j, k, dg, Ap, pc);
         if (dg <= 0.0f) return;
         const int c = cell_index(d, i, j, k);
-        if (!skip) p[c] = from_f32<ST>(rb_relax(pc, b[c], Ap, dg, omega));
+        if (!skip) p[c] = from_f32<ST>(rb_relax(pc, b[c] * bs, Ap, dg, omega));
     }
 }
 
@@ -299,7 +305,8 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
                                                    const ST* __restrict__ by, const ST* __restrict__ bz,
                                                    const ST* __restrict__ pin, ST* __restrict__ pout,
                                                    const float* __restrict__ b, int c1, float omega,
-                                                   const int* __restrict__ done) {
+                                                   const int* __restrict__ done,
+                                                   const float* __restrict__ bscale) {
     using T = RbTile<TI, TJ, TK, NTH>;
     __shared__ float P[T::NP];
     const int tid = threadIdx.x;
@@ -410,6 +417,16 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
         }
     }
     const bool skip = done && *done;
+    if (bscale) {
+        const float bs = *bscale;
+#pragma unroll
+        for (int q = 0; q < T::PPT; ++q) {
+            bb[q][0] *= bs;
+            bb[q][1] *= bs;
+        }
+#pragma unroll
+        for (int r = 0; r < T::SHT; ++r) sb[r] *= bs;
+    }
 
     if (ZINIT) {
         for (int e = tid; e < T::NP; e += T::NT) P[e] = 0.0f;
@@ -505,8 +522,10 @@ __global__ __launch_bounds__(RT) void k_resid_restrict(Dims d, Dims dc, const ST
                                                        const ST* __restrict__ p,
                                                        const float* __restrict__ b,
                                                        float* __restrict__ bc,
-                                                       const int* __restrict__ done) {
+                                                       const int* __restrict__ done,
+                                                       const float* __restrict__ bscale) {
     const bool skip = done && *done;
+    const float bs = bscale ? *bscale : 1.0f;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)dc.nx * dc.ny * dc.nz) return;
     const int K = (int)(t % dc.nz), J = (int)((t / dc.nz) % dc.ny), I = (int)(t / ((int64_t)dc.nz * dc.ny));
@@ -520,7 +539,7 @@ __global__ __launch_bounds__(RT) void k_resid_restrict(Dims d, Dims dc, const ST
                 const int i = 2 * I + a, j = 2 * J + bb, k = 2 * K + c;
                 float dg, Ap, pc;
                 stencil(d, bx, by, bz, p, i, j, k, dg, Ap, pc);
-                acc += b[cell_index(d, i, j, k)] - Ap;
+                acc += b[cell_index(d, i, j, k)] * bs - Ap;
             }
     if (!skip) bc[t] = acc;
 }
@@ -850,9 +869,15 @@ struct CGK {
     const float* q_old;  // the previous one (the two alternate between iterations)
     float* Ap;
     const ZT* z;
+    const float* zscale;  // NULL: z as stored; else z times *zscale
     CGState* st;
     double tol;
 };
+
+template <typename ZT>
+__device__ inline float zval(const CGK<ZT>& a, float zs, int64_t i) {
+    return to_f32(a.z[i]) * zs;
+}
 
 // r = b - mean(b); x = 0; q = 0; partial max|r| (poisson.py:363-372)
 template <typename ZT>
@@ -876,9 +901,10 @@ template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_first(CGK<ZT> a) {
     __shared__ double sh[3 * 16];
     if (a.st->done) return;
+    const float zs = a.zscale ? *a.zscale : 1.0f;
     double v[3] = {0.0, 0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
-        const double zz = (double)to_f32(a.z[i]), rr = (double)a.r[i];
+        const double zz = (double)zval(a, zs, i), rr = (double)a.r[i];
         v[0] += zz;
         v[1] += rr;
         v[2] += rr * zz;
@@ -898,10 +924,11 @@ __global__ __launch_bounds__(RT) void k_cg_apply(CGK<ZT> a) {
     if (a.st->done) return;
     const float cm = (float)(-a.st->mu);
     const float be = (float)a.st->beta;
+    const float zs = a.zscale ? *a.zscale : 1.0f;
     double v[1] = {0.0};
     const Dims d = a.d;
     auto qn = [&](int c) {
-        const float zz = to_f32(a.z[c]) + cm;
+        const float zz = zval(a, zs, c) + cm;
         return zz + be * a.q_old[c];
     };
     for (int64_t c = (int64_t)blockIdx.x * RT + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * RT) {
@@ -945,9 +972,10 @@ template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_beta(CGK<ZT> a) {
     __shared__ double sh[5 * 16];
     if (a.st->done) return;
+    const float zs = a.zscale ? *a.zscale : 1.0f;
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
-        const double zz = (double)to_f32(a.z[i]), rr = (double)a.r[i], ap = (double)a.Ap[i];
+        const double zz = (double)zval(a, zs, i), rr = (double)a.r[i], ap = (double)a.Ap[i];
         v[0] += zz;
         v[1] += rr;
         v[2] += ap;
@@ -961,6 +989,17 @@ __global__ __launch_bounds__(RT) void k_cg_beta(CGK<ZT> a) {
 
 // One-block combine of the previous kernel's partials, then the scalar step.
 enum CombOp { C_MEAN = 0, C_MEAN_RESET, C_INIT, C_FIRST, C_APPLY, C_UPDATE, C_BETA };
+
+// vscale = 2^-e with m = f 2^e, f in [1/2, 1): m * vscale in [1/2, 1)
+__device__ inline void set_vscale(CGState* st, float m) {
+    int e = 0;
+    if (m > 0.0f && m < 3.0e38f) {
+        (void)frexpf(m, &e);
+        e = max(min(e, 120), -120);
+    }
+    st->vscale = ldexpf(1.0f, -e);
+    st->vscale_inv = ldexpf(1.0f, e);
+}
 
 __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, int64_t n,
                                              double* out, double tol) {
@@ -985,6 +1024,7 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
         if (threadIdx.x == 0) {
             st->b_norm = (double)mm;
             st->tol = tol;
+            set_vscale(st, mm);
             if (mm == 0.0f) {
                 st->done = 2;
                 st->rel = 0.0;
@@ -1022,6 +1062,7 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
             const double rel = (double)mm / st->b_norm;
             st->rel = rel;
             if (rel < st->tol) st->done = 1;
+            set_vscale(st, mm);
         }
     } else if (op == C_BETA) {
         // Polak-Ribiere with the flexible numerator (poisson.py:400-406):
@@ -1126,17 +1167,17 @@ static int ensure_small_lds() {
 template <typename ST, bool ZINIT>
 static int launch_full_sweep(const Dims& d, const ST* fx, const ST* fy, const ST* fz, const ST* pin,
                              ST* pout, const float* b, int c1, float omega, const int* done,
-                             hipStream_t s) {
+                             const float* bscale, hipStream_t s) {
     const int64_t n = (int64_t)d.nx * d.ny * d.nz;
     auto tiles = [&](int ti, int tj, int tk) {
         return (unsigned)(cdiv(d.nx, ti) * cdiv(d.ny, tj) * cdiv(d.nz, tk));
     };
     if (n >= (int64_t)1 << 20)
-        k_rbgs_full<ST, ZINIT, 8, 8, 64, 512><<<tiles(8, 8, 64), 512, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done);
+        k_rbgs_full<ST, ZINIT, 8, 8, 64, 512><<<tiles(8, 8, 64), 512, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale);
     else if (n >= (int64_t)1 << 17)
-        k_rbgs_full<ST, ZINIT, 8, 8, 16><<<tiles(8, 8, 16), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done);
+        k_rbgs_full<ST, ZINIT, 8, 8, 16><<<tiles(8, 8, 16), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale);
     else
-        k_rbgs_full<ST, ZINIT, 4, 8, 16><<<tiles(4, 8, 16), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done);
+        k_rbgs_full<ST, ZINIT, 4, 8, 16><<<tiles(4, 8, 16), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }
@@ -1155,7 +1196,8 @@ static bool fused_smoother(const pf_mg& mg, int l) {
 
 // One V-cycle: z0 = M^{-1} r0 (no mean removal at level 0).
 template <typename ST>
-static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hipStream_t s) {
+static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, const float* bscale,
+                  hipStream_t s) {
     const int L = mg.nlev;
     const int sm = mg.small_from;             // first LDS level (L if none)
     const int G = sm < L ? sm : L - 1;        // levels 0..G-1 run out of HBM with restriction
@@ -1171,6 +1213,7 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
         const Dims d = ldims(lv);
         const ST *fx = Lv<ST>::fx(lv), *fy = Lv<ST>::fy(lv), *fz = Lv<ST>::fz(lv);
         const float* b = B(l);
+        const float* bsc = l == 0 ? bscale : nullptr;
         if (from_zero) cur[l] = P(l);
         if (from_zero && sweeps == 0) {
             k_zero<ST><<<red_blocks(ncell(lv)), RT, 0, s>>>(cur[l], ncell(lv), done);
@@ -1181,8 +1224,10 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
                 ST* out = other(l, cur[l]);
                 const int c1 = reverse ? 1 : 0;
                 const int rc = (from_zero && sw == 0)
-                                   ? launch_full_sweep<ST, true>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done, s)
-                                   : launch_full_sweep<ST, false>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done, s);
+                                   ? launch_full_sweep<ST, true>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done,
+                                                                 bsc, s)
+                                   : launch_full_sweep<ST, false>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done,
+                                                                  bsc, s);
                 if (rc) return rc;
                 cur[l] = out;
             }
@@ -1194,9 +1239,9 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
             for (int h = 0; h < 2; ++h) {
                 const int colour = reverse ? 1 - h : h;
                 if (from_zero && sw == 0 && h == 0)
-                    k_rbgs<ST, true><<<grid, RT, 0, s>>>(d, fx, fy, fz, cur[l], b, colour, mg.omega, done);
+                    k_rbgs<ST, true><<<grid, RT, 0, s>>>(d, fx, fy, fz, cur[l], b, colour, mg.omega, done, bsc);
                 else
-                    k_rbgs<ST, false><<<grid, RT, 0, s>>>(d, fx, fy, fz, cur[l], b, colour, mg.omega, done);
+                    k_rbgs<ST, false><<<grid, RT, 0, s>>>(d, fx, fy, fz, cur[l], b, colour, mg.omega, done, bsc);
                 PF_CHECK_LAUNCH();
             }
         }
@@ -1210,7 +1255,7 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, hip
         const pf_mg_level& cv = mg.lev[l + 1];
         k_resid_restrict<ST><<<cdiv(ncell(cv), RT), RT, 0, s>>>(ldims(lv), ldims(cv), Lv<ST>::fx(lv),
                                                               Lv<ST>::fy(lv), Lv<ST>::fz(lv), cur[l], B(l),
-                                                              mg.lev[l + 1].b, done);
+                                                              mg.lev[l + 1].b, done, l == 0 ? bscale : nullptr);
         PF_CHECK_LAUNCH();
     }
     if (sm < L) {
@@ -1302,6 +1347,12 @@ static int get_poll(HostPoll*& out) {
     return PF_OK;
 }
 
+// The V-cycle's right-hand-side scale of a solve that reads z scaled.
+template <typename ST>
+static const float* vcycle_scale(const CGK<ST>& a) {
+    return a.zscale ? &a.st->vscale : nullptr;
+}
+
 // Kernels of one FPCG iteration; `flag` (pinned host memory) receives the
 // done flag after the residual test.  Every kernel returns at once when the
 // flag is set, so an iteration enqueued past convergence does nothing.
@@ -1318,7 +1369,7 @@ static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int
     k_comb<<<1, RT, 0, s>>>(st, C_UPDATE, (int)nb, a.n, nullptr, 0.0);
     PF_CHECK_LAUNCH();
     PF_CHECK_HIP(hipMemcpyAsync(flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
-    int rc = vcycle<ST>(mg, a.r, z, done, s);
+    int rc = vcycle<ST>(mg, a.r, z, done, vcycle_scale(a), s);
     if (rc) return rc;
     k_cg_beta<ST><<<nb, RT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
@@ -1332,7 +1383,7 @@ static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int
 template <typename ST>
 static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, unsigned nb,
                             hipStream_t s) {
-    int rc = vcycle<ST>(mg, a.r, z, done, s);
+    int rc = vcycle<ST>(mg, a.r, z, done, vcycle_scale(a), s);
     if (rc) return rc;
     k_cg_first<ST><<<nb, RT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
@@ -1458,6 +1509,9 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
     a.q_old = mg.rhs;
     a.Ap = mg.Ap;
     a.z = z;
+    // 16-bit iterates: scale the V-cycle's input (not for a whole hierarchy
+    // in LDS, which computes in FP32 and rounds only its output)
+    a.zscale = (sizeof(ST) == 2 && mg.small_from > 0) ? &st->vscale_inv : nullptr;
     a.st = st;
     a.tol = tol;
     const unsigned nb = red_blocks(n);
@@ -1532,7 +1586,7 @@ static int precondition_impl(const pf_mg& mg, const float* r, float* z, hipStrea
     CGState* st = static_cast<CGState*>(mg.scalars);
     const int64_t n = ncell(mg.lev[0]);
     ST* zt = (sizeof(ST) == 4) ? reinterpret_cast<ST*>(z) : Lv<ST>::p(mg.lev[0]);
-    int rc = vcycle<ST>(mg, r, zt, nullptr, s);
+    int rc = vcycle<ST>(mg, r, zt, nullptr, nullptr, s);
     if (rc) return rc;
     k_part_sum<ST><<<red_blocks(n), RT, 0, s>>>(zt, n, st->part);
     PF_CHECK_LAUNCH();
@@ -1678,12 +1732,12 @@ static int time_smoother_impl(const pf_mg& mg, int level, int reps, float* ms_ou
             if (fused) {
                 const int rc = launch_full_sweep<ST, false>(d, Lv<ST>::fx(lv), Lv<ST>::fy(lv), Lv<ST>::fz(lv),
                                                             r % 2 ? pb : pa, r % 2 ? pa : pb, b, 0, mg.omega,
-                                                            nullptr, s);
+                                                            nullptr, nullptr, s);
                 if (rc) return rc;
             } else {
                 for (int c = 0; c < 2; ++c) {
                     k_rbgs<ST, false><<<grid, RT, 0, s>>>(d, Lv<ST>::fx(lv), Lv<ST>::fy(lv), Lv<ST>::fz(lv), pa,
-                                                          b, c, mg.omega, nullptr);
+                                                          b, c, mg.omega, nullptr, nullptr);
                     PF_CHECK_LAUNCH();
                 }
             }

@@ -700,6 +700,79 @@ __device__ inline void rk3_rest(const float* __restrict__ f0, const float* __res
     z = z + dt * (c1 * k1z + c2 * k2z + c3 * k3z);
 }
 
+// Wall reflection (kernels.py:567-595), solid push-out (kernels.py:630-679)
+// and the stores, shared by the per-particle and the tiled kernels.
+__device__ inline void finish_particle(const G2PArgs& a, int64_t p, float x, float y, float z, float vx, float vy,
+                                       float vz) {
+    const Dims d = a.d;
+    if (a.flags & PF_GA_REFLECT) {
+        // kernels.py:567-595
+        float xs[3] = {x, y, z}, vs[3] = {vx, vy, vz};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (xs[ax] <= a.lo[ax]) {
+                xs[ax] = a.lo[ax];
+                vs[ax] = fmaxf(vs[ax], 0.f);
+            } else if (xs[ax] >= a.hi[ax]) {
+                xs[ax] = a.hi[ax];
+                vs[ax] = fminf(vs[ax], 0.f);
+            }
+        }
+        x = xs[0];
+        y = xs[1];
+        z = xs[2];
+        vx = vs[0];
+        vy = vs[1];
+        vz = vs[2];
+    }
+    if ((a.flags & PF_GA_SOLIDS) && a.solid) {
+        // kernels.py:630-679
+        const int i = min(max((int)(x / a.dx), 0), d.nx - 1);
+        const int j = min(max((int)(y / a.dx), 0), d.ny - 1);
+        const int k = min(max((int)(z / a.dx), 0), d.nz - 1);
+        if (a.solid[cell_index(d, i, j, k)] != 0) {
+            float best = 1.0e9f;
+            int bax = 0, bsg = 1;
+            for (int dd = 0; dd < 6; ++dd) {
+                const int ax = dd / 2, sg = (dd % 2 == 0) ? -1 : 1;
+                for (int t = 1; t < 8; ++t) {
+                    const int ii = i + (ax == 0 ? sg * t : 0);
+                    const int jj = j + (ax == 1 ? sg * t : 0);
+                    const int kk = k + (ax == 2 ? sg * t : 0);
+                    if (ii < 0 || jj < 0 || kk < 0 || ii >= d.nx || jj >= d.ny || kk >= d.nz) break;
+                    if (a.solid[cell_index(d, ii, jj, kk)] == 0) {
+                        const float dist = (float)t;
+                        if (dist < best) {
+                            best = dist;
+                            bax = ax;
+                            bsg = sg;
+                        }
+                        break;
+                    }
+                }
+            }
+            if (best <= 1.0e8f) {
+                const float off = (best + 0.5f) * a.dx * (float)bsg;
+                float xs[3] = {x, y, z}, vs[3] = {vx, vy, vz};
+                xs[bax] = xs[bax] + off;
+                if ((float)bsg * vs[bax] < 0.f) vs[bax] = 0.f;
+                x = xs[0];
+                y = xs[1];
+                z = xs[2];
+                vx = vs[0];
+                vy = vs[1];
+                vz = vs[2];
+            }
+        }
+    }
+    a.px[3 * p] = x;
+    a.px[3 * p + 1] = y;
+    a.px[3 * p + 2] = z;
+    a.pv[3 * p] = vx;
+    a.pv[3 * p + 1] = vy;
+    a.pv[3 * p + 2] = vz;
+}
+
 // At most 5 waves per SIMD: the kernel is bound by its 60 row-pair gathers
 // per particle, and more resident waves evict each other's rows from L1/L2
 // (measured at 128^3 x 27 ppc: 7 waves/SIMD is 10 % slower than 5).
@@ -768,72 +841,237 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) voi
         y = fminf(fmaxf(y, a.lo[1]), a.hi[1]);
         z = fminf(fmaxf(z, a.lo[2]), a.hi[2]);
     }
-    if (a.flags & PF_GA_REFLECT) {
-        // kernels.py:567-595
-        float xs[3] = {x, y, z}, vs[3] = {vx, vy, vz};
+    finish_particle(a, p, x, y, z, vx, vy, vz);
+}
+
+// ============================================================== tiled G2P + advection
+// One workgroup per tile of T^3 cells.  The face grids the tile's particles
+// sample (vel, dvel, vel_liq) are staged in LDS over the tile plus a 2-cell
+// halo on every side (GB = T + 4 entries per axis), indices clamped into each
+// grid while staging: an LDS tap then holds exactly what the reference's
+// clamped tap reads, so every sample is the reference's sum in the
+// reference's order with no per-tap clamps.  The staged box covers every
+// sample within one cell of the tile (the first RK3 stage always, the later
+// stages at CFL <= 1); a sample outside it reads global memory.
+constexpr int GB = T + 4;
+constexpr int GB2 = GB * GB;
+constexpr int GB3 = GB * GB * GB;
+constexpr int kG2PThreads = 512;
+
+struct BoxRows {
+    int base;    // LDS offset of tap (0,0,0)
+    float w[8];  // wi * wj * wk, index di*4 + dj*2 + dk
+};
+
+template <int A>
+__device__ inline bool box_rows(const Coord& cx, const Coord& cy, const Coord& cz, int ox, int oy, int oz,
+                                BoxRows& r) {
+    const int li = (A == 0 ? cx.in : cx.it) - ox;
+    const int lj = (A == 1 ? cy.in : cy.it) - oy;
+    const int lk = (A == 2 ? cz.in : cz.it) - oz;
+    const float fx = A == 0 ? cx.fn : cx.ft, fy = A == 1 ? cy.fn : cy.ft, fz = A == 2 ? cz.fn : cz.ft;
+    r.base = (li * GB + lj) * GB + lk;
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            if (xs[ax] <= a.lo[ax]) {
-                xs[ax] = a.lo[ax];
-                vs[ax] = fmaxf(vs[ax], 0.f);
-            } else if (xs[ax] >= a.hi[ax]) {
-                xs[ax] = a.hi[ax];
-                vs[ax] = fminf(vs[ax], 0.f);
-            }
+    for (int di = 0; di < 2; ++di) {
+        const float wi = di == 0 ? 1.0f - fx : fx;
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const float wj = dj == 0 ? 1.0f - fy : fy;
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) r.w[di * 4 + dj * 2 + dk] = wi * wj * (dk == 0 ? 1.0f - fz : fz);
         }
-        x = xs[0];
-        y = xs[1];
-        z = xs[2];
-        vx = vs[0];
-        vy = vs[1];
-        vz = vs[2];
     }
-    if ((a.flags & PF_GA_SOLIDS) && a.solid) {
-        // kernels.py:630-679
-        const int i = min(max((int)(x / a.dx), 0), d.nx - 1);
-        const int j = min(max((int)(y / a.dx), 0), d.ny - 1);
-        const int k = min(max((int)(z / a.dx), 0), d.nz - 1);
-        if (a.solid[cell_index(d, i, j, k)] != 0) {
-            float best = 1.0e9f;
-            int bax = 0, bsg = 1;
-            for (int dd = 0; dd < 6; ++dd) {
-                const int ax = dd / 2, sg = (dd % 2 == 0) ? -1 : 1;
-                for (int t = 1; t < 8; ++t) {
-                    const int ii = i + (ax == 0 ? sg * t : 0);
-                    const int jj = j + (ax == 1 ? sg * t : 0);
-                    const int kk = k + (ax == 2 ? sg * t : 0);
-                    if (ii < 0 || jj < 0 || kk < 0 || ii >= d.nx || jj >= d.ny || kk >= d.nz) break;
-                    if (a.solid[cell_index(d, ii, jj, kk)] == 0) {
-                        const float dist = (float)t;
-                        if (dist < best) {
-                            best = dist;
-                            bax = ax;
-                            bsg = sg;
-                        }
-                        break;
+    return (unsigned)li <= (unsigned)(GB - 2) && (unsigned)lj <= (unsigned)(GB - 2) &&
+           (unsigned)lk <= (unsigned)(GB - 2);
+}
+
+__device__ inline float sample_box(const BoxRows& r, const float* L) {
+    const float* q = L + r.base;
+    float acc = 0.f;
+    acc += r.w[0] * q[0];
+    acc += r.w[1] * q[1];
+    acc += r.w[2] * q[GB];
+    acc += r.w[3] * q[GB + 1];
+    acc += r.w[4] * q[GB2];
+    acc += r.w[5] * q[GB2 + 1];
+    acc += r.w[6] * q[GB2 + GB];
+    acc += r.w[7] * q[GB2 + GB + 1];
+    return acc;
+}
+
+struct BoxRows3 {
+    BoxRows r0, r1, r2;
+};
+
+__device__ inline bool box_rows3(float x, float y, float z, float inv_dx, int ox, int oy, int oz, BoxRows3& b) {
+    const Coord cx = coord(x * inv_dx), cy = coord(y * inv_dx), cz = coord(z * inv_dx);
+    const bool i0 = box_rows<0>(cx, cy, cz, ox, oy, oz, b.r0);
+    const bool i1 = box_rows<1>(cx, cy, cz, ox, oy, oz, b.r1);
+    const bool i2 = box_rows<2>(cx, cy, cz, ox, oy, oz, b.r2);
+    return i0 && i1 && i2;
+}
+
+// Velocity sample at (x, y, z) from the staged field `fb` (LDS, 3 axes of
+// GB3 entries each) or, outside the box, from the global fields `fg`.
+__device__ inline void sample3(const float* fb, const float* const* fg, float x, float y, float z,
+                               const G2PArgs& a, int ox, int oy, int oz, float& u, float& v, float& w) {
+    BoxRows3 b;
+    if (box_rows3(x, y, z, a.inv_dx, ox, oy, oz, b)) {
+        u = sample_box(b.r0, fb);
+        v = sample_box(b.r1, fb + GB3);
+        w = sample_box(b.r2, fb + 2 * GB3);
+    } else {
+        Rows3 r;
+        make_rows3(x, y, z, a.inv_dx, a.d, r);
+        u = sample_rows(r.t0, fg[0]);
+        v = sample_rows(r.t1, fg[1]);
+        w = sample_rows(r.t2, fg[2]);
+    }
+}
+
+template <int AX>
+__device__ inline void stage_face(float* __restrict__ dst, const float* __restrict__ src, const Dims& d, int ox, int oy,
+                                  int oz) {
+    const int n0 = d.nx + (AX == 0), n1 = d.ny + (AX == 1), n2 = d.nz + (AX == 2);
+    for (int e = threadIdx.x; e < GB3; e += kG2PThreads) {
+        const int li = e / GB2, lj = (e / GB) % GB, lk = e % GB;
+        const int gi = min(max(ox + li, 0), n0 - 1);
+        const int gj = min(max(oy + lj, 0), n1 - 1);
+        const int gk = min(max(oz + lk, 0), n2 - 1);
+        dst[e] = src[((int64_t)gi * n1 + gj) * n2 + gk];
+    }
+}
+
+__device__ inline void stage_field(float* dst, const float* const* src, const Dims& d, int ox, int oy, int oz) {
+    stage_face<0>(dst, src[0], d, ox, oy, oz);
+    stage_face<1>(dst + GB3, src[1], d, ox, oy, oz);
+    stage_face<2>(dst + 2 * GB3, src[2], d, ox, oy, oz);
+}
+
+__global__ __launch_bounds__(kG2PThreads) void k_g2p_tiled(G2PArgs a, const int* __restrict__ perm,
+                                                           const int* __restrict__ start, int ntx, int nty,
+                                                           int ntz) {
+    // fields: [0, 3 GB3) vel, [3 GB3, 6 GB3) dvel, [6 GB3, 9 GB3) vel_liq
+    __shared__ float box[9 * GB3];
+    const int nt = ntx * nty * ntz;
+    const int t = xcd_block(blockIdx.x, nt);
+    const int tz = t % ntz, ty = (t / ntz) % nty, tx = t / (ntz * nty);
+    const int ox = tx * T - 2, oy = ty * T - 2, oz = tz * T - 2;
+    const int s0 = start[(int64_t)t * T3], s1 = start[(int64_t)(t + 1) * T3];
+    if (s0 == s1) return;
+    const bool g2p = a.flags & PF_GA_G2P, adv = a.flags & PF_GA_ADVECT;
+    stage_field(box, a.vel, a.d, ox, oy, oz);
+    if (g2p) stage_field(box + 3 * GB3, a.dvel, a.d, ox, oy, oz);
+    if (adv) stage_field(box + 6 * GB3, a.vel_liq, a.d, ox, oy, oz);
+    __syncthreads();
+
+    for (int s = s0 + (int)threadIdx.x; s < s1; s += kG2PThreads) {
+        const int64_t p = perm[s];
+        const bool liquid = a.pflag ? (a.pflag[p] & 1) : (a.ptype[p] == 1);
+        float x = a.px[3 * p], y = a.px[3 * p + 1], z = a.px[3 * p + 2];
+        float vx = a.pv[3 * p], vy = a.pv[3 * p + 1], vz = a.pv[3 * p + 2];
+        const float* fk = liquid ? box + 6 * GB3 : box;                 // RK3 field in LDS
+        const float* const* gk = liquid ? a.vel_liq : a.vel;            // and in HBM
+        float k1x = 0.f, k1y = 0.f, k1z = 0.f;
+        {
+            BoxRows3 b;
+            if (box_rows3(x, y, z, a.inv_dx, ox, oy, oz, b)) {
+                float pic[3] = {0.f, 0.f, 0.f};
+                if (g2p || !liquid) {
+                    pic[0] = sample_box(b.r0, box);
+                    pic[1] = sample_box(b.r1, box + GB3);
+                    pic[2] = sample_box(b.r2, box + 2 * GB3);
+                }
+                if (adv) {
+                    if (liquid) {
+                        k1x = sample_box(b.r0, box + 6 * GB3);
+                        k1y = sample_box(b.r1, box + 7 * GB3);
+                        k1z = sample_box(b.r2, box + 8 * GB3);
+                    } else {
+                        k1x = pic[0];
+                        k1y = pic[1];
+                        k1z = pic[2];
                     }
                 }
-            }
-            if (best <= 1.0e8f) {
-                const float off = (best + 0.5f) * a.dx * (float)bsg;
-                float xs[3] = {x, y, z}, vs[3] = {vx, vy, vz};
-                xs[bax] = xs[bax] + off;
-                if ((float)bsg * vs[bax] < 0.f) vs[bax] = 0.f;
-                x = xs[0];
-                y = xs[1];
-                z = xs[2];
-                vx = vs[0];
-                vy = vs[1];
-                vz = vs[2];
+                if (g2p) {
+                    const float dux = sample_box(b.r0, box + 3 * GB3), duy = sample_box(b.r1, box + 4 * GB3),
+                                duz = sample_box(b.r2, box + 5 * GB3);
+                    const float al = liquid ? a.alpha_l : a.alpha_a;
+                    vx = al * (vx + dux) + (1.0f - al) * pic[0];
+                    vy = al * (vy + duy) + (1.0f - al) * pic[1];
+                    vz = al * (vz + duz) + (1.0f - al) * pic[2];
+                }
+            } else {
+                // a particle outside its tile's box (a position the binning
+                // clamped into the grid): the per-particle path's samples
+                Rows3 r;
+                make_rows3(x, y, z, a.inv_dx, a.d, r);
+                float pic[3] = {0.f, 0.f, 0.f};
+                if (g2p || !liquid) {
+                    pic[0] = sample_rows(r.t0, a.vel[0]);
+                    pic[1] = sample_rows(r.t1, a.vel[1]);
+                    pic[2] = sample_rows(r.t2, a.vel[2]);
+                }
+                if (adv) {
+                    if (liquid) {
+                        k1x = sample_rows(r.t0, a.vel_liq[0]);
+                        k1y = sample_rows(r.t1, a.vel_liq[1]);
+                        k1z = sample_rows(r.t2, a.vel_liq[2]);
+                    } else {
+                        k1x = pic[0];
+                        k1y = pic[1];
+                        k1z = pic[2];
+                    }
+                }
+                if (g2p) {
+                    const float dux = sample_rows(r.t0, a.dvel[0]), duy = sample_rows(r.t1, a.dvel[1]),
+                                duz = sample_rows(r.t2, a.dvel[2]);
+                    const float al = liquid ? a.alpha_l : a.alpha_a;
+                    vx = al * (vx + dux) + (1.0f - al) * pic[0];
+                    vy = al * (vy + duy) + (1.0f - al) * pic[1];
+                    vz = al * (vz + duz) + (1.0f - al) * pic[2];
+                }
             }
         }
+        if (adv) {
+            // Ralston RK3 (kernels.py:508-549)
+            const float dt = a.dt, h1 = 0.5f * dt, h2 = 0.75f * dt;
+            float k2x, k2y, k2z, k3x, k3y, k3z;
+            sample3(fk, gk, x + h1 * k1x, y + h1 * k1y, z + h1 * k1z, a, ox, oy, oz, k2x, k2y, k2z);
+            sample3(fk, gk, x + h2 * k2x, y + h2 * k2y, z + h2 * k2z, a, ox, oy, oz, k3x, k3y, k3z);
+            const float c1 = 2.0f / 9.0f, c2 = 3.0f / 9.0f, c3 = 4.0f / 9.0f;
+            x = x + dt * (c1 * k1x + c2 * k2x + c3 * k3x);
+            y = y + dt * (c1 * k1y + c2 * k2y + c3 * k3y);
+            z = z + dt * (c1 * k1z + c2 * k2z + c3 * k3z);
+            x = fminf(fmaxf(x, a.lo[0]), a.hi[0]);
+            y = fminf(fmaxf(y, a.lo[1]), a.hi[1]);
+            z = fminf(fmaxf(z, a.lo[2]), a.hi[2]);
+        }
+        finish_particle(a, p, x, y, z, vx, vy, vz);
     }
-    a.px[3 * p] = x;
-    a.px[3 * p + 1] = y;
-    a.px[3 * p + 2] = z;
-    a.pv[3 * p] = vx;
-    a.pv[3 * p + 1] = vy;
-    a.pv[3 * p + 2] = vz;
+}
+
+// Particles outside the bins (pstate != 0 and not dead, i.e. droplets) after
+// the tiled pass: wall reflection and solids only, as in the per-particle
+// kernel.  Exits at once when the bins held every particle.
+__global__ __launch_bounds__(256) void k_g2p_unbinned(G2PArgs a, const int* __restrict__ n_binned) {
+    if (*n_binned == a.n) return;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        bool active, dead;
+        if (a.pflag) {
+            const uint8_t fl = a.pflag[p];
+            active = fl & 2;
+            dead = fl & 4;
+        } else {
+            const int st = a.pstate[p];
+            active = st == 0;
+            dead = st == 2;
+        }
+        if (active || dead) continue;
+        finish_particle(a, p, a.px[3 * p], a.px[3 * p + 1], a.px[3 * p + 2], a.pv[3 * p], a.pv[3 * p + 1],
+                        a.pv[3 * p + 2]);
+    }
 }
 
 // ============================================================== CFL speed + splat bounds
@@ -1039,6 +1277,61 @@ PF_API int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream) {
     a.alpha_a = in.alpha_air;
     k_g2p_advect<<<cdiv(in.n, 256), 256, 0, S(stream)>>>(a);
     PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_g2p_advect_binned(const pf_g2p_args* args, const int32_t* perm, const int32_t* cell_start,
+                                const int32_t* n_active, pf_stream_t stream) {
+    PF_REQUIRE(args, "pf_g2p_advect_binned: null args");
+    const pf_g2p_args& in = *args;
+    PF_REQUIRE(perm && cell_start && n_active, "pf_g2p_advect_binned: null bins");
+    PF_REQUIRE(in.nx > 0 && in.ny > 0 && in.nz > 0 && in.n >= 0, "pf_g2p_advect_binned: bad sizes");
+    PF_REQUIRE(in.nz >= 2, "pf_g2p_advect_binned: nz >= 2 is required (row-pair samples)");
+    PF_REQUIRE(in.n < (int64_t)1 << 31, "pf_g2p_advect_binned: more than 2^31 particles");
+    PF_REQUIRE((int64_t)in.nx * in.ny * in.nz < (int64_t)1 << 31, "pf_g2p_advect_binned: grid too large");
+    PF_REQUIRE(in.dx > 0.f, "pf_g2p_advect_binned: dx must be positive");
+    if (in.n == 0) return PF_OK;
+    PF_REQUIRE(in.px && in.pv, "pf_g2p_advect_binned: null particles");
+    PF_REQUIRE(in.pflag || (in.ptype && in.pstate), "pf_g2p_advect_binned: need pflag or ptype+pstate");
+    PF_REQUIRE(in.flags & (PF_GA_G2P | PF_GA_ADVECT), "pf_g2p_advect_binned: nothing to do without G2P/ADVECT");
+    for (int q = 0; q < 3; ++q) {
+        PF_REQUIRE(in.vel[q], "pf_g2p_advect_binned: null vel");
+        PF_REQUIRE(!(in.flags & PF_GA_G2P) || in.dvel[q], "pf_g2p_advect_binned: G2P needs dvel");
+        PF_REQUIRE(!(in.flags & PF_GA_ADVECT) || in.vel_liq[q], "pf_g2p_advect_binned: advection needs vel_liq");
+    }
+    PF_REQUIRE(!(in.flags & PF_GA_SOLIDS) || in.solid, "pf_g2p_advect_binned: solids flag without mask");
+    G2PArgs a;
+    a.px = in.px;
+    a.pv = in.pv;
+    a.ptype = in.ptype;
+    a.pstate = in.pstate;
+    a.pflag = in.pflag;
+    a.n = in.n;
+    a.d = Dims{in.nx, in.ny, in.nz};
+    a.flags = in.flags;
+    for (int q = 0; q < 3; ++q) {
+        a.vel[q] = in.vel[q];
+        a.dvel[q] = in.dvel[q];
+        a.vel_liq[q] = in.vel_liq[q];
+        a.lo[q] = in.lo[q];
+        a.hi[q] = in.hi[q];
+    }
+    a.solid = in.solid;
+    a.dx = in.dx;
+    a.inv_dx = 1.0f / in.dx;
+    a.dt = in.dt;
+    a.alpha_l = in.alpha_liquid;
+    a.alpha_a = in.alpha_air;
+    const TileGeom g = tile_geom(in.nx, in.ny, in.nz);
+    const int64_t nt = n_tiles(g);
+    k_g2p_tiled<<<(unsigned)nt, kG2PThreads, 0, S(stream)>>>(a, perm, cell_start, g.ntx, g.nty, g.ntz);
+    PF_CHECK_LAUNCH();
+    // particles outside the bins (droplets): reflection and solids only
+    if (in.flags & (PF_GA_REFLECT | PF_GA_SOLIDS)) {
+        const int64_t nb = cdiv(in.n, 256);
+        k_g2p_unbinned<<<(unsigned)(nb < 2048 ? nb : 2048), 256, 0, S(stream)>>>(a, n_active);
+        PF_CHECK_LAUNCH();
+    }
     return PF_OK;
 }
 

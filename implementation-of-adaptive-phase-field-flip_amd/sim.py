@@ -334,16 +334,19 @@ class Simulator:
         self._mark("project")
         self._extrapolate_liquid_velocity()
         self._mark("extrapolate")
+        # the particles have not moved since _bin: G2P samples LDS tiles
+        # through the same bins
+        bins = (_lib.ptr(self.perm), _lib.ptr(self.cell_start), _lib.ptr(self._n_active))
         if cfg.droplets:
             # G2P, then the droplet stage, then advection (sim.py:180-185)
             a = self._g2p_args(_lib.GA_G2P, dt)
-            _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
+            _lib.call("pf_g2p_advect_binned", ctypes.byref(a), *bins, _lib.stream())
             self._launch_droplets(dt)
             a = self._g2p_args(_lib.GA_ADVECT | _lib.GA_REFLECT, dt)
             _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
         else:
             a = self._g2p_args(_lib.GA_G2P | _lib.GA_ADVECT | _lib.GA_REFLECT, dt)
-            _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
+            _lib.call("pf_g2p_advect_binned", ctypes.byref(a), *bins, _lib.stream())
         self._mark("g2p_advect")
         div_after, counters = self._read_scalar_block()   # synchronises
         n_drop = 0

@@ -347,6 +347,35 @@ def test_mgpcg_solves(pf, contrast, precision):
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-3
 
 
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_mixed_solve_is_scale_equivariant(pf, precision):
+    """The 16-bit V-cycle reads its right-hand side times a power of two set
+    from max|r| each iteration, so a right-hand side 2^-24 times smaller
+    gives the same iterations and 2^-24 times the solution, bit for bit (FP16
+    alone would flush such residuals to zero), and converges to 1e-6 within
+    a few iterations of the FP32 solve."""
+    say(f"scale-equivariant {precision} 32^3")
+    res = (32, 32, 32)
+    rng = np.random.default_rng(11)
+    bx, by, bz = random_beta(res, rng, 1000.0)
+    b = rng.standard_normal(res).astype(np.float32)
+    b -= b.mean()
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
+    out = {}
+    for prec, scale in ((precision, 1.0), (precision, 2.0 ** -24), ("fp32", 1.0)):
+        solver = pf.MGPCG(res, "cuda:0", pre=3, post=3, precision=prec)
+        solver.build_hierarchy(g(bx), g(by), g(bz))
+        p = torch.zeros(res, device="cuda")
+        it, rel = solver.solve(g(b * np.float32(scale)), p, tol=1e-6, max_iter=200)
+        out[(prec, scale)] = (it, rel, np_(p).copy())
+    it1, rel1, p1 = out[(precision, 1.0)]
+    it2, rel2, p2 = out[(precision, 2.0 ** -24)]
+    it32 = out[("fp32", 1.0)][0]
+    assert rel1 < 1e-6 and it1 == it2 and rel1 == rel2
+    assert np.array_equal(p2, (p1 * np.float32(2.0 ** -24)).astype(np.float32))
+    assert it1 <= it32 + 4, (it1, it32)
+
+
 def test_mgpcg_matches_oracle_iterations(pf):
     """Same iteration count and solution as the FP64 restatement on a
     well-conditioned system (HBM levels active at 32^3)."""
@@ -498,6 +527,48 @@ def test_g2p_advect_match_oracle(pf):
     px_ref, pv_ref = O.reflect_at_walls(px_ref, pv_ref, st, lo, hi)
     assert np.abs(np_(sim.px) - px_ref).max() < 2e-6
     assert rel_err(np_(sim.pv), pv_ref) < 2e-6
+
+
+@pytest.mark.parametrize("res,speed,solids", [((16, 16, 16), 0.5, False), ((20, 12, 18), 0.5, True),
+                                              ((16, 16, 16), 40.0, False)])
+def test_g2p_binned_is_bitwise_per_particle(pf, res, speed, solids):
+    """The LDS-tiled G2P + RK3 equals the per-particle kernel bit for bit,
+    with droplets and dead particles outside the bins, partial tiles, solids,
+    and (speed 40) RK3 stages that leave the staged box and read HBM."""
+    say(f"binned g2p {res} speed={speed} solids={solids}")
+    lib = __import__("importlib").import_module("implementation-of-adaptive-phase-field-flip_amd._lib")
+    cfg = pf.Config(res=res, dx=1.0 / 16, ppc=8, droplets=False)
+    sim = pf.Simulator(cfg)
+    pos, vel, pt = pf.scenes.dam_break(cfg)
+    sim.set_particles(pos, vel, pt)
+    rng = np.random.default_rng(8)
+    if solids:
+        m = np.zeros(res, bool)
+        m[8:11, 4:7, :] = True
+        sim.set_solids(m)
+    for lst in (sim.vel, sim.dvel, sim.vel_liq):
+        for t in lst:
+            t.copy_(torch.from_numpy((rng.standard_normal(tuple(t.shape)) * speed).astype(np.float32)))
+    st = np.zeros(sim.n_particles, np.int32)
+    st[::97] = 1
+    st[::101] = 2
+    sim.pstate.copy_(torch.from_numpy(st))
+    sim.pv.copy_(torch.from_numpy(rng.standard_normal((sim.n_particles, 3)).astype(np.float32)))
+    x0, v0 = sim.px.clone(), sim.pv.clone()
+    sim._bin()
+    dt = 0.02
+    out = []
+    for name in ("pf_g2p_advect", "pf_g2p_advect_binned"):
+        sim.px.copy_(x0)
+        sim.pv.copy_(v0)
+        a = sim._g2p_args(lib.GA_G2P | lib.GA_ADVECT | lib.GA_REFLECT, dt)
+        extra = ((lib.ptr(sim.perm), lib.ptr(sim.cell_start), lib.ptr(sim._n_active))
+                 if name.endswith("binned") else ())
+        lib.call(name, __import__("ctypes").byref(a), *extra, lib.stream())
+        torch.cuda.synchronize()
+        out.append((np_(sim.px).copy(), np_(sim.pv).copy()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert not np.array_equal(out[0][0], np_(x0))
 
 
 # ------------------------------------------------------------------ whole step
