@@ -22,6 +22,9 @@ int fill_u32(void* dst, int64_t n, uint32_t v, hipStream_t s);
 constexpr int MAXB = 1024;  // blocks of a reduction launch
 constexpr int RT = 256;     // threads of element-wise / reduction kernels
 
+// Largest coarsest level whose smoother is applied as a precomputed matrix.
+constexpr int kCoarseOpMax = 512;
+
 struct CGState {
     double rz, alpha, beta, mu, b_norm, rel, qAq, mu_x, tol;
     int it, done, pad0, pad1;
@@ -32,6 +35,10 @@ struct CGState {
     unsigned ticket[8];
     double part[6 * MAXB];
     float fpart[MAXB];
+    // the coarsest level's `coarsest` red-black sweeps from zero followed by
+    // mean removal, a linear map of its right-hand side: column j (the image
+    // of the unit vector e_j) at [j * n, (j + 1) * n), built by pf_mg_build
+    float coarse_op[kCoarseOpMax * kCoarseOpMax];
 };
 
 static unsigned red_blocks(int64_t n) {
@@ -604,6 +611,7 @@ struct SmallArgs {
     int seg_len[3 * PF_MAX_LEVELS + 1];
     void* p_top;
     const int* done;
+    const float* cop;  // non-NULL: the coarsest level's smoother as a matrix
 };
 
 constexpr int kSmallThreads = 1024;
@@ -815,7 +823,17 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
         __syncthreads();
     }
     // coarsest: sweeps, then remove the mean (poisson.py:325-328)
-    {
+    if (a.cop) {
+        // the same map, precomputed: p = M b (column j of M contiguous)
+        const LdsLevel v = lds_level(a, lds, a.L - 1);
+        const int n = v.d.nx * v.d.ny * v.d.nz;
+        for (int t = tid; t < n; t += nt) {
+            float acc = 0.0f;
+            for (int j = 0; j < n; ++j) acc += a.cop[(int64_t)j * n + t] * v.b[j];
+            v.p[t] = acc;
+        }
+        __syncthreads();
+    } else {
         const LdsLevel v = lds_level(a, lds, a.L - 1);
         const int n = v.d.nx * v.d.ny * v.d.nz;
         for (int e = tid; e < n; e += nt) v.p[e] = 0.0f;
@@ -851,6 +869,71 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
         OT* out = static_cast<OT*>(a.p_top);
         for (int e = tid; e < n; e += nt) out[e] = from_f32<OT>(v.p[e]);
     }
+}
+
+// Column j of the coarsest level's smoother map: `sweeps` red-black sweeps
+// (colours (0,1)) from p = 0 with b = e_j, then mean removal, as the
+// single-workgroup V-cycle runs them.  One workgroup per column, one thread
+// per cell (n <= kCoarseOpMax).
+__global__ __launch_bounds__(kCoarseOpMax) void k_coarse_op(Dims d, const float* __restrict__ bx,
+                                                            const float* __restrict__ by,
+                                                            const float* __restrict__ bz, int sweeps, float omega,
+                                                            float* __restrict__ op) {
+    __shared__ float P[kCoarseOpMax];
+    __shared__ double sh[16];
+    __shared__ float shift;
+    const int n = d.nx * d.ny * d.nz, j = blockIdx.x, t = threadIdx.x;
+    const bool own = t < n;
+    float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int nb[6] = {0, 0, 0, 0, 0, 0};
+    int colour = 0;
+    float dg = 0.f;
+    if (own) {
+        const int k = t % d.nz, jj = (t / d.nz) % d.ny, i = t / (d.nz * d.ny);
+        f[0] = bx[face_index<0>(d, i, jj, k)];
+        f[1] = bx[face_index<0>(d, i + 1, jj, k)];
+        f[2] = by[face_index<1>(d, i, jj, k)];
+        f[3] = by[face_index<1>(d, i, jj + 1, k)];
+        f[4] = bz[face_index<2>(d, i, jj, k)];
+        f[5] = bz[face_index<2>(d, i, jj, k + 1)];
+        nb[0] = cell_index(d, max(i - 1, 0), jj, k);
+        nb[1] = cell_index(d, min(i + 1, d.nx - 1), jj, k);
+        nb[2] = cell_index(d, i, max(jj - 1, 0), k);
+        nb[3] = cell_index(d, i, min(jj + 1, d.ny - 1), k);
+        nb[4] = cell_index(d, i, jj, max(k - 1, 0));
+        nb[5] = cell_index(d, i, jj, min(k + 1, d.nz - 1));
+        colour = (i + jj + k) & 1;
+        dg = f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
+        P[t] = 0.0f;
+    }
+    const float b = t == j ? 1.0f : 0.0f;
+    __syncthreads();
+    for (int sw = 0; sw < sweeps; ++sw) {
+        for (int c = 0; c < 2; ++c) {
+            const bool upd = own && colour == c && dg > 0.0f;
+            float v = 0.0f;
+            if (upd) {
+                const float pc = P[t];
+                const float pn[6] = {P[nb[0]], P[nb[1]], P[nb[2]], P[nb[3]], P[nb[4]], P[nb[5]]};
+                float dd, Ap;
+                stencil_vals(pc, f, pn, dd, Ap);
+                v = rb_relax(pc, b, Ap, dg, omega);
+            }
+            __syncthreads();
+            if (upd) P[t] = v;
+            __syncthreads();
+        }
+    }
+    double sm[1] = {own ? (double)P[t] : 0.0};
+    block_sum<1>(sm, sh);
+    if (t == 0) shift = (float)(-(sm[0] / (double)n));
+    __syncthreads();
+    if (own) op[(int64_t)j * n + t] = P[t] + shift;
+}
+
+static bool use_coarse_op(const pf_mg& mg) {
+    const pf_mg_level& c = mg.lev[mg.nlev - 1];
+    return mg.small_from < mg.nlev && (int64_t)c.nx * c.ny * c.nz <= kCoarseOpMax;
 }
 
 // ================================================================ FPCG kernels
@@ -1284,6 +1367,7 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
         }
         seg(B(sm), a.off[sm][4], ncell(mg.lev[sm]));
         a.done = done;
+        a.cop = use_coarse_op(mg) ? static_cast<CGState*>(mg.scalars)->coarse_op : nullptr;
         const size_t lds = (size_t)nf * sizeof(float);
         if (sm == 0) {
             a.p_top = z0;
@@ -1683,6 +1767,14 @@ PF_API int pf_mg_build(const pf_mg* mg, const float* bx, const float* by, const 
             k_diag_copy<bf16s><<<cdiv(n, RT), RT, 0, s>>>(
                 d, v.bx, v.by, v.bz, v.diag, copy ? reinterpret_cast<bf16s*>(v.bx_h) : nullptr,
                 reinterpret_cast<bf16s*>(v.by_h), reinterpret_cast<bf16s*>(v.bz_h));
+        PF_CHECK_LAUNCH();
+    }
+    if (use_coarse_op(*mg)) {
+        const pf_mg_level& c = mg->lev[mg->nlev - 1];
+        const int n = c.nx * c.ny * c.nz;
+        const int threads = ((n + 63) / 64) * 64;
+        k_coarse_op<<<n, threads, 0, s>>>(ldims(c), c.bx, c.by, c.bz, mg->coarsest, mg->omega,
+                                          static_cast<CGState*>(mg->scalars)->coarse_op);
         PF_CHECK_LAUNCH();
     }
     return PF_OK;

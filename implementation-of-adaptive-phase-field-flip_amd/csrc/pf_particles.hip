@@ -965,11 +965,29 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_tiled(G2PArgs a, const int*
     if (adv) stage_field(box + 6 * GB3, a.vel_liq, a.d, ox, oy, oz);
     __syncthreads();
 
-    for (int s = s0 + (int)threadIdx.x; s < s1; s += kG2PThreads) {
-        const int64_t p = perm[s];
-        const bool liquid = a.pflag ? (a.pflag[p] & 1) : (a.ptype[p] == 1);
-        float x = a.px[3 * p], y = a.px[3 * p + 1], z = a.px[3 * p + 2];
-        float vx = a.pv[3 * p], vy = a.pv[3 * p + 1], vz = a.pv[3 * p + 2];
+    // the next particle's record is loaded before the current one is
+    // processed, so its latency hides behind that work
+    int s = s0 + (int)threadIdx.x;
+    int64_t pn = 0;
+    bool liq_n = false;
+    float xn = 0.f, yn = 0.f, zn = 0.f, vxn = 0.f, vyn = 0.f, vzn = 0.f;
+    if (s < s1) {
+        pn = perm[s];
+        liq_n = a.pflag ? (a.pflag[pn] & 1) : (a.ptype[pn] == 1);
+        xn = a.px[3 * pn], yn = a.px[3 * pn + 1], zn = a.px[3 * pn + 2];
+        vxn = a.pv[3 * pn], vyn = a.pv[3 * pn + 1], vzn = a.pv[3 * pn + 2];
+    }
+    for (; s < s1; s += kG2PThreads) {
+        const int64_t p = pn;
+        const bool liquid = liq_n;
+        float x = xn, y = yn, z = zn;
+        float vx = vxn, vy = vyn, vz = vzn;
+        if (s + kG2PThreads < s1) {
+            pn = perm[s + kG2PThreads];
+            liq_n = a.pflag ? (a.pflag[pn] & 1) : (a.ptype[pn] == 1);
+            xn = a.px[3 * pn], yn = a.px[3 * pn + 1], zn = a.px[3 * pn + 2];
+            vxn = a.pv[3 * pn], vyn = a.pv[3 * pn + 1], vzn = a.pv[3 * pn + 2];
+        }
         const float* fk = liquid ? box + 6 * GB3 : box;                 // RK3 field in LDS
         const float* const* gk = liquid ? a.vel_liq : a.vel;            // and in HBM
         float k1x = 0.f, k1y = 0.f, k1z = 0.f;
