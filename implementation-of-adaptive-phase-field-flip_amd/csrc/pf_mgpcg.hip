@@ -307,13 +307,18 @@ struct RbTile {
 
 // (4 waves per SIMD: a 128^3 level's 512 workgroups of 8x8x64 tiles are all
 // resident at once, so the sweep costs one memory round trip, not two)
-template <typename ST, bool ZINIT, int TI, int TJ, int TK, int NTH = 256>
+// PROL: the sweep's input is pin plus the piecewise-constant prolongation of
+// the coarse iterate pc (poisson.py:178-182), formed in LDS while staging
+// (rounded to ST as the separate prolongation stores it): the first
+// post-sweep of a level without a separate prolongation pass.
+template <typename ST, bool ZINIT, int TI, int TJ, int TK, int NTH = 256, bool PROL = false, typename CT = float>
 __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restrict__ bx,
                                                    const ST* __restrict__ by, const ST* __restrict__ bz,
                                                    const ST* __restrict__ pin, ST* __restrict__ pout,
                                                    const float* __restrict__ b, int c1, float omega,
                                                    const int* __restrict__ done,
-                                                   const float* __restrict__ bscale) {
+                                                   const float* __restrict__ bscale,
+                                                   const CT* __restrict__ pcoarse, Dims dc) {
     using T = RbTile<TI, TJ, TK, NTH>;
     __shared__ float P[T::NP];
     const int tid = threadIdx.x;
@@ -356,6 +361,26 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
                         (const void*)(pin + cell_index(d, gi, gj, kc)),
                         (__attribute__((address_space(3))) void*)(R + e0 + (tid & ~63)), 4, 0, 0);
                 }
+            }
+        }
+    }
+
+    // (1b) PROL: the coarse iterate under the staged region, parents
+    // [i0/2 - 1, i0/2 + TI/2 + 1) per axis (coordinates clamped)
+    constexpr int CI = TI / 2 + 2, CJ = TJ / 2 + 2, CK = TK / 2 + 2, CN = CI * CJ * CK;
+    constexpr int CPT = PROL ? (CN + T::NT - 1) / T::NT : 1;
+    float cv[CPT];
+    if constexpr (PROL) {
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) {
+            const int e = tid + q * T::NT;
+            cv[q] = 0.0f;
+            if (e < CN) {
+                const int ck = e % CK, cj = (e / CK) % CJ, ci = e / (CK * CJ);
+                const int gi = min(max(i0 / 2 - 1 + ci, 0), dc.nx - 1);
+                const int gj = min(max(j0 / 2 - 1 + cj, 0), dc.ny - 1);
+                const int gk = min(max(k0 / 2 - 1 + ck, 0), dc.nz - 1);
+                cv[q] = to_f32(pcoarse[cell_index(dc, gi, gj, gk)]);
             }
         }
     }
@@ -456,6 +481,22 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
             }
             __syncthreads();
         }
+    }
+    if constexpr (PROL) {
+        __shared__ float C[PROL ? CN : 1];
+#pragma unroll
+        for (int q = 0; q < CPT; ++q)
+            if (tid + q * T::NT < CN) C[tid + q * T::NT] = cv[q];
+        __syncthreads();
+        for (int e = tid; e < T::NP; e += T::NT) {
+            const int kk = e % T::PK, jj = (e / T::PK) % T::PJ, ii = e / SI;
+            const int gi = min(max(i0 - 2 + ii, 0), d.nx - 1);
+            const int gj = min(max(j0 - 2 + jj, 0), d.ny - 1);
+            const int gk = min(max(k0 - 2 + kk, 0), d.nz - 1);
+            const int ci = (gi >> 1) - (i0 / 2 - 1), cj = (gj >> 1) - (j0 / 2 - 1), ck = (gk >> 1) - (k0 / 2 - 1);
+            P[e] = round_st<ST>(P[e] + C[(ci * CJ + cj) * CK + ck]);
+        }
+        __syncthreads();
     }
 
     // first colour on the tile
@@ -1247,20 +1288,26 @@ static int ensure_small_lds() {
 
 // Fused full sweep on one level: tile shape by level size (big tiles cut
 // the redundant halo work, small ones keep enough workgroups in flight).
-template <typename ST, bool ZINIT>
+// PROL (CT = the coarse iterate's type): the input is pin plus the
+// prolongation of pcoarse (dims dc); the fine dims must be twice dc.
+template <typename ST, bool ZINIT, bool PROL = false, typename CT = float>
 static int launch_full_sweep(const Dims& d, const ST* fx, const ST* fy, const ST* fz, const ST* pin,
                              ST* pout, const float* b, int c1, float omega, const int* done,
-                             const float* bscale, hipStream_t s) {
+                             const float* bscale, hipStream_t s, const CT* pcoarse = nullptr,
+                             Dims dc = Dims{1, 1, 1}) {
     const int64_t n = (int64_t)d.nx * d.ny * d.nz;
     auto tiles = [&](int ti, int tj, int tk) {
         return (unsigned)(cdiv(d.nx, ti) * cdiv(d.ny, tj) * cdiv(d.nz, tk));
     };
     if (n >= (int64_t)1 << 20)
-        k_rbgs_full<ST, ZINIT, 8, 8, 64, 512><<<tiles(8, 8, 64), 512, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale);
+        k_rbgs_full<ST, ZINIT, 8, 8, 64, 512, PROL, CT><<<tiles(8, 8, 64), 512, 0, s>>>(
+            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc);
     else if (n >= (int64_t)1 << 17)
-        k_rbgs_full<ST, ZINIT, 8, 8, 16><<<tiles(8, 8, 16), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale);
+        k_rbgs_full<ST, ZINIT, 8, 8, 16, 256, PROL, CT><<<tiles(8, 8, 16), 256, 0, s>>>(
+            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc);
     else
-        k_rbgs_full<ST, ZINIT, 4, 8, 16><<<tiles(4, 8, 16), 256, 0, s>>>(d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale);
+        k_rbgs_full<ST, ZINIT, 4, 8, 16, 256, PROL, CT><<<tiles(4, 8, 16), 256, 0, s>>>(
+            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }
@@ -1395,6 +1442,27 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
     for (int l = G - 1; l >= 0; --l) {
         const pf_mg_level& lv = mg.lev[l];
         const pf_mg_level& cv = mg.lev[l + 1];
+        // the first post-sweep forms the prolongation itself when the last
+        // post-sweep then lands where it must (level 0: in z0)
+        if (fused_smoother(mg, l) && mg.post > 0 &&
+            (l > 0 || ((mg.post % 2) ? other(0, cur[0]) : cur[0]) == z0)) {
+            const Dims d = ldims(lv);
+            ST* out = other(l, cur[l]);
+            const float* bsc = l == 0 ? bscale : nullptr;
+            int rc = (l + 1 == sm)
+                         ? launch_full_sweep<ST, false, true, float>(d, Lv<ST>::fx(lv), Lv<ST>::fy(lv),
+                                                                           Lv<ST>::fz(lv), cur[l], out, B(l), 1,
+                                                                           mg.omega, done, bsc, s, cv.p, ldims(cv))
+                               : launch_full_sweep<ST, false, true, ST>(d, Lv<ST>::fx(lv), Lv<ST>::fy(lv),
+                                                                        Lv<ST>::fz(lv), cur[l], out, B(l), 1,
+                                                                        mg.omega, done, bsc, s, cur[l + 1],
+                                                                        ldims(cv));
+            if (rc) return rc;
+            cur[l] = out;
+            rc = smooth(l, mg.post - 1, true, false);
+            if (rc) return rc;
+            continue;
+        }
         // prolongation target: in place, except on level 0 where the last
         // post-sweep must land in z0
         ST* dst = cur[l];
