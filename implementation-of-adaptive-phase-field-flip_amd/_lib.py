@@ -124,7 +124,7 @@ SIGNATURES = {
     "pf_extrapolate": (ctypes.c_int, [PP, P, PP, PP, PP, PP, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, P]),
     "pf_g2p_advect": (ctypes.c_int, [ctypes.POINTER(G2PArgs), P]),
-    "pf_g2p_advect_binned": (ctypes.c_int, [ctypes.POINTER(G2PArgs), P, P, P, P]),
+    "pf_g2p_advect_binned": (ctypes.c_int, [ctypes.POINTER(G2PArgs), P, P, P, P, P, P]),
     "pf_max_speed": (ctypes.c_int, [P, P, P, I64, P, P]),
     "pf_droplets": (ctypes.c_int, [ctypes.POINTER(DropletArgs), P]),
     "pf_velocity_gradient": (ctypes.c_int, [PP, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32, P]),

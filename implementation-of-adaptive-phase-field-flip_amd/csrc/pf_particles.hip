@@ -702,8 +702,8 @@ __device__ inline void rk3_rest(const float* __restrict__ f0, const float* __res
 
 // Wall reflection (kernels.py:567-595), solid push-out (kernels.py:630-679)
 // and the stores, shared by the per-particle and the tiled kernels.
-__device__ inline void finish_particle(const G2PArgs& a, int64_t p, float x, float y, float z, float vx, float vy,
-                                       float vz) {
+__device__ inline void finish_particle(const G2PArgs& a, int64_t p, float x, float y, float z, float& vx, float& vy,
+                                       float& vz) {
     const Dims d = a.d;
     if (a.flags & PF_GA_REFLECT) {
         // kernels.py:567-595
@@ -844,6 +844,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 5))) voi
     finish_particle(a, p, x, y, z, vx, vy, vz);
 }
 
+// Per-particle terms of the CFL speed and the P2G fixed-point bounds: |v|,
+// |m|, |m| max_a |v_a| (particle_max_speed, kernels.py:552-564).  One
+// definition for both kernels that produce them, so the maxima agree bit for
+// bit.
+struct SpeedBounds {
+    float v = 0.f, m = 0.f, mu = 0.f;
+    __device__ inline void add(float vx, float vy, float vz, float mass) {
+        v = fmaxf(v, sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(vx, vx), __fmul_rn(vy, vy)), __fmul_rn(vz, vz))));
+        const float ma = fabsf(mass);
+        m = fmaxf(m, ma);
+        mu = fmaxf(mu, __fmul_rn(ma, fmaxf(fabsf(vx), fmaxf(fabsf(vy), fabsf(vz)))));
+    }
+    // block maxima into out[0..2] (non-negative floats, zeroed beforehand)
+    __device__ inline void flush(float* out, float* sh) {
+        const float a = block_max(v, sh);
+        if (threadIdx.x == 0) atomic_max_nonneg(out, a);
+        const float b = block_max(m, sh);
+        if (threadIdx.x == 0) atomic_max_nonneg(out + 1, b);
+        const float c = block_max(mu, sh);
+        if (threadIdx.x == 0) atomic_max_nonneg(out + 2, c);
+    }
+};
+
 // ============================================================== tiled G2P + advection
 // One workgroup per tile of T^3 cells.  The face grids the tile's particles
 // sample (vel, dvel, vel_liq) are staged in LDS over the tile plus a 2-cell
@@ -948,9 +971,12 @@ __device__ inline void stage_field(float* dst, const float* const* src, const Di
     stage_face<2>(dst + 2 * GB3, src[2], d, ox, oy, oz);
 }
 
+// bounds (optional, with pmass): the CFL speed and P2G bounds of the
+// particles after the step, as pf_max_speed would compute them next step.
 __global__ __launch_bounds__(kG2PThreads) void k_g2p_tiled(G2PArgs a, const int* __restrict__ perm,
                                                            const int* __restrict__ start, int ntx, int nty,
-                                                           int ntz) {
+                                                           int ntz, const float* __restrict__ pmass,
+                                                           float* __restrict__ bounds) {
     // fields: [0, 3 GB3) vel, [3 GB3, 6 GB3) dvel, [6 GB3, 9 GB3) vel_liq
     __shared__ float box[9 * GB3];
     const int nt = ntx * nty * ntz;
@@ -960,6 +986,7 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_tiled(G2PArgs a, const int*
     const int s0 = start[(int64_t)t * T3], s1 = start[(int64_t)(t + 1) * T3];
     if (s0 == s1) return;
     const bool g2p = a.flags & PF_GA_G2P, adv = a.flags & PF_GA_ADVECT;
+    SpeedBounds sb;
     stage_field(box, a.vel, a.d, ox, oy, oz);
     if (g2p) stage_field(box + 3 * GB3, a.dvel, a.d, ox, oy, oz);
     if (adv) stage_field(box + 6 * GB3, a.vel_liq, a.d, ox, oy, oz);
@@ -1066,6 +1093,11 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_tiled(G2PArgs a, const int*
             z = fminf(fmaxf(z, a.lo[2]), a.hi[2]);
         }
         finish_particle(a, p, x, y, z, vx, vy, vz);
+        if (bounds) sb.add(vx, vy, vz, pmass[p]);
+    }
+    if (bounds) {
+        __shared__ float sh[kG2PThreads / 64];
+        sb.flush(bounds, sh);
     }
 }
 
@@ -1087,8 +1119,8 @@ __global__ __launch_bounds__(256) void k_g2p_unbinned(G2PArgs a, const int* __re
             dead = st == 2;
         }
         if (active || dead) continue;
-        finish_particle(a, p, a.px[3 * p], a.px[3 * p + 1], a.px[3 * p + 2], a.pv[3 * p], a.pv[3 * p + 1],
-                        a.pv[3 * p + 2]);
+        float vx = a.pv[3 * p], vy = a.pv[3 * p + 1], vz = a.pv[3 * p + 2];
+        finish_particle(a, p, a.px[3 * p], a.px[3 * p + 1], a.px[3 * p + 2], vx, vy, vz);
     }
 }
 
@@ -1102,25 +1134,17 @@ __global__ __launch_bounds__(256) void k_max_speed(const float* __restrict__ pv,
                                                    const int* __restrict__ pstate, int64_t n,
                                                    float* out) {
     __shared__ float sh[4];
-    float m = 0.f, mm = 0.f, mu = 0.f;
+    SpeedBounds sb;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += (int64_t)gridDim.x * blockDim.x) {
         if (pstate[p] != 0) continue;
-        const float x = pv[3 * p], y = pv[3 * p + 1], z = pv[3 * p + 2];
-        m = fmaxf(m, sqrtf(x * x + y * y + z * z));
-        if (pmass) {
-            const float ma = fabsf(pmass[p]);
-            mm = fmaxf(mm, ma);
-            mu = fmaxf(mu, ma * fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))));
-        }
+        sb.add(pv[3 * p], pv[3 * p + 1], pv[3 * p + 2], pmass ? pmass[p] : 0.0f);
     }
-    m = block_max(m, sh);
-    if (threadIdx.x == 0) atomic_max_nonneg(out, m);
     if (pmass) {
-        mm = block_max(mm, sh);
-        if (threadIdx.x == 0) atomic_max_nonneg(out + 1, mm);
-        mu = block_max(mu, sh);
-        if (threadIdx.x == 0) atomic_max_nonneg(out + 2, mu);
+        sb.flush(out, sh);
+    } else {
+        const float m = block_max(sb.v, sh);
+        if (threadIdx.x == 0) atomic_max_nonneg(out, m);
     }
 }
 
@@ -1299,8 +1323,10 @@ PF_API int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream) {
 }
 
 PF_API int pf_g2p_advect_binned(const pf_g2p_args* args, const int32_t* perm, const int32_t* cell_start,
-                                const int32_t* n_active, pf_stream_t stream) {
+                                const int32_t* n_active, const float* pmass, float* bounds_out,
+                                pf_stream_t stream) {
     PF_REQUIRE(args, "pf_g2p_advect_binned: null args");
+    PF_REQUIRE(!bounds_out || pmass, "pf_g2p_advect_binned: bounds need pmass");
     const pf_g2p_args& in = *args;
     PF_REQUIRE(perm && cell_start && n_active, "pf_g2p_advect_binned: null bins");
     PF_REQUIRE(in.nx > 0 && in.ny > 0 && in.nz > 0 && in.n >= 0, "pf_g2p_advect_binned: bad sizes");
@@ -1342,7 +1368,12 @@ PF_API int pf_g2p_advect_binned(const pf_g2p_args* args, const int32_t* perm, co
     a.alpha_a = in.alpha_air;
     const TileGeom g = tile_geom(in.nx, in.ny, in.nz);
     const int64_t nt = n_tiles(g);
-    k_g2p_tiled<<<(unsigned)nt, kG2PThreads, 0, S(stream)>>>(a, perm, cell_start, g.ntx, g.nty, g.ntz);
+    if (bounds_out) {
+        const int rc = fill_u32(bounds_out, 3, 0u, S(stream));
+        if (rc) return rc;
+    }
+    k_g2p_tiled<<<(unsigned)nt, kG2PThreads, 0, S(stream)>>>(a, perm, cell_start, g.ntx, g.nty, g.ntz, pmass,
+                                                             bounds_out);
     PF_CHECK_LAUNCH();
     // particles outside the bins (droplets): reflection and solids only
     if (in.flags & (PF_GA_REFLECT | PF_GA_SOLIDS)) {

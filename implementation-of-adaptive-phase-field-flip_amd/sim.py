@@ -137,8 +137,12 @@ class Simulator:
         self._n_active = zi((1,))
         sb = int(lib.pf_p2g_stage_bytes(nx, ny, nz, 1 if cfg.k_div > 0.0 else 0))
         self._p2g_stage = torch.empty((sb + 7) // 8, dtype=torch.int64, device=dev)
-        # [max|v|, max m, max m|v|_inf] of the FLIP particles (pf_max_speed)
+        # [max|v|, max m, max m|v|_inf] of the FLIP particles (pf_max_speed);
+        # the fused G2P pass of step() leaves the next step's values here, valid
+        # while _bounds_token matches the particle tensors (_bounds_key)
         self._bounds = z((4,))
+        self._bounds_token = None
+        self._particles_epoch = 0
 
     # ------------------------------------------------------------------ setup
     def set_solids(self, mask):
@@ -170,6 +174,8 @@ class Simulator:
         self._keys = torch.empty(n, dtype=torch.int32, device=dev)
         self._rank = torch.empty(n, dtype=torch.int32, device=dev)
         self.perm = torch.empty(n, dtype=torch.int32, device=dev)
+        self._particles_epoch += 1
+        self._bounds_token = None
 
     # ------------------------------------------------------------------ helpers
     def _read(self, t, count=1):
@@ -267,8 +273,16 @@ class Simulator:
                   *[_lib.ptr(t) for t in self.mom], _lib.ptr(self.cell_mass), _lib.ptr(wsum),
                   _lib.ptr(self._p2g_stage), 8 * self._p2g_stage.numel(), _lib.stream())
 
+    def _bounds_key(self):
+        # torch bumps _version on every in-place write through torch; the
+        # kernels here write through raw pointers and reset the token instead
+        return (self._particles_epoch, self.pv._version, self.pstate._version, self.pmass._version)
+
     def _particle_bounds(self):
-        """max |v| (CFL) and the P2G fixed-point bounds, on the device."""
+        """max |v| (CFL) and the P2G fixed-point bounds, on the device (kept
+        from the previous step's G2P pass when the particles are unchanged)."""
+        if self._bounds_token is not None and self._bounds_token == self._bounds_key():
+            return
         _lib.call("pf_max_speed", _lib.ptr(self.pv), _lib.ptr(self.pmass), _lib.ptr(self.pstate),
                   self.n_particles, _lib.ptr(self._bounds), _lib.stream())
 
@@ -337,16 +351,20 @@ class Simulator:
         # the particles have not moved since _bin: G2P samples LDS tiles
         # through the same bins
         bins = (_lib.ptr(self.perm), _lib.ptr(self.cell_start), _lib.ptr(self._n_active))
+        self._bounds_token = None
         if cfg.droplets:
             # G2P, then the droplet stage, then advection (sim.py:180-185)
             a = self._g2p_args(_lib.GA_G2P, dt)
-            _lib.call("pf_g2p_advect_binned", ctypes.byref(a), *bins, _lib.stream())
+            _lib.call("pf_g2p_advect_binned", ctypes.byref(a), *bins, None, None, _lib.stream())
             self._launch_droplets(dt)
             a = self._g2p_args(_lib.GA_ADVECT | _lib.GA_REFLECT, dt)
             _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
         else:
+            # the same pass leaves the next step's CFL / P2G bounds
             a = self._g2p_args(_lib.GA_G2P | _lib.GA_ADVECT | _lib.GA_REFLECT, dt)
-            _lib.call("pf_g2p_advect_binned", ctypes.byref(a), *bins, _lib.stream())
+            _lib.call("pf_g2p_advect_binned", ctypes.byref(a), *bins, _lib.ptr(self.pmass),
+                      _lib.ptr(self._bounds), _lib.stream())
+            self._bounds_token = self._bounds_key()
         self._mark("g2p_advect")
         div_after, counters = self._read_scalar_block()   # synchronises
         n_drop = 0
@@ -405,6 +423,7 @@ class Simulator:
         _lib.call("pf_project", None, None, _lib.ptr3(self.vel), _lib.ptr3(self.vel_star),
                   _lib.ptr3(self.dvel), nx, ny, nz, self.cfg.dx, 0.0, None, _lib.stream())
         a = self._g2p_args(_lib.GA_G2P)
+        self._bounds_token = None
         _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
 
     def _extrapolate_liquid_velocity(self):
@@ -451,6 +470,7 @@ class Simulator:
             _lib.call("pf_velocity_gradient", _lib.ptr3(self.vel), _lib.ptr(self.shear), nx, ny, nz,
                       cfg.dx, _lib.stream())
         a = self._droplet_args(dt)
+        self._bounds_token = None
         _lib.call("pf_droplets", ctypes.byref(a), _lib.stream())
 
     def _add_droplet_stats(self, counters):
@@ -476,6 +496,7 @@ class Simulator:
     def _advect(self, dt: float):
         """RK3, wall reflection, solid push-out (pfflip/sim.py:408-424)."""
         a = self._g2p_args(_lib.GA_ADVECT | _lib.GA_REFLECT, dt)
+        self._bounds_token = None
         _lib.call("pf_g2p_advect", ctypes.byref(a), _lib.stream())
 
     def state_counts(self):

@@ -280,10 +280,14 @@ int pf_g2p_advect(const pf_g2p_args* args, pf_stream_t stream);
  * positions (perm, cell_start, n_active from that call; pstate unchanged
  * since): one workgroup per 8^3 tile samples LDS copies of the tile's face
  * fields, then particles outside the bins (pstate 1) get the reflection /
- * solids part.  Results equal pf_g2p_advect's.  Replaces the same reference
- * kernels (kernels.py:468-549,567-679). */
+ * solids part.  Results equal pf_g2p_advect's.  If bounds_out is non-NULL
+ * (then pmass is required) it receives what pf_max_speed(pv, pmass, pstate)
+ * returns after the call, computed from the updated particles in the same
+ * pass.  Replaces the same reference kernels (kernels.py:468-549,552-564,
+ * 567-679). */
 int pf_g2p_advect_binned(const pf_g2p_args* args, const int32_t* perm, const int32_t* cell_start,
-                         const int32_t* n_active, pf_stream_t stream);
+                         const int32_t* n_active, const float* pmass, float* bounds_out,
+                         pf_stream_t stream);
 
 /* particle_max_speed (kernels.py:552-564): out[0] = max |v| over
  * pstate == 0 (device floats, overwritten).  With pmass non-NULL also

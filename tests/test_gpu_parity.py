@@ -562,13 +562,41 @@ def test_g2p_binned_is_bitwise_per_particle(pf, res, speed, solids):
         sim.px.copy_(x0)
         sim.pv.copy_(v0)
         a = sim._g2p_args(lib.GA_G2P | lib.GA_ADVECT | lib.GA_REFLECT, dt)
-        extra = ((lib.ptr(sim.perm), lib.ptr(sim.cell_start), lib.ptr(sim._n_active))
+        extra = ((lib.ptr(sim.perm), lib.ptr(sim.cell_start), lib.ptr(sim._n_active), None, None)
                  if name.endswith("binned") else ())
         lib.call(name, __import__("ctypes").byref(a), *extra, lib.stream())
         torch.cuda.synchronize()
         out.append((np_(sim.px).copy(), np_(sim.pv).copy()))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert not np.array_equal(out[0][0], np_(x0))
+
+
+def test_step_bounds_from_g2p_equal_max_speed(pf):
+    """The fused G2P pass leaves the next step's CFL speed and P2G bounds;
+    they equal a fresh pf_max_speed over the same particles bit for bit, so
+    skipping that pass changes nothing; a write to pv through torch makes
+    the simulator recompute them."""
+    say("bounds from g2p")
+    cfg, sim = make(pf, 16, 8, vel_noise=0.3)
+    st = np.zeros(sim.n_particles, np.int32)
+    st[::50] = 1
+    sim.pstate.copy_(torch.from_numpy(st))
+    for _ in range(2):
+        sim.step()
+    assert sim._bounds_token == sim._bounds_key()
+    cached = np_(sim._bounds)[:3].copy()
+    sim._bounds_token = None
+    sim._particle_bounds()
+    torch.cuda.synchronize()
+    assert np.array_equal(np_(sim._bounds)[:3], cached)
+    pv = np_(sim.pv).astype(np.float64)
+    live = st == 0
+    assert cached[0] == np.float32(np.sqrt((pv[live] ** 2).sum(1)).max()) or \
+        abs(cached[0] / np.sqrt((pv[live] ** 2).sum(1)).max() - 1) < 1e-6
+    sim.step()
+    key = sim._bounds_key()
+    sim.pv.mul_(2.0)
+    assert sim._bounds_key() != key
 
 
 # ------------------------------------------------------------------ whole step
