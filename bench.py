@@ -33,16 +33,17 @@ HBM_PEAK_GBS = 8000.0
 
 
 def p2g_bytes(n_particles, n_cells):
-    """Algorithmic bytes of one P2G: x, v (12 B each) and a 1-byte phase per
+    """Algorithmic bytes of one P2G: x, v (12 B each) and the mass (4 B) per
     particle; face mass x3, momentum x3 and cell mass (4 B each) per cell."""
-    return 25 * n_particles + 28 * n_cells
+    return 28 * n_particles + 28 * n_cells
 
 
 def g2p_bytes(n_particles, n_cells):
-    """Algorithmic bytes of the fused G2P + RK3 + reflection pass: read x, v,
-    phase (25 B) and write x, v (24 B) per particle; read vel, dvel and
-    vel_liq (3 x 3 faces x 4 B) per cell."""
-    return 49 * n_particles + 36 * n_cells
+    """Algorithmic bytes of the fused G2P + RK3 + reflection pass, which also
+    leaves the next step's speed bounds: read x, v, phase and mass (32 B) and
+    write x, v (24 B) per particle; read vel, dvel and vel_liq (3 x 3 faces
+    x 4 B) per cell."""
+    return 56 * n_particles + 36 * n_cells
 
 
 def smoother_bytes(n_cells, s_p=4, s_beta=4, s_b=4):
@@ -301,7 +302,7 @@ def stage_rooflines(pf, sim, stages, n_p, n_c, s16):
         out.append(roofline(f"k_rbgs_full level-0 sweep ({sim.res[0]}^3)",
                             fused_sweep_bytes(n_c, s16, s16), ms))
     out.append(roofline("p2g stage (k_max_count + k_p2g + k_p2g_gather)", p2g_bytes(n_p, n_c), stages["p2g"]))
-    out.append(roofline("k_g2p_advect (G2P + RK3 + reflect)", g2p_bytes(n_p, n_c), stages["g2p_advect"]))
+    out.append(roofline("k_g2p_tiled (G2P + RK3 + reflect + bounds)", g2p_bytes(n_p, n_c), stages["g2p_advect"]))
     return out
 
 
@@ -488,7 +489,7 @@ def main():
     kernels = [
         roofline(f"k_rbgs_full level-0 sweep ({n}^3)", fused_sweep_bytes(n_c, s16, s16), smooth_ms),
         roofline("p2g stage (k_max_count + k_p2g + k_p2g_gather)", p2g_bytes(n_p, n_c), stages["p2g"]),
-        roofline("k_g2p_advect (G2P + RK3 + reflect)", g2p_bytes(n_p, n_c), stages["g2p_advect"]),
+        roofline("k_g2p_tiled (G2P + RK3 + reflect + bounds)", g2p_bytes(n_p, n_c), stages["g2p_advect"]),
     ]
     workload = f"dam break z-up {n}^3 x {args.ppc} ppc, 1000:1, MG-PCG tol 1e-6"
     copy_gbs = stream_copy_gbs(torch)

@@ -45,8 +45,8 @@ def test_roofline_bytes():
     assert bench.smoother_bytes(n, 2, 2) == 11 * n    # BF16 p, beta; FP32 b
     assert bench.fused_sweep_bytes(n) == 24 * n
     assert bench.fused_sweep_bytes(n, 2, 2) == 14 * n
-    assert bench.p2g_bytes(10, 1) == 25 * 10 + 28
-    assert bench.g2p_bytes(10, 1) == 49 * 10 + 36
+    assert bench.p2g_bytes(10, 1) == 28 * 10 + 28
+    assert bench.g2p_bytes(10, 1) == 56 * 10 + 36
 
 
 def test_multi_gpu_request_exits():
