@@ -320,7 +320,7 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
                                                    const float* __restrict__ bscale,
                                                    const CT* __restrict__ pcoarse, Dims dc) {
     using T = RbTile<TI, TJ, TK, NTH>;
-    __shared__ float P[T::NP];
+    __shared__ __attribute__((aligned(16))) float P[T::NP];
     const int tid = threadIdx.x;
     const int ntk = (d.nz + TK - 1) / TK, ntj = (d.ny + TJ - 1) / TJ;
     const int tile = xcd_block(blockIdx.x, gridDim.x);
@@ -488,13 +488,20 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
         for (int q = 0; q < CPT; ++q)
             if (tid + q * T::NT < CN) C[tid + q * T::NT] = cv[q];
         __syncthreads();
-        for (int e = tid; e < T::NP; e += T::NT) {
-            const int kk = e % T::PK, jj = (e / T::PK) % T::PJ, ii = e / SI;
+        // staged pairs (k, k+1) with k even share their parent, clamping
+        // included (k0 is even, so a clamped pair clamps to one cell)
+        for (int e2 = tid; e2 < T::NP / 2; e2 += T::NT) {
+            const int m = e2 % (T::PK / 2), row = e2 / (T::PK / 2);
+            const int jj = row % T::PJ, ii = row / T::PJ;
             const int gi = min(max(i0 - 2 + ii, 0), d.nx - 1);
             const int gj = min(max(j0 - 2 + jj, 0), d.ny - 1);
-            const int gk = min(max(k0 - 2 + kk, 0), d.nz - 1);
+            const int gk = min(max(k0 - 2 + 2 * m, 0), d.nz - 1);
             const int ci = (gi >> 1) - (i0 / 2 - 1), cj = (gj >> 1) - (j0 / 2 - 1), ck = (gk >> 1) - (k0 / 2 - 1);
-            P[e] = round_st<ST>(P[e] + C[(ci * CJ + cj) * CK + ck]);
+            const float c = C[(ci * CJ + cj) * CK + ck];
+            float2 v = *reinterpret_cast<float2*>(P + 2 * e2);
+            v.x = round_st<ST>(v.x + c);
+            v.y = round_st<ST>(v.y + c);
+            *reinterpret_cast<float2*>(P + 2 * e2) = v;
         }
         __syncthreads();
     }
