@@ -37,8 +37,10 @@ struct CGState {
     float fpart[MAXB];
     // the coarsest level's `coarsest` red-black sweeps from zero followed by
     // mean removal, a linear map of its right-hand side: column j (the image
-    // of the unit vector e_j) at [j * n, (j + 1) * n), built by pf_mg_build
-    float coarse_op[kCoarseOpMax * kCoarseOpMax];
+    // of the unit vector e_j) at [j * n, (j + 1) * n), built by pf_mg_build.
+    // FP64: at 1000:1 contrast its entries are large and of both signs, and
+    // an FP32 map loses the result to cancellation
+    double coarse_op[kCoarseOpMax * kCoarseOpMax];
 };
 
 static unsigned red_blocks(int64_t n) {
@@ -659,7 +661,7 @@ struct SmallArgs {
     int seg_len[3 * PF_MAX_LEVELS + 1];
     void* p_top;
     const int* done;
-    const float* cop;  // non-NULL: the coarsest level's smoother as a matrix
+    const double* cop;  // non-NULL: the coarsest level's smoother as a matrix
 };
 
 constexpr int kSmallThreads = 1024;
@@ -876,9 +878,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
         const LdsLevel v = lds_level(a, lds, a.L - 1);
         const int n = v.d.nx * v.d.ny * v.d.nz;
         for (int t = tid; t < n; t += nt) {
-            float acc = 0.0f;
-            for (int j = 0; j < n; ++j) acc += a.cop[(int64_t)j * n + t] * v.b[j];
-            v.p[t] = acc;
+            double acc = 0.0;
+            for (int j = 0; j < n; ++j) acc += a.cop[(int64_t)j * n + t] * (double)v.b[j];
+            v.p[t] = (float)acc;
         }
         __syncthreads();
     } else {
@@ -921,21 +923,21 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
 
 // Column j of the coarsest level's smoother map: `sweeps` red-black sweeps
 // (colours (0,1)) from p = 0 with b = e_j, then mean removal, as the
-// single-workgroup V-cycle runs them.  One workgroup per column, one thread
-// per cell (n <= kCoarseOpMax).
+// single-workgroup V-cycle runs them, in FP64.  One workgroup per column,
+// one thread per cell (n <= kCoarseOpMax).
 __global__ __launch_bounds__(kCoarseOpMax) void k_coarse_op(Dims d, const float* __restrict__ bx,
                                                             const float* __restrict__ by,
                                                             const float* __restrict__ bz, int sweeps, float omega,
-                                                            float* __restrict__ op) {
-    __shared__ float P[kCoarseOpMax];
+                                                            double* __restrict__ op) {
+    __shared__ double P[kCoarseOpMax];
     __shared__ double sh[16];
-    __shared__ float shift;
+    __shared__ double shift;
     const int n = d.nx * d.ny * d.nz, j = blockIdx.x, t = threadIdx.x;
     const bool own = t < n;
-    float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    double f[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int nb[6] = {0, 0, 0, 0, 0, 0};
     int colour = 0;
-    float dg = 0.f;
+    double dg = 0.0;
     if (own) {
         const int k = t % d.nz, jj = (t / d.nz) % d.ny, i = t / (d.nz * d.ny);
         f[0] = bx[face_index<0>(d, i, jj, k)];
@@ -952,29 +954,29 @@ __global__ __launch_bounds__(kCoarseOpMax) void k_coarse_op(Dims d, const float*
         nb[5] = cell_index(d, i, jj, min(k + 1, d.nz - 1));
         colour = (i + jj + k) & 1;
         dg = f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
-        P[t] = 0.0f;
+        P[t] = 0.0;
     }
-    const float b = t == j ? 1.0f : 0.0f;
+    const double b = t == j ? 1.0 : 0.0, w = (double)omega;
     __syncthreads();
     for (int sw = 0; sw < sweeps; ++sw) {
         for (int c = 0; c < 2; ++c) {
-            const bool upd = own && colour == c && dg > 0.0f;
-            float v = 0.0f;
+            const bool upd = own && colour == c && dg > 0.0;
+            double v = 0.0;
             if (upd) {
-                const float pc = P[t];
-                const float pn[6] = {P[nb[0]], P[nb[1]], P[nb[2]], P[nb[3]], P[nb[4]], P[nb[5]]};
-                float dd, Ap;
-                stencil_vals(pc, f, pn, dd, Ap);
-                v = rb_relax(pc, b, Ap, dg, omega);
+                const double pc = P[t];
+                double ap = 0.0;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) ap += f[q] * (pc - P[nb[q]]);
+                v = pc + w * (b - ap) / dg;
             }
             __syncthreads();
             if (upd) P[t] = v;
             __syncthreads();
         }
     }
-    double sm[1] = {own ? (double)P[t] : 0.0};
+    double sm[1] = {own ? P[t] : 0.0};
     block_sum<1>(sm, sh);
-    if (t == 0) shift = (float)(-(sm[0] / (double)n));
+    if (t == 0) shift = -(sm[0] / (double)n);
     __syncthreads();
     if (own) op[(int64_t)j * n + t] = P[t] + shift;
 }
