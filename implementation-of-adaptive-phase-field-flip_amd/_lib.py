@@ -96,6 +96,15 @@ class DropletArgs(ctypes.Structure):
     ]
 
 
+class MsbgGrid(ctypes.Structure):
+    _fields_ = [
+        ("nbx", ctypes.c_int32), ("nby", ctypes.c_int32), ("nbz", ctypes.c_int32),
+        ("n0", ctypes.c_int32), ("sh0", ctypes.c_int32), ("nslots", ctypes.c_int32),
+        ("ncells", ctypes.c_int32), ("h0", ctypes.c_float),
+        ("level", P), ("slot", P), ("blk", P), ("blev", P), ("coff", P), ("cslot", P),
+    ]
+
+
 # name -> (restype, argtypes); mirrors include/pfflip.h
 I32, I64, F32, F64, SZ = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
 SIGNATURES = {
@@ -127,6 +136,18 @@ SIGNATURES = {
     "pf_g2p_advect_binned": (ctypes.c_int, [ctypes.POINTER(G2PArgs), P, P, P, P, P, P]),
     "pf_max_speed": (ctypes.c_int, [P, P, P, I64, P, P]),
     "pf_droplets": (ctypes.c_int, [ctypes.POINTER(DropletArgs), P]),
+    "pf_msbg_apply": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, I32, P]),
+    "pf_msbg_residual": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, P]),
+    "pf_msbg_rbgs": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, P, I32, F32, P]),
+    "pf_msbg_coarsen_beta": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MsbgGrid), P, P, P]),
+    "pf_msbg_restrict": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MsbgGrid), P, P, P]),
+    "pf_msbg_prolong_add": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MsbgGrid), P, P, P]),
+    "pf_msbg_geometry": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P]),
+    "pf_msbg_cover": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P]),
+    "pf_vec_reduce_bytes": (SZ, []),
+    "pf_vec_reduce": (ctypes.c_int, [I32, P, P, I64, P, P]),
+    "pf_vec_axpby": (ctypes.c_int, [F32, P, F32, P, I64, P]),
+    "pf_vec_jacobi": (ctypes.c_int, [P, P, P, I64, P]),
     "pf_velocity_gradient": (ctypes.c_int, [PP, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32, P]),
     "pf_rasterize_spray": (ctypes.c_int, [P, P, P, I64, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           F32, P]),

@@ -343,6 +343,68 @@ int pf_velocity_gradient(const float* const vel[3], float* out, int nx, int ny, 
 int pf_rasterize_spray(const float* px, const int32_t* pstate, const float* pdrop_r, int64_t n,
                        uint16_t* density, int sx, int sy, int sz, float dx_spray, pf_stream_t stream);
 
+/* ---------------------------------------------------------------- MSBG
+ * Multiresolution sparse block grid (pfflip/msbg.py): nb blocks of one
+ * geometric size; a level-l block holds (n0 >> l)^3 cells of spacing
+ * h0 * 2^l.  The device arrays are the reference's layout (msbg.py:47-65):
+ * level[nblk] (-1 inactive), slot[nblk] (-1 inactive), blk[nslots],
+ * blev[nslots], coff[nslots] (first cell address), cslot[ncells]; blocks in
+ * slots are Morton ordered; cell (i,j,k) of a slot at coff + (i n + j) n + k.
+ * Face coefficients: bface[3 * addr + axis] is the cell's low face. */
+typedef struct pf_msbg_grid {
+    int32_t nbx, nby, nbz, n0, sh0, nslots, ncells;
+    float h0;
+    const int32_t* level;
+    const int32_t* slot;
+    const int32_t* blk;
+    const int32_t* blev;
+    const int32_t* coff;
+    const int32_t* cslot;
+} pf_msbg_grid;
+
+/* apply_A / compute_diag (mr_operator.py:74-180): out = A p, or with
+ * want_diag the diagonal (p unused). */
+int pf_msbg_apply(const pf_msbg_grid* g, const float* p, const float* bface, float* out, int32_t want_diag,
+                  pf_stream_t stream);
+/* residual (mr_operator.py:183-190): r = b - A p. */
+int pf_msbg_residual(const pf_msbg_grid* g, const float* p, const float* b, const float* bface, float* r,
+                     pf_stream_t stream);
+/* rbgs_sweep (mr_operator.py:193-212): one colour of weighted Gauss-Seidel,
+ * colour from the finest-coordinate checkerboard, cells of that colour
+ * updated from pin (all of them read the values before the half-sweep),
+ * the others copied; pout != pin. */
+int pf_msbg_rbgs(const pf_msbg_grid* g, const float* pin, float* pout, const float* b, const float* bface,
+                 const float* diag, int32_t colour, float omega, pf_stream_t stream);
+/* coarsen_beta (mr_multigrid.py:52-89): coarse face = mean of the 2x2 fine
+ * faces it covers. */
+int pf_msbg_coarsen_beta(const pf_msbg_grid* coarse, const pf_msbg_grid* fine, const float* bfine, float* bcoarse,
+                         pf_stream_t stream);
+/* restrict_sum (mr_multigrid.py:92-99): bcoarse (overwritten) = sum of the
+ * fine values each coarse cell covers. */
+int pf_msbg_restrict(const pf_msbg_grid* fine, const pf_msbg_grid* coarse, const float* rfine, float* bcoarse,
+                     pf_stream_t stream);
+/* prolong_add (mr_multigrid.py:102-109): pfine += pcoarse of the covering
+ * coarse cell. */
+int pf_msbg_prolong_add(const pf_msbg_grid* fine, const pf_msbg_grid* coarse, const float* pcoarse, float* pfine,
+                        pf_stream_t stream);
+/* cell_center / cell_volume (msbg.py:144-153, mr_operator.py:215-220):
+ * centre (ncells x 3), volume, interior (1 if all six neighbours exist);
+ * each may be NULL. */
+int pf_msbg_geometry(const pf_msbg_grid* g, float* centre, float* volume, int32_t* interior, pf_stream_t stream);
+/* hits[c] (overwritten) = number of finest coordinates locate() resolves to
+ * cell c (the addressing check of tests/test_msbg.py:69-89). */
+int pf_msbg_cover(const pf_msbg_grid* g, int32_t* hits, pf_stream_t stream);
+
+/* Vector kernels of the MSBG solver (mr_solver.py:27-69).  pf_vec_reduce:
+ * work[0] = sum x (op 0), <x, y> (op 1) or max |x| (op 2) in FP64, combined
+ * in a fixed order; work holds pf_vec_reduce_bytes() bytes.  pf_vec_axpby:
+ * y = a x + b y, or y = b y + a when x is NULL. */
+size_t pf_vec_reduce_bytes(void);
+int pf_vec_reduce(int32_t op, const float* x, const float* y, int64_t n, double* work, pf_stream_t stream);
+int pf_vec_axpby(float a, const float* x, float b, float* y, int64_t n, pf_stream_t stream);
+/* _jacobi (mr_solver.py:64-69): z = r / diag where diag > 0, else 0. */
+int pf_vec_jacobi(const float* r, const float* diag, float* z, int64_t n, pf_stream_t stream);
+
 /* ---------------------------------------------------------------- adjoint
  * Reverse-mode derivative of one step with dt held fixed (no reference
  * counterpart: the reference has no adjoint).  Each entry point transposes
