@@ -105,6 +105,27 @@ class MsbgGrid(ctypes.Structure):
     ]
 
 
+class MrParams(ctypes.Structure):
+    _fields_ = [
+        ("r0", ctypes.c_float), ("rho_l", ctypes.c_float), ("rho_g", ctypes.c_float),
+        ("rho_min", ctypes.c_float), ("alpha_phi", ctypes.c_float), ("rho_0l", ctypes.c_float),
+        ("rho_tilde_0", ctypes.c_float), ("k_div", ctypes.c_float), ("gravity", ctypes.c_float * 3),
+        ("dt", ctypes.c_float), ("nu_l", ctypes.c_float), ("nu_g", ctypes.c_float),
+        ("alpha_cap", ctypes.c_float), ("lo", ctypes.c_float * 3), ("hi", ctypes.c_float * 3),
+        ("max_level", ctypes.c_int32),
+        ("mass_scale", ctypes.c_double), ("mom_scale", ctypes.c_double), ("w_scale", ctypes.c_double),
+    ]
+
+
+class MrParticles(ctypes.Structure):
+    _fields_ = [("x", P), ("v", P), ("type", P), ("level", P), ("defer", P), ("n", ctypes.c_int64)]
+
+
+class MrFields(ctypes.Structure):
+    _fields_ = [(k, P) for k in ("acc", "cwsum", "phi", "is_liquid", "fmass", "fvel", "fvel_star", "fdvel",
+                                 "fbeta", "rhs", "pressure")]
+
+
 # name -> (restype, argtypes); mirrors include/pfflip.h
 I32, I64, F32, F64, SZ = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
 SIGNATURES = {
@@ -148,6 +169,27 @@ SIGNATURES = {
     "pf_vec_reduce": (ctypes.c_int, [I32, P, P, I64, P, P]),
     "pf_vec_axpby": (ctypes.c_int, [F32, P, F32, P, I64, P]),
     "pf_vec_jacobi": (ctypes.c_int, [P, P, P, I64, P]),
+    "pf_mr_refine_map": (ctypes.c_int, [ctypes.POINTER(MrParticles), P, I32, I32, I32, F32, F32, I32, P]),
+    "pf_mr_max_speed": (ctypes.c_int, [P, I64, P, P]),
+    "pf_mr_p2g": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParticles), ctypes.POINTER(MrParams),
+                                 ctypes.POINTER(MrFields), P]),
+    "pf_mr_finish": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParams), ctypes.POINTER(MrFields), P]),
+    "pf_mr_face_prepare": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParams),
+                                          ctypes.POINTER(MrFields), P]),
+    "pf_mr_rhs": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParams), ctypes.POINTER(MrFields), P]),
+    "pf_mr_project": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParams), ctypes.POINTER(MrFields),
+                                     P]),
+    "pf_mr_max_divergence": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P]),
+    "pf_mr_g2p": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParticles), ctypes.POINTER(MrParams),
+                                 ctypes.POINTER(MrFields), P]),
+    "pf_mr_advect": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParticles), ctypes.POINTER(MrParams),
+                                    P, P]),
+    "pf_mr_block_distance": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, F32, P]),
+    "pf_mr_scan_tmp_ints": (I64, [I64]),
+    "pf_mr_classify": (ctypes.c_int, [ctypes.POINTER(MrParticles), P, I32, I32, I32, F32, F32, I32, I32, I32, P, P,
+                                      P, P, P]),
+    "pf_mr_emit": (ctypes.c_int, [ctypes.POINTER(MrParticles), P, P, P, I32, F32, I32, ctypes.POINTER(MrParticles),
+                                  P]),
     "pf_velocity_gradient": (ctypes.c_int, [PP, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32, P]),
     "pf_rasterize_spray": (ctypes.c_int, [P, P, P, I64, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           F32, P]),

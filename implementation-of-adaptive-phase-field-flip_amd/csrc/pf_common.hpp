@@ -184,4 +184,31 @@ __device__ inline int xcd_block(int b, int nb) {
     return b;  // unreachable for nb = nxcd*q_max layouts; keeps the map total
 }
 
+// ------------------------------------------------------------------ random stream
+// PCG hash stream (Jarzynski & Olano, "Hash Functions for GPU Rendering",
+// JCGT 2020): one 32-bit state per (seed, index) pair.
+__host__ __device__ inline uint32_t pcg_hash(uint32_t v) {
+    const uint32_t s = v * 747796405u + 2891336453u;
+    const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+
+__device__ inline uint32_t rng_init(int32_t seed, int32_t offset) {
+    return pcg_hash((uint32_t)seed + pcg_hash((uint32_t)offset));
+}
+
+// uniform in [0, 1): 24 bits
+__device__ inline float rng_uniform(uint32_t& s) {
+    s = pcg_hash(s);
+    return (float)(s >> 8) * (1.0f / 16777216.0f);
+}
+
+// standard normal (Box-Muller); u1 = 0 is moved to 2^-25 so log stays finite
+__device__ inline float rng_normal(uint32_t& s) {
+    const float u1 = rng_uniform(s);
+    const float u2 = rng_uniform(s);
+    const float r = sqrtf(-2.0f * logf(fmaxf(u1, 2.98023224e-8f)));
+    return r * cosf(6.28318530718f * u2);
+}
+
 }  // namespace pf

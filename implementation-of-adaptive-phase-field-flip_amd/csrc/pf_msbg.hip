@@ -15,78 +15,11 @@
 // resolution jump; reading the old values there makes the half-sweep
 // deterministic.  Restriction gathers each coarse cell's children in a fixed
 // order instead of adding them with atomics.
-#include "pf_common.hpp"
+#include "pf_msbg.hpp"
 
 namespace pf {
 
 int fill_u32(void* dst, int64_t n, uint32_t v, hipStream_t s);
-
-struct MGrid {
-    int nbx, nby, nbz, n0, sh0, nslots, ncells;
-    float h0;
-    const int* level;
-    const int* slot;
-    const int* blk;
-    const int* blev;
-    const int* coff;
-    const int* cslot;
-};
-
-struct MCell {
-    int addr, slot, level;
-};
-
-static MGrid mgrid(const pf_msbg_grid& g) {
-    MGrid m;
-    m.nbx = g.nbx;
-    m.nby = g.nby;
-    m.nbz = g.nbz;
-    m.n0 = g.n0;
-    m.sh0 = g.sh0;
-    m.nslots = g.nslots;
-    m.ncells = g.ncells;
-    m.h0 = g.h0;
-    m.level = g.level;
-    m.slot = g.slot;
-    m.blk = g.blk;
-    m.blev = g.blev;
-    m.coff = g.coff;
-    m.cslot = g.cslot;
-    return m;
-}
-
-// msbg.py:81-108
-__device__ inline MCell locate(const MGrid& g, int fx, int fy, int fz) {
-    MCell c{-1, -1, -1};
-    if (fx < 0 || fy < 0 || fz < 0) return c;
-    const int bx = fx >> g.sh0, by = fy >> g.sh0, bz = fz >> g.sh0;
-    if (bx >= g.nbx || by >= g.nby || bz >= g.nbz) return c;
-    const int s = g.slot[(bx * g.nby + by) * g.nbz + bz];
-    if (s < 0) return c;
-    const int l = g.blev[s];
-    const int n = g.n0 >> l, m = g.n0 - 1;
-    const int i = (fx & m) >> l, j = (fy & m) >> l, k = (fz & m) >> l;
-    c.addr = g.coff[s] + (i * n + j) * n + k;
-    c.slot = s;
-    c.level = l;
-    return c;
-}
-
-// finest coordinate of the cell's low corner, and its level (msbg.py:111-141)
-__device__ inline int4 origin(const MGrid& g, int addr) {
-    const int s = g.cslot[addr];
-    const int l = g.blev[s];
-    const int n = g.n0 >> l;
-    int r = addr - g.coff[s];
-    const int i = r / (n * n);
-    r -= i * n * n;
-    const int j = r / n, k = r - j * n;
-    const int b = g.blk[s];
-    const int bz = b % g.nbz, by = (b / g.nbz) % g.nby, bx = b / (g.nbz * g.nby);
-    return make_int4((bx << g.sh0) + (i << l), (by << g.sh0) + (j << l), (bz << g.sh0) + (k << l), l);
-}
-
-__device__ inline int comp(int axis, int c) { return axis == c ? 1 : 0; }
 
 // sum_f T_f (p_A - p_B) over the faces of cell `addr`, or (want_diag) the
 // sum of T_f, with T = beta * min(hA, hB)^2 / ((hA + hB) / 2)

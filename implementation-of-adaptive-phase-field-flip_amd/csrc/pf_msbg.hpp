@@ -1,0 +1,82 @@
+// MSBG device addressing shared by the Poisson stack and the particle
+// kernels (pfflip/msbg.py:81-141): every lookup goes through finest-level
+// integer coordinates, block = F >> log2(n0), cell = (F & (n0 - 1)) >> l.
+#pragma once
+
+#include "pf_common.hpp"
+
+namespace pf {
+
+struct MGrid {
+    int nbx, nby, nbz, n0, sh0, nslots, ncells;
+    float h0;
+    const int* level;
+    const int* slot;
+    const int* blk;
+    const int* blev;
+    const int* coff;
+    const int* cslot;
+};
+
+struct MCell {
+    int addr, slot, level;
+};
+
+inline MGrid mgrid(const pf_msbg_grid& g) {
+    MGrid m;
+    m.nbx = g.nbx;
+    m.nby = g.nby;
+    m.nbz = g.nbz;
+    m.n0 = g.n0;
+    m.sh0 = g.sh0;
+    m.nslots = g.nslots;
+    m.ncells = g.ncells;
+    m.h0 = g.h0;
+    m.level = g.level;
+    m.slot = g.slot;
+    m.blk = g.blk;
+    m.blev = g.blev;
+    m.coff = g.coff;
+    m.cslot = g.cslot;
+    return m;
+}
+
+// msbg.py:81-108
+__device__ inline MCell locate(const MGrid& g, int fx, int fy, int fz) {
+    MCell c{-1, -1, -1};
+    if (fx < 0 || fy < 0 || fz < 0) return c;
+    const int bx = fx >> g.sh0, by = fy >> g.sh0, bz = fz >> g.sh0;
+    if (bx >= g.nbx || by >= g.nby || bz >= g.nbz) return c;
+    const int s = g.slot[(bx * g.nby + by) * g.nbz + bz];
+    if (s < 0) return c;
+    const int l = g.blev[s];
+    const int n = g.n0 >> l, m = g.n0 - 1;
+    const int i = (fx & m) >> l, j = (fy & m) >> l, k = (fz & m) >> l;
+    c.addr = g.coff[s] + (i * n + j) * n + k;
+    c.slot = s;
+    c.level = l;
+    return c;
+}
+
+// finest coordinate of the cell's low corner, and its level (msbg.py:111-141)
+__device__ inline int4 origin(const MGrid& g, int addr) {
+    const int s = g.cslot[addr];
+    const int l = g.blev[s];
+    const int n = g.n0 >> l;
+    int r = addr - g.coff[s];
+    const int i = r / (n * n);
+    r -= i * n * n;
+    const int j = r / n, k = r - j * n;
+    const int b = g.blk[s];
+    const int bz = b % g.nbz, by = (b / g.nbz) % g.nby, bx = b / (g.nbz * g.nby);
+    return make_int4((bx << g.sh0) + (i << l), (by << g.sh0) + (j << l), (bz << g.sh0) + (k << l), l);
+}
+
+__device__ inline int comp(int axis, int c) { return axis == c ? 1 : 0; }
+
+// the cell containing a world position (msbg.py:154-161)
+__device__ inline MCell locate_at(const MGrid& g, float x, float y, float z) {
+    return locate(g, (int)floorf(x / g.h0), (int)floorf(y / g.h0), (int)floorf(z / g.h0));
+}
+
+}  // namespace pf

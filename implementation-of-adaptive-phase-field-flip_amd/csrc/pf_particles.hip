@@ -170,7 +170,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int* __restri
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = offsets[nb];
 }
 
-static int exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStream_t s) {
+int64_t scan_tmp_ints(int64_t n) { return 2 * (int64_t)cdiv(n, SCAN_TILE) + 1; }
+
+// out[0..n] = exclusive prefix sum of in[0..n), out[n] = total; tmp holds
+// scan_tmp_ints(n) ints
+int exclusive_scan(const int* in, int64_t n, int* out, int* tmp, hipStream_t s) {
     const int nb = (int)cdiv(n, SCAN_TILE);
     int* block_sums = tmp;
     int* offsets = tmp + nb;

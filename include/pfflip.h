@@ -405,6 +405,105 @@ int pf_vec_axpby(float a, const float* x, float b, float* y, int64_t n, pf_strea
 /* _jacobi (mr_solver.py:64-69): z = r / diag where diag > 0, else 0. */
 int pf_vec_jacobi(const float* r, const float* diag, float* z, int64_t n, pf_stream_t stream);
 
+/* ---------------------------------------------------------------- adaptive step
+ * The adaptive PF-FLIP step on an MSBG (pfflip/mr_sim.py:203-310 with the
+ * kernels of pfflip/mr_kernels.py and pfflip/particles.py).  Particles are
+ * (n, 3) float positions and velocities plus int32 type (1 = liquid), level
+ * l_p (radius r0 2^l_p, mass rho 8^l_p) and the deferred-coarsening counter.
+ * Cell fields have ncells entries, face fields 3 * ncells (a cell's three
+ * low faces).  acc holds 8 * ncells int64 fixed-point P2G accumulators
+ * (cell mass, weight sum, face mass x3, face momentum x3, channel-major),
+ * all zero before pf_mr_p2g; pf_mr_finish clears them again.  The scales
+ * are powers of two: value = acc / scale. */
+typedef struct pf_mr_params {
+    float r0;                 /* base particle radius, the finest spacing */
+    float rho_l, rho_g;       /* phase densities */
+    float rho_min;            /* eta_phi rho_g rho~_0: phase-field floor */
+    float alpha_phi;          /* eq. (7) stiffness */
+    float rho_0l;             /* rho~_0 rho_l: eq. (7) liquid reference */
+    float rho_tilde_0;        /* weight sum of a level-0 lattice cell */
+    float k_div;              /* density-correction gain (0: off) */
+    float gravity[3];
+    float dt;
+    float nu_l, nu_g;         /* per-phase viscosity that sets alpha_FLIP */
+    float alpha_cap;          /* upper clamp of alpha_FLIP */
+    float lo[3], hi[3];       /* advection clamp box */
+    int32_t max_level;
+    double mass_scale, mom_scale, w_scale;
+} pf_mr_params;
+
+typedef struct pf_mr_particles {
+    float* x;
+    float* v;
+    int32_t* type;
+    int32_t* level;
+    int32_t* defer;
+    int64_t n;                /* particles (pf_mr_emit's output: its capacity) */
+} pf_mr_particles;
+
+typedef struct pf_mr_fields {
+    int64_t* acc;
+    float* cwsum;
+    float* phi;
+    int32_t* is_liquid;
+    float* fmass;
+    float* fvel;
+    float* fvel_star;
+    float* fdvel;
+    float* fbeta;
+    float* rhs;
+    float* pressure;
+} pf_mr_fields;
+
+/* Steps 1-3, the refinement map (particles.py:49-85): level_map (nbx nby nbz,
+ * x-major) = max_level, then min(l_p) over the blocks each particle's box of
+ * half width cfl_eps touches. */
+int pf_mr_refine_map(const pf_mr_particles* ps, int32_t* level_map, int32_t nbx, int32_t nby, int32_t nbz,
+                     float block_extent, float cfl_eps, int32_t max_level, pf_stream_t stream);
+/* particle_max_speed (mr_kernels.py:507-510): out[0] = max |v|. */
+int pf_mr_max_speed(const float* v, int64_t n, float* out, pf_stream_t stream);
+/* Step 4, p2g (mr_kernels.py:48-133) with derive_mass_radius
+ * (particles.py:234-248) folded in: adds into acc. */
+int pf_mr_p2g(const pf_msbg_grid* g, const pf_mr_particles* ps, const pf_mr_params* prm, pf_mr_fields* f,
+              pf_stream_t stream);
+/* Step 5, face_velocity + cell_phase (mr_kernels.py:136-165): cwsum, phi,
+ * is_liquid, fmass, fvel from acc; clears acc. */
+int pf_mr_finish(const pf_msbg_grid* g, const pf_mr_params* prm, pf_mr_fields* f, pf_stream_t stream);
+/* face_beta + zero_wall_velocity + the u* copy + add_gravity
+ * (mr_kernels.py:168-227, mr_sim.py:233-245): fbeta, fvel_star, fvel. */
+int pf_mr_face_prepare(const pf_msbg_grid* g, const pf_mr_params* prm, pf_mr_fields* f, pf_stream_t stream);
+/* build_rhs (mr_kernels.py:286-313): rhs. */
+int pf_mr_rhs(const pf_msbg_grid* g, const pf_mr_params* prm, pf_mr_fields* f, pf_stream_t stream);
+/* project + field_difference (mr_kernels.py:316-343, 415-418): fvel, fdvel. */
+int pf_mr_project(const pf_msbg_grid* g, const pf_mr_params* prm, pf_mr_fields* f, pf_stream_t stream);
+/* max_divergence (mr_kernels.py:346-353): out[0] = max |div u|. */
+int pf_mr_max_divergence(const pf_msbg_grid* g, const float* fvel, float* out, pf_stream_t stream);
+/* per_particle_alpha_flip + g2p (particles.py:251-276, mr_kernels.py:413-430): v. */
+int pf_mr_g2p(const pf_msbg_grid* g, const pf_mr_particles* ps, const pf_mr_params* prm, const pf_mr_fields* f,
+              pf_stream_t stream);
+/* Step 7, advect_rk3 + reflect_at_walls (mr_kernels.py:421-461): x, v. */
+int pf_mr_advect(const pf_msbg_grid* g, const pf_mr_particles* ps, const pf_mr_params* prm, const float* fvel,
+                 pf_stream_t stream);
+/* Step 6a, interface_flag_per_block + block_distance_field
+ * (mr_kernels.py:464-477, particles.py:387-419, mr_sim.py:163-182):
+ * mixed[blk] = 1 where a block holds both phases, dist (nbx nby nbz) =
+ * Euclidean distance in blocks to the nearest such block, at most max_dist
+ * (max_dist everywhere without one); tmp holds 2 nbx nby nbz ints. */
+int pf_mr_block_distance(const pf_msbg_grid* g, const int32_t* is_liquid, int32_t* mixed, int32_t* tmp, float* dist,
+                         float max_dist, pf_stream_t stream);
+/* Step 6b, classify_particles (particles.py:88-168) and the prefix sum of
+ * resample (particles.py:331-372): lnew, nout (0 delete, 1 keep, 8 split),
+ * defer updated, offsets[0..n] with offsets[n] = output count; scan_tmp
+ * holds pf_mr_scan_tmp_ints(n) ints. */
+int64_t pf_mr_scan_tmp_ints(int64_t n);
+int pf_mr_classify(const pf_mr_particles* ps, const float* dist, int32_t nbx, int32_t nby, int32_t nbz,
+                   float block_extent, float beta_bw, int32_t max_level, int32_t defer_threshold, int32_t seed,
+                   int32_t* lnew, int32_t* nout, int32_t* offsets, int32_t* scan_tmp, pf_stream_t stream);
+/* Step 6c, emit_particles (particles.py:171-231): survivors and split
+ * children into out (out->n = its capacity, at least offsets[n]). */
+int pf_mr_emit(const pf_mr_particles* in, const int32_t* lnew, const int32_t* nout, const int32_t* offsets,
+               int32_t defer_threshold, float r0, int32_t seed, pf_mr_particles* out, pf_stream_t stream);
+
 /* ---------------------------------------------------------------- adjoint
  * Reverse-mode derivative of one step with dt held fixed (no reference
  * counterpart: the reference has no adjoint).  Each entry point transposes

@@ -66,6 +66,12 @@ int main(void) {
   P(pf_mg, lev); P(pf_mg, rhs); P(pf_mg, scalars);
   P(pf_mg_level, p_h);
   P(pf_g2p_args, n); P(pf_g2p_args, solid); P(pf_g2p_args, hi);
+  printf("pf_msbg_grid %zu\n", sizeof(pf_msbg_grid));
+  printf("pf_mr_params %zu\n", sizeof(pf_mr_params));
+  printf("pf_mr_particles %zu\n", sizeof(pf_mr_particles));
+  printf("pf_mr_fields %zu\n", sizeof(pf_mr_fields));
+  P(pf_msbg_grid, level); P(pf_mr_params, dt); P(pf_mr_params, hi); P(pf_mr_params, max_level);
+  P(pf_mr_params, mass_scale); P(pf_mr_params, w_scale); P(pf_mr_particles, n); P(pf_mr_fields, pressure);
   return 0;
 }
 """
@@ -97,6 +103,12 @@ def test_struct_layouts_match_c(tmp_path):
         "pf_droplet_args": ctypes.sizeof(L.DropletArgs),
         **{f"pf_droplet_args.{f}": getattr(L.DropletArgs, f).offset
            for f in ("seed", "vel", "gravity", "lo", "max_substeps", "counters")},
+        "pf_msbg_grid": ctypes.sizeof(L.MsbgGrid), "pf_mr_params": ctypes.sizeof(L.MrParams),
+        "pf_mr_particles": ctypes.sizeof(L.MrParticles), "pf_mr_fields": ctypes.sizeof(L.MrFields),
+        "pf_msbg_grid.level": L.MsbgGrid.level.offset, "pf_mr_particles.n": L.MrParticles.n.offset,
+        "pf_mr_fields.pressure": L.MrFields.pressure.offset,
+        **{f"pf_mr_params.{f}": getattr(L.MrParams, f).offset
+           for f in ("dt", "hi", "max_level", "mass_scale", "w_scale")},
     }
     assert {k: int(v) for k, v in out.items()} == py
 

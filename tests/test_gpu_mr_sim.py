@@ -1,0 +1,369 @@
+"""Adaptive PF-FLIP step on the GPU.
+
+Stage parity: every stage of ``AdaptiveSimulator.step`` runs on the GPU and
+is compared with ``oracle/ref_mr_sim.py`` applied to the same inputs (the
+GPU's state just before the stage), to the FP32 tolerances written at each
+check; discrete outcomes (refinement map, liquid tags, resample decisions,
+distance field) must match exactly except where the oracle's value sits
+within rounding of a threshold.  Behaviour: the reference's own tests
+(tests/test_mr_sim.py) on the HIP simulator.
+"""
+
+import math
+import sys
+from importlib import import_module
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available, load_pkg
+from oracle import ref_msbg as M
+from oracle import ref_mr_sim as R
+
+pytestmark = pytest.mark.gpu
+
+if gpu_available():
+    import torch
+
+
+def say(msg):
+    print(f"[gpu-mr] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def pk():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    load_pkg()
+    name = "implementation-of-adaptive-phase-field-flip_amd"
+    return {k: import_module(f"{name}.{k}") for k in ("config", "mr_sim", "_lib")}
+
+
+def np_(t):
+    return t.detach().cpu().numpy()
+
+
+def seed_pool(sim, cfg, fill=0.5, per=2, seed=0, vel=0.0):
+    """Jittered ppc lattice, liquid below fill (tests/test_mr_sim.py:21-30)."""
+    rng = np.random.default_rng(seed)
+    nb, n0 = sim.grid.nb, sim.grid.n0
+    step = cfg.dx / per
+    idx = np.stack(np.meshgrid(*[np.arange(n * n0 * per) for n in nb], indexing="ij"), -1).reshape(-1, 3)
+    pos = ((idx + 0.5) * step + (rng.random((len(idx), 3)) - 0.5) * step * 0.8).astype(np.float32)
+    liq = pos[:, 1] < fill * sim.domain[1]
+    v = (rng.standard_normal(pos.shape) * vel).astype(np.float32)
+    sim.set_particles(pos, v, np.where(liq, 1, 0).astype(np.int32))
+    return liq
+
+
+def liquid_stats(sim):
+    p = sim.particles.numpy()
+    t = p["type"] == 1
+    return np.linalg.norm(p["v"], axis=1)[t].max(), p["x"][t, 1].mean()
+
+
+def oracle_grid(sim):
+    g = M.Grid(sim.grid.nb, sim.grid.n0, sim.grid.h0, sim.grid.max_level)
+    g.build(sim.grid.level_h, grade=False)
+    return g
+
+
+# ---------------------------------------------------------------- stage parity
+@pytest.fixture(scope="module")
+def staged(pk):
+    """One adaptive step run stage by stage with the GPU state captured
+    before and after each stage."""
+    say("staged step: 8 x 24 x 8, two levels, moving pool")
+    # beta_bw = 0.45 keeps beta d~ off the integers on every plateau of the
+    # block distance (d~ = 1, 2, 3, 4, sqrt 2, ...): there floor() would
+    # depend on how the eight trilinear weights round
+    cfg = pk["config"].Config(res=(8, 24, 8), dx=1 / 16, cfl=1.0, pcg_max_iter=400, pcg_tol=1e-6)
+    sim = pk["mr_sim"].AdaptiveSimulator(cfg, (2, 6, 2), n0=4, max_level=1, beta_bw=0.45)
+    seed_pool(sim, cfg, fill=0.3, vel=0.4)
+    rng = np.random.default_rng(5)
+    # level-1 particles (thinned 1 in 8) in a band next to the interface (they
+    # split) and above y = 1.18, where the level-0 particles' boxes (half
+    # width dx) stay below the top block row, which then refines to level 1
+    p = sim.particles.numpy()
+    y = p["x"][:, 1]
+    up = (y > 1.18) | ((y > 0.55) & (y < 0.7))
+    keep = ~up | (rng.random(len(up)) < 0.125)
+    lev = np.where(up, 1, 0)[keep].astype(np.int32)
+    sim.set_particles(p["x"][keep], p["v"][keep], p["type"][keep], lev)
+    # drive some particles to the coarsening threshold
+    d = np.where(rng.random(keep.sum()) < 0.3, 2, 0).astype(np.int32)
+    sim.particles.defer[: sim.particles.n].copy_(torch.from_numpy(d).cuda())
+    S = {}
+    ps = sim.particles
+    S["x0"], S["v0"], S["type"], S["lev0"], S["defer0"] = (np_(t[: ps.n]) for t in
+                                                           (ps.x, ps.v, ps.type, ps.level, ps.defer))
+    vmax = sim._max_speed()
+    dt = sim.dt_from_cfl(vmax)
+    S["vmax"], S["dt"] = vmax, dt
+    sim._rebuild_grid(dt)
+    S["level_map"] = np_(sim._level_map)
+    prm = sim._params(dt, vmax)
+    sim.stage_p2g(prm)
+    for k in ("cwsum", "phi", "is_liquid", "fmass", "fvel"):
+        S["p2g_" + k] = np_(getattr(sim, k)).copy()
+    sim.stage_forces(prm)
+    for k in ("fbeta", "fvel", "fvel_star", "rhs"):
+        S["forces_" + k] = np_(getattr(sim, k)).copy()
+    S["it"], S["rel"] = sim.stage_solve()
+    S["pressure"] = np_(sim.pressure).copy()
+    sim.stage_project(prm)
+    S["proj_fvel"], S["proj_fdvel"] = np_(sim.fvel).copy(), np_(sim.fdvel).copy()
+    S["max_div"] = float(sim._scalars[1].item())
+    sim.stage_g2p(prm)
+    S["v_g2p"] = np_(ps.v[: ps.n]).copy()
+    S["x_g2p"] = np_(ps.x[: ps.n]).copy()
+    S["defer_g2p"] = np_(ps.defer[: ps.n]).copy()
+    S["lev_g2p"] = np_(ps.level[: ps.n]).copy()
+    S["n_before"], S["n_after"] = sim.stage_resample()
+    S["dist"] = np_(sim._dist).copy()
+    S["mixed"] = np_(sim._mixed).copy()
+    S["res"] = {k: np_(getattr(ps, k)[: ps.n]).copy() for k in ("x", "v", "type", "level", "defer")}
+    sim.stage_advect(prm)
+    S["x_end"], S["v_end"] = np_(ps.x[: ps.n]).copy(), np_(ps.v[: ps.n]).copy()
+    torch.cuda.synchronize()
+    S["sim"], S["cfg"], S["g"] = sim, cfg, oracle_grid(sim)
+    S["c"] = R.Consts(cfg, sim.rho_tilde_0, dt, sim.domain, sim.max_level)
+    return S
+
+
+def test_refinement_map(staged):
+    say("refine map")
+    S = staged
+    g = S["g"]
+    ref = R.refine_map(g, S["x0"].astype(np.float64), S["lev0"], max(S["cfg"].cfl, 1.0) * S["cfg"].dx, 1)
+    assert np.array_equal(S["level_map"], ref)
+    assert set(np.unique(S["sim"].grid.level_h)) == {0, 1}
+
+
+def test_p2g_and_phase(staged):
+    say("p2g + finish")
+    S, g, c = staged, staged["g"], staged["c"]
+    cm, cw, fm, fp = R.p2g(g, S["x0"], S["v0"], S["type"], S["lev0"], c.r0, c.rho_l, c.rho_g)
+    phi, tag, fv = R.finish(g, cm, fm, fp, c)
+    assert np.abs(S["p2g_cwsum"] - cw).max() < 2e-6 * cw.max()
+    assert np.abs(S["p2g_fmass"] - fm).max() < 2e-6 * fm.max()
+    assert np.abs(S["p2g_phi"] - phi).max() < 1e-5
+    sure = np.abs(phi - 0.5) > 1e-4
+    assert np.array_equal(S["p2g_is_liquid"][sure], tag[sure])
+    heavy = fm > 1e-3 * fm.max()
+    assert np.abs(S["p2g_fvel"][heavy] - fv[heavy]).max() < 1e-4 * max(np.abs(fv).max(), 1e-3)
+    assert 0.2 < tag.mean() < 0.8
+
+
+def test_forces_and_rhs(staged):
+    say("face prepare + rhs")
+    S, g, c = staged, staged["g"], staged["c"]
+    beta, u, star = R.face_prepare(g, S["p2g_is_liquid"], S["p2g_fmass"].astype(np.float64),
+                                   S["p2g_fvel"].astype(np.float64), c)
+    assert np.allclose(S["forces_fbeta"], beta, rtol=2e-6, atol=0)
+    assert np.abs(S["forces_fvel_star"] - star).max() == 0.0
+    assert np.abs(S["forces_fvel"] - u).max() < 1e-6 * max(np.abs(u).max(), 1.0)
+    b = R.rhs(g, S["forces_fvel"].astype(np.float64), S["p2g_cwsum"].astype(np.float64), c)
+    assert np.abs(S["forces_rhs"] - b).max() < 1e-5 * np.abs(b).max()
+
+
+def test_solve_project_and_divergence(staged):
+    say("solve + project")
+    S, g, c = staged, staged["g"], staged["c"]
+    assert S["rel"] < 1e-6
+    # the pressure solves the operator of the GPU's face coefficients, to
+    # the componentwise backward error of an FP32 vector: |b - A p| is
+    # bounded by a few ulps of |A| |p| (at a 1000:1 density ratio |A| |p|
+    # is far above |b|, so a bound relative to |b| would measure rounding)
+    A = M.assemble(g, S["forces_fbeta"].astype(np.float64))
+    b = S["forces_rhs"].astype(np.float64)
+    p = S["pressure"].astype(np.float64)
+    r = (b - b.mean()) - A @ p
+    scale = abs(A) @ np.abs(p)
+    assert (np.abs(r) <= 1e-5 * scale + 1e-6 * np.abs(b).max()).all()
+    u, du = R.project(g, S["pressure"].astype(np.float64), S["forces_fbeta"].astype(np.float64),
+                      S["forces_fvel"].astype(np.float64), S["forces_fvel_star"].astype(np.float64), c.dt)
+    scale = np.abs(u).max()
+    assert np.abs(S["proj_fvel"] - u).max() < 1e-5 * scale
+    assert np.abs(S["proj_fdvel"] - du).max() < 1e-5 * scale
+    div = R.max_divergence(g, S["proj_fvel"].astype(np.float64))
+    assert S["max_div"] == pytest.approx(div, rel=1e-4, abs=1e-6)
+
+
+def test_g2p(staged):
+    say("g2p")
+    S, g, c = staged, staged["g"], staged["c"]
+    v = R.g2p(g, S["x0"], S["v0"], S["type"], S["proj_fvel"].astype(np.float64), S["proj_fdvel"].astype(np.float64),
+              c)
+    assert np.abs(S["v_g2p"] - v).max() < 1e-5 * max(np.abs(v).max(), 1.0)
+    assert np.array_equal(S["x_g2p"], S["x0"])
+
+
+def test_distance_field_and_resample(staged):
+    say("distance + resample")
+    S, g = staged, staged["g"]
+    sim = S["sim"]
+    mixed = R.block_mixed(g, S["p2g_is_liquid"])
+    assert np.array_equal(S["mixed"], mixed)
+    dist = R.block_distance(mixed, g.nb)
+    assert np.array_equal(S["dist"].reshape(g.nb), dist)
+    ln, nout, defer = R.classify(S["x_g2p"], S["lev_g2p"], S["defer_g2p"], dist, sim.block_extent, sim.beta_bw,
+                                 sim.max_level, sim.defer_threshold, 0)
+    assert S["n_before"] == len(nout) and S["n_after"] == int(nout.sum())
+    xo, vo, to, lo, do = R.emit(S["x_g2p"], S["v_g2p"], S["type"], S["lev_g2p"], defer, ln, nout,
+                                sim.defer_threshold, S["cfg"].r_p, 7919)
+    res = S["res"]
+    assert np.array_equal(res["type"], to) and np.array_equal(res["level"], lo) and np.array_equal(res["defer"], do)
+    assert np.abs(res["x"] - xo).max() < 1e-6
+    assert np.array_equal(res["v"], vo.astype(np.float32))
+    promoted = (ln > S["lev_g2p"]) & (defer >= sim.defer_threshold) & (nout == 1)
+    assert (nout == 0).any() and (nout == 8).any() and promoted.any()
+
+
+def test_advect(staged):
+    say("advect")
+    S, g, c = staged, staged["g"], staged["c"]
+    res = S["res"]
+    x, v = R.advect(g, res["x"], res["v"], S["proj_fvel"].astype(np.float64), c)
+    assert np.abs(S["x_end"] - x).max() < 1e-6
+    assert np.abs(S["v_end"] - v).max() == 0.0
+
+
+def test_step_is_deterministic(pk):
+    """Two runs of the same adaptive steps give bitwise equal particles (P2G
+    sums are order independent)."""
+    say("determinism")
+    out = []
+    for _ in range(2):
+        cfg = pk["config"].Config(res=(8, 16, 8), dx=1 / 16, cfl=1.0, pcg_max_iter=400)
+        sim = pk["mr_sim"].AdaptiveSimulator(cfg, (2, 4, 2), n0=4, max_level=1, beta_bw=0.5)
+        seed_pool(sim, cfg, fill=0.45, vel=0.4)
+        for _ in range(3):
+            sim.step()
+        out.append(sim.particles.numpy())
+    for k in ("x", "v", "level", "defer"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+# ---------------------------------------------------------------- reference behaviour
+def test_msbg_uniform_matches_stage1_hydrostatics(pk):
+    """tests/test_mr_sim.py:42-62: with max_level = 0 the path holds a pool at rest."""
+    say("hydrostatic pool, max_level 0")
+    cfg = pk["config"].Config(res=(32,) * 3, dx=1 / 32, cfl=1.0, pcg_max_iter=1500)
+    sim = pk["mr_sim"].AdaptiveSimulator(cfg, (4, 4, 4), n0=8, max_level=0)
+    seed_pool(sim, cfg)
+    speeds, y0 = [], None
+    for _ in range(10):
+        sim.step()
+        v, y = liquid_stats(sim)
+        speeds.append(v)
+        y0 = y if y0 is None else y0
+    _, y_end = liquid_stats(sim)
+    g_dt = 9.81 * sim.stats[-1]["dt"]
+    assert max(speeds) < 0.6 * g_dt, (max(speeds), g_dt)
+    assert abs(y_end - y0) < 1e-3
+    assert sim.grid.compression == pytest.approx(1.0)
+
+
+@pytest.mark.parametrize("max_level", [0, 1, 2])
+def test_p2g_recovers_uniform_density_at_every_level(pk, max_level):
+    """tests/test_mr_sim.py:65-116."""
+    say(f"density at level {max_level}")
+    cfg = pk["config"].Config(res=(32,) * 3, dx=1 / 32, cfl=1.0, pcg_max_iter=50)
+    sim = pk["mr_sim"].AdaptiveSimulator(cfg, (4, 4, 4), n0=8, max_level=max_level)
+    seed_pool(sim, cfg)
+    p = sim.particles.numpy()
+    full = len(p["x"])
+    rng = np.random.default_rng(1)
+    sel = rng.choice(full, size=full // 8 ** max_level, replace=False)
+    sim.set_particles(p["x"][sel], np.zeros_like(p["x"][sel]), p["type"][sel],
+                      np.full(len(sel), max_level, np.int32))
+    sim._rebuild_grid(1e-3)
+    sim.stage_p2g(sim._params(1e-3, 0.0))
+    phi = np_(sim.phi)
+    assert phi.max() > 0.9, phi.max()
+    assert phi.min() == 0.0
+    frac = (np_(sim.is_liquid) > 0).mean()
+    assert 0.4 < frac < 0.6, frac
+
+
+def test_adaptive_run_conserves_liquid_centroid(pk):
+    """tests/test_mr_sim.py:119-136."""
+    say("adaptive pool, 8 steps")
+    cfg = pk["config"].Config(res=(32,) * 3, dx=1 / 32, cfl=1.0, pcg_max_iter=800)
+    sim = pk["mr_sim"].AdaptiveSimulator(cfg, (4, 4, 4), n0=8, max_level=1, beta_bw=0.34)
+    seed_pool(sim, cfg)
+    ys = []
+    for _ in range(8):
+        sim.step()
+        ys.append(liquid_stats(sim)[1])
+    assert max(ys) - min(ys) < 0.02 * sim.domain[1]
+    assert any(r["compression"] > 1.5 for r in sim.stats)
+
+
+def test_multigrid_iterations_are_resolution_independent(pk):
+    """tests/test_mr_sim.py:140-197: MG-PCG iterations stay flat over a 4x
+    resolution range on a graded grid with a 1000:1 coefficient jump."""
+    say("MG iterations vs resolution")
+    name = "implementation-of-adaptive-phase-field-flip_amd"
+    msbg = import_module(f"{name}.msbg")
+    mr_solver = import_module(f"{name}.mr_solver")
+    counts = []
+    for nb in (2, 4, 8):
+        n0 = 16
+        g = msbg.MSBG((nb,) * 3, n0=n0, h0=1.0 / (nb * n0), max_level=1)
+        bx, _, _ = np.meshgrid(*[np.arange(nb)] * 3, indexing="ij")
+        g.build(np.where(bx < nb // 2, 0, 1).ravel().astype(np.int32))
+        c, v, _ = g.geometry()
+        x, V = np_(c).astype(np.float64), np_(v).astype(np.float64)
+        bf = np.ones(g.ncells * 3, np.float32)
+        bf[np.repeat(x[:, 1] < 0.5, 3)] = 1e-3
+        k = 2 * np.pi
+        ex = np.cos(k * x[:, 0]) * np.cos(k * x[:, 1]) * np.cos(k * x[:, 2])
+        b = 3 * k * k * ex * V
+        b -= b.mean()
+        p = g.zeros()
+        it, rel = mr_solver.MRSolver(g, preconditioner="mg").solve(
+            torch.from_numpy(bf).cuda(), torch.from_numpy(b.astype(np.float32)).cuda(), p, tol=1e-4, max_iter=500)
+        assert rel < 1e-4, (nb, rel)
+        counts.append(it)
+    say(f"  iterations {counts}")
+    assert max(counts) < 30, counts
+    assert counts[-1] / counts[0] < 2.5, counts
+
+
+def test_resample_does_not_mistag_bulk_liquid(pk):
+    """tests/test_mr_sim.py:201-245: at ppc = 8 alpha_phi = 1 coarsening
+    mis-tags bulk liquid cells; alpha_phi = 0.25 restores the margin and the
+    velocity settles."""
+    say("mistagging, 2 x 10 steps")
+
+    def mistagged(alpha_phi, ppc):
+        cfg = pk["config"].Config(res=(32,) * 3, dx=1 / 32, cfl=1.0, pcg_max_iter=400, alpha_phi=alpha_phi, ppc=ppc)
+        with pytest.warns(RuntimeWarning) if alpha_phi == 1.0 else _nullcontext():
+            sim = pk["mr_sim"].AdaptiveSimulator(cfg, (4, 4, 4), n0=8, max_level=1, beta_bw=0.34)
+        seed_pool(sim, cfg, per=round(ppc ** (1 / 3)))
+        worst, speeds = 0, []
+        for _ in range(10):
+            sim.step()
+            c, _, _ = sim.grid.geometry()
+            deep = np_(c)[:, 1] < 0.40 * sim.domain[1]
+            worst = max(worst, int((deep & (np_(sim.is_liquid) == 0)).sum()))
+            speeds.append(liquid_stats(sim)[0])
+        return worst, speeds
+
+    bad, speeds_bad = mistagged(1.0, 8)
+    good, speeds_good = mistagged(0.25, 8)
+    say(f"  bad {bad} good {good} speeds {speeds_bad[-1]:.4f} {speeds_good[-1]:.4f}")
+    assert bad > 0
+    assert good == 0
+    assert speeds_good[-1] < 0.1 * speeds_bad[-1]
+
+
+class _nullcontext:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *a):
+        return False
