@@ -160,6 +160,10 @@ SIGNATURES = {
     "pf_msbg_apply": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, I32, P]),
     "pf_msbg_residual": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, P]),
     "pf_msbg_rbgs": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, P, I32, F32, P]),
+    "pf_msbg_rbgs_active": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, P, P, I32, F32, P, P]),
+    "pf_msbg_relax_update": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, P, I32, F32, P, P]),
+    "pf_msbg_coarsest_max_cells": (I32, []),
+    "pf_msbg_coarsest_smooth": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, I32, F32, P]),
     "pf_msbg_coarsen_beta": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MsbgGrid), P, P, P]),
     "pf_msbg_restrict": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MsbgGrid), P, P, P]),
     "pf_msbg_prolong_add": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MsbgGrid), P, P, P]),

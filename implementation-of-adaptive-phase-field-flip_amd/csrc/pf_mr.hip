@@ -78,20 +78,52 @@ __device__ inline void acc_add(unsigned long long* p, float v, float scale) {
 
 // ------------------------------------------------------------------ 1. refinement map
 // M[block] = min(l_p, M[block]) over the blocks a particle's box of half
-// width cfl_eps touches (particles.py:49-85)
+// width cfl_eps touches (particles.py:49-85).  Neighbouring particles touch
+// the same few blocks, so a wave combines its lanes first: each lane walks
+// its blocks in index order; per round the first pending lane's block is
+// taken, the lanes at that block reduce min(l), and one lane issues the
+// atomic (the map only decreases, so it is skipped where the map already
+// holds the value or less).
 __global__ __launch_bounds__(256) void k_mr_refine_map(const float* __restrict__ x, const int* __restrict__ lev,
-                                                       int64_t n, int* __restrict__ map, int nbx, int nby, int nbz,
-                                                       float bext, float eps) {
+                                                       int64_t n, int* map, int nbx, int nby, int nbz, float bext,
+                                                       float eps) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const float px = x[3 * p], py = x[3 * p + 1], pz = x[3 * p + 2];
-    const int l = lev[p];
-    const int i0 = max((int)floorf((px - eps) / bext), 0), i1 = min((int)floorf((px + eps) / bext), nbx - 1);
-    const int j0 = max((int)floorf((py - eps) / bext), 0), j1 = min((int)floorf((py + eps) / bext), nby - 1);
-    const int k0 = max((int)floorf((pz - eps) / bext), 0), k1 = min((int)floorf((pz + eps) / bext), nbz - 1);
-    for (int i = i0; i <= i1; ++i)
-        for (int j = j0; j <= j1; ++j)
-            for (int k = k0; k <= k1; ++k) atomicMin(map + (i * nby + j) * nbz + k, l);
+    const int lane = threadIdx.x & 63;
+    int i0 = 0, i1 = -1, j0 = 0, j1 = 0, k0 = 0, k1 = 0, l = 0;
+    if (p < n) {
+        const float px = x[3 * p], py = x[3 * p + 1], pz = x[3 * p + 2];
+        l = lev[p];
+        i0 = max((int)floorf((px - eps) / bext), 0);
+        i1 = min((int)floorf((px + eps) / bext), nbx - 1);
+        j0 = max((int)floorf((py - eps) / bext), 0);
+        j1 = min((int)floorf((py + eps) / bext), nby - 1);
+        k0 = max((int)floorf((pz - eps) / bext), 0);
+        k1 = min((int)floorf((pz + eps) / bext), nbz - 1);
+        if (j0 > j1 || k0 > k1) i1 = i0 - 1;  // box outside the raster
+    }
+    int i = i0, j = j0, k = k0;
+    for (;;) {
+        const bool pending = i <= i1;
+        const unsigned long long act = __ballot(pending);
+        if (act == 0ull) break;
+        const int key = pending ? (i * nby + j) * nbz + k : -1;
+        const int first = __ffsll((long long)act) - 1;
+        const int lead = __shfl(key, first, 64);
+        const bool mine = pending && key == lead;
+        int m = mine ? l : 0x7fffffff;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+        if (lane == first && map[lead] > m) atomicMin(map + lead, m);
+        if (mine) {
+            if (++k > k1) {
+                k = k0;
+                if (++j > j1) {
+                    j = j0;
+                    ++i;
+                }
+            }
+        }
+    }
 }
 
 // max |v| over particles (mr_kernels.py:507-510)

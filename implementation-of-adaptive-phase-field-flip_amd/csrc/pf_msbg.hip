@@ -21,17 +21,16 @@ namespace pf {
 
 int fill_u32(void* dst, int64_t n, uint32_t v, hipStream_t s);
 
-// sum_f T_f (p_A - p_B) over the faces of cell `addr`, or (want_diag) the
-// sum of T_f, with T = beta * min(hA, hB)^2 / ((hA + hB) / 2)
-// (mr_operator.py:74-166).  The faces are walked in the reference's order.
-__device__ inline float apply_row(const MGrid& g, const float* __restrict__ p, const float* __restrict__ bface,
-                                  int addr, bool want_diag) {
+// Calls f(neighbour address, T) for every coupling of cell `addr`, with
+// T = beta * min(hA, hB)^2 / ((hA + hB) / 2), walking the faces in the
+// reference's order (mr_operator.py:74-166); across a face to finer cells
+// the 2x2 patch behind it.
+template <class F>
+__device__ inline void for_each_coupling(const MGrid& g, const float* __restrict__ bface, int addr, F&& f) {
     const int4 o = origin(g, addr);
     const int la = o.w;
     const int ext = 1 << la;
     const float hA = g.h0 * (float)ext;
-    const float pa = want_diag ? 0.0f : p[addr];
-    float acc = 0.0f, diag = 0.0f;
     for (int axis = 0; axis < 3; ++axis) {
         const int ex = comp(axis, 0), ey = comp(axis, 1), ez = comp(axis, 2);
         for (int side = 0; side < 2; ++side) {
@@ -54,9 +53,7 @@ __device__ inline float apply_row(const MGrid& g, const float* __restrict__ p, c
                 area = area * area;
                 const float dist = 0.5f * (hA + hB);
                 const float beta = side == 0 ? bface[addr * 3 + axis] : bface[nb.addr * 3 + axis];
-                const float t = beta * area / dist;
-                diag += t;
-                if (!want_diag) acc += t * (pa - p[nb.addr]);
+                f(nb.addr, beta * area / dist);
             } else {
                 // finer: the 2x2 patch across this face
                 const int sub = 1 << nb.level;
@@ -71,13 +68,23 @@ __device__ inline float apply_row(const MGrid& g, const float* __restrict__ p, c
                                                 fz + comp(a1, 2) * du + comp(a2, 2) * dv);
                         if (nc.addr < 0) continue;
                         const float beta = side == 0 ? bface[addr * 3 + axis] : bface[nc.addr * 3 + axis];
-                        const float t = beta * area / dist;
-                        diag += t;
-                        if (!want_diag) acc += t * (pa - p[nc.addr]);
+                        f(nc.addr, beta * area / dist);
                     }
             }
         }
     }
+}
+
+// sum_f T_f (p_A - p_B) over the couplings of cell `addr`, or (want_diag)
+// the sum of T_f
+__device__ inline float apply_row(const MGrid& g, const float* __restrict__ p, const float* __restrict__ bface,
+                                  int addr, bool want_diag) {
+    const float pa = want_diag ? 0.0f : p[addr];
+    float acc = 0.0f, diag = 0.0f;
+    for_each_coupling(g, bface, addr, [&](int nb, float t) {
+        diag += t;
+        if (!want_diag) acc += t * (pa - p[nb]);
+    });
     return want_diag ? diag : acc;
 }
 
@@ -111,6 +118,176 @@ __global__ __launch_bounds__(256) void k_msbg_rbgs(MGrid g, const float* __restr
         if (d > 0.0f) v = v + omega * (b[a] - apply_row(g, pin, bface, a, false)) / d;
     }
     pout[a] = v;
+}
+
+// All sweeps of the coarsest level in one workgroup: each row's couplings
+// (neighbour, T) are gathered into LDS once, then every half-sweep reads LDS
+// only.  Same couplings, same order, same arithmetic as k_msbg_rbgs, so the
+// result equals the launched half-sweeps bit for bit.  p is read and
+// overwritten; the sweep count is even in half-sweeps, so no copy is left.
+constexpr int kRowMax = 24;  // 6 faces x a 2x2 patch
+constexpr int kCoarsestThreads = 512;
+
+__host__ __device__ inline size_t coarsest_lds_bytes(int n) {
+    return (size_t)n * (kRowMax * (sizeof(float) + sizeof(unsigned short)) + 4 * sizeof(float) + 2);
+}
+
+__global__ __launch_bounds__(kCoarsestThreads) void k_msbg_coarsest(MGrid g, float* __restrict__ p,
+                                                                    const float* __restrict__ b,
+                                                                    const float* __restrict__ bface,
+                                                                    const float* __restrict__ diag, int sweeps,
+                                                                    float omega) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const int n = g.ncells;
+    float* coef = reinterpret_cast<float*>(lds);
+    float* p0 = coef + (size_t)n * kRowMax;
+    float* p1 = p0 + n;
+    float* bb = p1 + n;
+    float* dd = bb + n;
+    unsigned short* nbr = reinterpret_cast<unsigned short*>(dd + n);
+    unsigned char* cnt = reinterpret_cast<unsigned char*>(nbr + (size_t)n * kRowMax);
+    unsigned char* col = cnt + n;
+    for (int a = threadIdx.x; a < n; a += blockDim.x) {
+        int k = 0;
+        for_each_coupling(g, bface, a, [&](int nb, float t) {
+            coef[a * kRowMax + k] = t;
+            nbr[a * kRowMax + k] = (unsigned short)nb;
+            ++k;
+        });
+        cnt[a] = (unsigned char)k;
+        const int4 o = origin(g, a);
+        col[a] = (unsigned char)(((o.x >> o.w) + (o.y >> o.w) + (o.z >> o.w)) & 1);
+        p0[a] = p[a];
+        bb[a] = b[a];
+        dd[a] = diag[a];
+    }
+    __syncthreads();
+    float* pin = p0;
+    float* pout = p1;
+    for (int s = 0; s < sweeps; ++s)
+        for (int c = 0; c < 2; ++c) {
+            for (int a = threadIdx.x; a < n; a += blockDim.x) {
+                float v = pin[a];
+                if (col[a] == c) {
+                    const float d = dd[a];
+                    if (d > 0.0f) {
+                        const float pa = pin[a];
+                        float acc = 0.0f;
+                        const int m = cnt[a];
+                        for (int k = 0; k < m; ++k) acc += coef[a * kRowMax + k] * (pa - pin[nbr[a * kRowMax + k]]);
+                        v = v + omega * (bb[a] - acc) / d;
+                    }
+                }
+                pout[a] = v;
+            }
+            __syncthreads();
+            float* t = pin;
+            pin = pout;
+            pout = t;
+        }
+    for (int a = threadIdx.x; a < n; a += blockDim.x) p[a] = pin[a];
+}
+
+// ------------------------------------------------------------------ adaptive relaxation
+// Algorithm 2 of the paper as the reference runs it on a GPU
+// (adaptive_relax.py:1-241): red-black sweeps restricted to an active block
+// set, the per-block effective residual max |delta| diag, a threshold from
+// the log-histogram of the first residuals, and re-activation of the blocks
+// above it with their face neighbours.  Like k_msbg_rbgs the half-sweep reads
+// pin and writes every cell of pout.
+
+// one colour over the active blocks; delta = |update| of the updated cells
+// (adaptive_relax.py:84-109)
+__global__ __launch_bounds__(256) void k_msbg_rbgs_active(MGrid g, const float* __restrict__ pin,
+                                                          float* __restrict__ pout, const float* __restrict__ b,
+                                                          const float* __restrict__ bface,
+                                                          const float* __restrict__ diag,
+                                                          const int* __restrict__ active, int colour, float omega,
+                                                          float* __restrict__ delta) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= g.ncells) return;
+    float v = pin[a];
+    if (active[g.cslot[a]]) {
+        const int4 o = origin(g, a);
+        if ((((o.x >> o.w) + (o.y >> o.w) + (o.z >> o.w)) & 1) == colour) {
+            const float d = diag[a];
+            if (d > 0.0f) {
+                const float upd = omega * (b[a] - apply_row(g, pin, bface, a, false)) / d;
+                v = v + upd;
+                delta[a] = fabsf(upd);
+            }
+        }
+    }
+    pout[a] = v;
+}
+
+// res[slot] = max over the block's cells of delta * diag, 0 when inactive
+// (adaptive_relax.py:112-129); one workgroup per block
+__global__ __launch_bounds__(256) void k_msbg_block_res(MGrid g, const float* __restrict__ delta,
+                                                        const float* __restrict__ diag,
+                                                        const int* __restrict__ active, float* __restrict__ res) {
+    const int s = blockIdx.x;
+    __shared__ float sh[4];
+    float m = 0.0f;
+    if (active[s]) {
+        const int c0 = g.coff[s];
+        const int nn = g.n0 >> g.blev[s];
+        const int cnt = nn * nn * nn;
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) m = fmaxf(m, delta[c0 + i] * diag[c0 + i]);
+    }
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) res[s] = m;
+}
+
+// theta = exp(mean + sigma std) of log(res) over the positive entries
+// (population std, two passes, FP64 in a fixed order); +inf when none is
+// positive, which stops all further activation (adaptive_relax.py:220-231)
+__global__ __launch_bounds__(256) void k_msbg_theta(const float* __restrict__ res, int ns, float sigma,
+                                                    float* __restrict__ theta) {
+    __shared__ double sh[2 * 16];
+    double v[2] = {0.0, 0.0};
+    for (int s = threadIdx.x; s < ns; s += blockDim.x)
+        if (res[s] > 0.0f) {
+            v[0] += log((double)res[s]);
+            v[1] += 1.0;
+        }
+    block_sum<2>(v, sh);
+    __shared__ double mean_cnt[2];
+    if (threadIdx.x == 0) {
+        mean_cnt[0] = v[1] > 0.0 ? v[0] / v[1] : 0.0;
+        mean_cnt[1] = v[1];
+    }
+    __syncthreads();
+    const double mean = mean_cnt[0], cnt = mean_cnt[1];
+    double w[1] = {0.0};
+    for (int s = threadIdx.x; s < ns; s += blockDim.x)
+        if (res[s] > 0.0f) {
+            const double d = log((double)res[s]) - mean;
+            w[0] += d * d;
+        }
+    block_sum<1>(w, sh);
+    if (threadIdx.x == 0)
+        *theta = cnt > 0.0 ? (float)exp(mean + (double)sigma * sqrt(w[0] / cnt)) : __int_as_float(0x7f800000);
+}
+
+// blocks above theta and their face neighbours form the next active set
+// (adaptive_relax.py:132-160); the writers all store 1
+__global__ __launch_bounds__(256) void k_msbg_update_active(MGrid g, const float* __restrict__ res,
+                                                            const float* __restrict__ theta,
+                                                            int* __restrict__ active_next) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.nslots) return;
+    if (!(res[s] > *theta)) return;
+    active_next[s] = 1;
+    const int b = g.blk[s];
+    const int bz = b % g.nbz, by = (b / g.nbz) % g.nby, bx = b / (g.nbz * g.nby);
+    for (int d = 0; d < 6; ++d) {
+        const int ax = d >> 1, sg = (d & 1) ? 1 : -1;
+        const int nx = bx + (ax == 0 ? sg : 0), ny = by + (ax == 1 ? sg : 0), nz = bz + (ax == 2 ? sg : 0);
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= g.nbx || ny >= g.nby || nz >= g.nbz) continue;
+        const int ns = g.slot[(nx * g.nby + ny) * g.nbz + nz];
+        if (ns >= 0) active_next[ns] = 1;
+    }
 }
 
 // coarse face = mean of the 2x2 fine faces it covers (mr_multigrid.py:52-89)
@@ -239,12 +416,18 @@ __global__ __launch_bounds__(256) void k_vec_part(int op, const float* __restric
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
+// one wave combines the partials: lane t takes t, t + 64, ... in order,
+// then a fixed butterfly
 __global__ __launch_bounds__(64) void k_vec_comb(int op, const double* __restrict__ part, int np,
                                                  double* __restrict__ out) {
-    if (threadIdx.x != 0) return;
     double r = 0.0;
-    for (int b = 0; b < np; ++b) r = op == 2 ? fmax(r, part[b]) : r + part[b];
-    *out = r;
+    for (int b = threadIdx.x; b < np; b += 64) r = op == 2 ? fmax(r, part[b]) : r + part[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(r, o, 64);
+        r = op == 2 ? fmax(r, t) : r + t;
+    }
+    if (threadIdx.x == 0) *out = r;
 }
 
 // y = a x + b y (x may be NULL: y = b y + a)
@@ -304,6 +487,66 @@ PF_API int pf_msbg_rbgs(const pf_msbg_grid* g, const float* pin, float* pout, co
     PF_REQUIRE(pin != pout, "pf_msbg_rbgs: pin and pout must be distinct");
     PF_REQUIRE(colour == 0 || colour == 1, "pf_msbg_rbgs: colour must be 0 or 1");
     k_msbg_rbgs<<<cdiv(g->ncells, 256), 256, 0, S(stream)>>>(mgrid(*g), pin, pout, b, bface, diag, colour, omega);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_msbg_rbgs_active(const pf_msbg_grid* g, const float* pin, float* pout, const float* b,
+                               const float* bface, const float* diag, const int32_t* active, int32_t colour,
+                               float omega, float* delta, pf_stream_t stream) {
+    int rc = check_grid(g, "pf_msbg_rbgs_active");
+    if (rc) return rc;
+    PF_REQUIRE(pin && pout && b && bface && diag && active && delta, "pf_msbg_rbgs_active: null vector");
+    PF_REQUIRE(pin != pout, "pf_msbg_rbgs_active: pin and pout must be distinct");
+    PF_REQUIRE(colour == 0 || colour == 1, "pf_msbg_rbgs_active: colour must be 0 or 1");
+    k_msbg_rbgs_active<<<cdiv(g->ncells, 256), 256, 0, S(stream)>>>(mgrid(*g), pin, pout, b, bface, diag, active,
+                                                                    colour, omega, delta);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_msbg_relax_update(const pf_msbg_grid* g, const float* delta, const float* diag, const int32_t* active,
+                                float* res, float* theta, int32_t set_theta, float theta_sigma, int32_t* active_next,
+                                pf_stream_t stream) {
+    int rc = check_grid(g, "pf_msbg_relax_update");
+    if (rc) return rc;
+    PF_REQUIRE(delta && diag && active && res && theta && active_next, "pf_msbg_relax_update: null pointer");
+    PF_REQUIRE(active_next != active, "pf_msbg_relax_update: active_next must be a separate buffer");
+    k_msbg_block_res<<<g->nslots, 256, 0, S(stream)>>>(mgrid(*g), delta, diag, active, res);
+    PF_CHECK_LAUNCH();
+    if (set_theta) {
+        k_msbg_theta<<<1, 256, 0, S(stream)>>>(res, g->nslots, theta_sigma, theta);
+        PF_CHECK_LAUNCH();
+    }
+    rc = fill_u32(active_next, g->nslots, 0u, S(stream));
+    if (rc) return rc;
+    k_msbg_update_active<<<cdiv(g->nslots, 256), 256, 0, S(stream)>>>(mgrid(*g), res, theta, active_next);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int32_t pf_msbg_coarsest_max_cells(void) {
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    int n = (int)(lim / coarsest_lds_bytes(1));
+    return n < 65536 ? n : 65535;  // neighbour indices are 16-bit
+}
+
+PF_API int pf_msbg_coarsest_smooth(const pf_msbg_grid* g, float* p, const float* b, const float* bface,
+                                   const float* diag, int32_t sweeps, float omega, pf_stream_t stream) {
+    int rc = check_grid(g, "pf_msbg_coarsest_smooth");
+    if (rc) return rc;
+    PF_REQUIRE(p && b && bface && diag && sweeps >= 0, "pf_msbg_coarsest_smooth: bad arguments");
+    PF_REQUIRE(g->ncells <= pf_msbg_coarsest_max_cells(),
+               "pf_msbg_coarsest_smooth: %d cells exceed the workgroup's LDS", g->ncells);
+    const size_t bytes = coarsest_lds_bytes(g->ncells);
+    if (bytes > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void*)k_msbg_coarsest,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        PF_REQUIRE(e == hipSuccess, "pf_msbg_coarsest_smooth: cannot raise the LDS limit to %zu bytes", bytes);
+    }
+    k_msbg_coarsest<<<1, kCoarsestThreads, bytes, S(stream)>>>(mgrid(*g), p, b, bface, diag, sweeps, omega);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }

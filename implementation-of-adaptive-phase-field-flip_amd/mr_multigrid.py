@@ -54,9 +54,8 @@ class MRMultigrid:
 
     def __init__(self, grid: MSBG, device=None, *, omega=6.0 / 7.0, pre=3, post=3,
                  coarsest_sweeps=80, min_cells=512, smoother="rbgs", n_iter_max=12, theta_sigma=2.0):
-        if smoother != "rbgs":
-            raise NotImplementedError("only the red-black smoother is built (the adaptive relaxation of "
-                                      "pfflip/adaptive_relax.py is not)")
+        if smoother not in ("rbgs", "adaptive"):
+            raise ValueError(f"smoother {smoother!r}: 'rbgs' or 'adaptive'")
         self.omega, self.pre, self.post = float(omega), int(pre), int(post)
         self.coarsest_sweeps = int(coarsest_sweeps)
         self._work = torch.zeros(int(_lib.load().pf_vec_reduce_bytes()) // 8, dtype=torch.float64,
@@ -76,6 +75,15 @@ class MRMultigrid:
                 break
             L += 1
         self.nlev = len(self.levels)
+        # Algorithm 2 on every level but the coarsest (mr_multigrid.py:170-178)
+        self.adaptive = None
+        if smoother == "adaptive":
+            from .adaptive_relax import AdaptiveSmoother
+            self.adaptive = [AdaptiveSmoother(lv.grid, omega=omega, n_iter_max=n_iter_max, theta_sigma=theta_sigma)
+                             for lv in self.levels[:-1]]
+        self._record = False
+        # the coarsest sweeps run in one workgroup from LDS when they fit
+        self.coarsest_in_lds = self.levels[-1].n <= int(_lib.load().pf_msbg_coarsest_max_cells())
 
     def build_hierarchy(self, bface0):
         """Level-0 coefficients, then coarsen down the hierarchy (mr_multigrid.py:187-198)."""
@@ -88,7 +96,12 @@ class MRMultigrid:
                           _lib.ptr(lv.bface), _lib.stream())
             mro.compute_diag(lv.grid, lv.bface, lv.diag)
 
-    def _smooth(self, lv: _MGLevel, n_sweeps, reverse=False):
+    def _smooth(self, lv: _MGLevel, n_sweeps, reverse=False, index=None):
+        """n_sweeps red-black sweeps, or the adaptive relaxation loop on a
+        non-coarsest level when the smoother is adaptive (mr_multigrid.py:200-211)."""
+        if self.adaptive is not None and index is not None and index < self.nlev - 1:
+            self.adaptive[index].smooth(lv, record=self._record)
+            return
         colours = (1, 0) if reverse else (0, 1)
         for _ in range(n_sweeps):
             for c in colours:
@@ -99,10 +112,14 @@ class MRMultigrid:
         """mr_multigrid.py:213-235."""
         lv = self.levels[i]
         if i == self.nlev - 1:
-            self._smooth(lv, self.coarsest_sweeps)
+            if self.coarsest_in_lds:
+                _lib.call("pf_msbg_coarsest_smooth", lv.grid.ref(), _lib.ptr(lv.p), _lib.ptr(lv.b),
+                          _lib.ptr(lv.bface), _lib.ptr(lv.diag), self.coarsest_sweeps, self.omega, _lib.stream())
+            else:
+                self._smooth(lv, self.coarsest_sweeps)
             remove_mean(lv.p, self._work)
             return
-        self._smooth(lv, self.pre)
+        self._smooth(lv, self.pre, index=i)
         mro.residual(lv.grid, lv.p, lv.b, lv.bface, lv.r)
         c = self.levels[i + 1]
         c.p.zero_()
@@ -110,7 +127,7 @@ class MRMultigrid:
         self._vcycle(i + 1)
         _lib.call("pf_msbg_prolong_add", lv.grid.ref(), c.grid.ref(), _lib.ptr(c.p), _lib.ptr(lv.p),
                   _lib.stream())
-        self._smooth(lv, self.post, reverse=True)
+        self._smooth(lv, self.post, reverse=True, index=i)
 
     def precondition(self, r_in, z_out):
         """z = V-cycle(r), mean removed (mr_multigrid.py:237-243)."""

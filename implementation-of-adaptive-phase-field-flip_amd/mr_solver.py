@@ -2,8 +2,9 @@
 
 Host-driven like the reference (one device-to-host read per reduction);
 the vector kernels and reductions (FP64, fixed combination order) run in
-``pf_msbg.hip``.  Preconditioners: the multigrid V-cycle (``"mg"``) or
-Jacobi (``"jacobi"``).
+``pf_msbg.hip``.  Preconditioners: the multigrid V-cycle (``"mg"``), the
+same with the adaptive relaxation smoother (``"mg-adaptive"``) or Jacobi
+(``"jacobi"``).
 """
 
 from __future__ import annotations
@@ -17,12 +18,13 @@ from .mr_multigrid import MRMultigrid, axpby, reduce, remove_mean
 
 class MRSolver:
     def __init__(self, grid, device=None, preconditioner="mg", **mg_kwargs):
-        if preconditioner not in ("mg", "jacobi"):
-            raise NotImplementedError(f"preconditioner {preconditioner!r} (mg-adaptive needs the adaptive "
-                                      "relaxation, which is not built)")
+        if preconditioner not in ("mg", "mg-adaptive", "jacobi"):
+            raise ValueError(f"preconditioner {preconditioner!r}: 'mg', 'mg-adaptive' or 'jacobi'")
         self.grid = grid
         self.kind = preconditioner
-        self.mg = MRMultigrid(grid, **mg_kwargs) if preconditioner == "mg" else None
+        if preconditioner == "mg-adaptive":  # mr_solver.py:77-82
+            mg_kwargs.setdefault("smoother", "adaptive")
+        self.mg = MRMultigrid(grid, **mg_kwargs) if preconditioner != "jacobi" else None
         n = grid.ncells
         self.n = n
         z = lambda: torch.zeros(n, dtype=torch.float32, device=grid.dev)  # noqa: E731

@@ -375,6 +375,28 @@ int pf_msbg_residual(const pf_msbg_grid* g, const float* p, const float* b, cons
  * the others copied; pout != pin. */
 int pf_msbg_rbgs(const pf_msbg_grid* g, const float* pin, float* pout, const float* b, const float* bface,
                  const float* diag, int32_t colour, float omega, pf_stream_t stream);
+/* Adaptive relaxation (adaptive_relax.py:84-160).  pf_msbg_rbgs_active:
+ * rbgs_sweep_active, one colour over the blocks with active[slot] != 0 from
+ * pin into pout (all cells written), delta[c] = |update| of the updated
+ * cells (the caller zeroes delta before each sweep).  pf_msbg_relax_update:
+ * block_effective_residual (res[slot] = max delta diag, 0 if inactive);
+ * with set_theta, theta[0] = exp(mean + theta_sigma std) of log res over
+ * the positive entries (+inf if none); update_active (active_next = blocks
+ * with res > theta and their face neighbours). */
+int pf_msbg_rbgs_active(const pf_msbg_grid* g, const float* pin, float* pout, const float* b, const float* bface,
+                        const float* diag, const int32_t* active, int32_t colour, float omega, float* delta,
+                        pf_stream_t stream);
+int pf_msbg_relax_update(const pf_msbg_grid* g, const float* delta, const float* diag, const int32_t* active,
+                         float* res, float* theta, int32_t set_theta, float theta_sigma, int32_t* active_next,
+                         pf_stream_t stream);
+/* The coarsest level's smoothing of _vcycle (mr_multigrid.py:216-219): all
+ * `sweeps` red-black sweeps in one workgroup from LDS, equal bit for bit to
+ * as many pf_msbg_rbgs pairs; p is read and overwritten.  At most
+ * pf_msbg_coarsest_max_cells() cells (the current device's LDS per
+ * workgroup). */
+int32_t pf_msbg_coarsest_max_cells(void);
+int pf_msbg_coarsest_smooth(const pf_msbg_grid* g, float* p, const float* b, const float* bface, const float* diag,
+                            int32_t sweeps, float omega, pf_stream_t stream);
 /* coarsen_beta (mr_multigrid.py:52-89): coarse face = mean of the 2x2 fine
  * faces it covers. */
 int pf_msbg_coarsen_beta(const pf_msbg_grid* coarse, const pf_msbg_grid* fine, const float* bfine, float* bcoarse,

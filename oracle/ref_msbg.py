@@ -22,7 +22,11 @@ is reproduced exactly across 2:1 jumps while x^2 and x^3 leave the
 h-independent per-volume truncation errors 0.5 and about 0.8 the reference
 tests derive; a manufactured cosine solution is recovered; restriction is the
 transpose of prolongation; the multigrid-preconditioned solver reaches the
-tolerance in far fewer iterations than Jacobi.
+tolerance in far fewer iterations than Jacobi; the adaptive smoother reduces
+to the plain one with every block kept active (theta below every residual)
+and to warm-up plus one sweep with none kept (theta above), re-activates
+exactly the face neighbours of a block above theta, and as a preconditioner
+reaches the plain V-cycle's solution.
 """
 
 from __future__ import annotations
@@ -228,11 +232,80 @@ def prolongation(fg, cg):
     return sp.csr_matrix((np.ones(fg.ncells), (rows, cols)), shape=(fg.ncells, cg.ncells))
 
 
-class Multigrid:
-    """mr_multigrid.py:139-246 with the red-black smoother."""
+def block_neighbours(grid):
+    """Face-neighbour slots of every active block (adaptive_relax.py:146-160)."""
+    out = []
+    for s in range(len(grid.blk)):
+        b = int(grid.blk[s])
+        bz = b % grid.nb[2]
+        by = (b // grid.nb[2]) % grid.nb[1]
+        bx = b // (grid.nb[2] * grid.nb[1])
+        nbrs = []
+        for ax in range(3):
+            for sg in (-1, 1):
+                q = [bx, by, bz]
+                q[ax] += sg
+                if all(0 <= q[c] < grid.nb[c] for c in range(3)):
+                    ns = grid.slot[(q[0] * grid.nb[1] + q[1]) * grid.nb[2] + q[2]]
+                    if ns >= 0:
+                        nbrs.append(int(ns))
+        out.append(nbrs)
+    return out
 
-    def __init__(self, grid, omega=6.0 / 7.0, pre=3, post=3, coarsest_sweeps=80, min_cells=512):
+
+def adaptive_smooth(grid, A, diag, col, p, b, omega, n_iter_max=12, theta_sigma=2.0, warmup=2, inner=1,
+                    nbrs=None):
+    """Algorithm 2 as adaptive_relax.py:163-241 runs it: warm-up sweeps over
+    all blocks, then sweeps over the active set; per-block effective residual
+    max |delta| diag; theta from the log-histogram of the first iteration's
+    positive residuals; blocks above theta and their face neighbours stay
+    active.  -> (p, theta, active fractions)."""
+    nbrs = block_neighbours(grid) if nbrs is None else nbrs
+    ns = len(grid.blk)
+    active = np.ones(ns, bool)
+
+    def sweep(p, reverse):
+        delta = np.zeros(grid.ncells)
+        for _ in range(inner):
+            for c in ((1, 0) if reverse else (0, 1)):
+                upd = (col == c) & (diag > 0) & active[grid.cslot]
+                r = b - A @ p
+                step = omega * r / np.where(diag > 0, diag, 1.0)
+                out = p.copy()
+                out[upd] = p[upd] + step[upd]
+                delta[upd] = np.abs(step[upd])
+                p = out
+        return p, delta
+
+    for w in range(warmup):
+        p, _ = sweep(p, w % 2 == 1)
+    theta, frac = None, []
+    for it in range(n_iter_max):
+        frac.append(active.mean())
+        p, delta = sweep(p, it % 2 == 1)
+        res = np.zeros(ns)
+        np.maximum.at(res, grid.cslot, np.where(active[grid.cslot], delta * diag, 0.0))
+        if theta is None:
+            pos = res[res > 0]
+            if pos.size == 0:
+                break
+            lg = np.log(pos)
+            theta = float(np.exp(lg.mean() + theta_sigma * lg.std()))
+        nxt = np.zeros(ns, bool)
+        for s_ in np.flatnonzero(res > theta):
+            nxt[s_] = True
+            nxt[nbrs[s_]] = True
+        active = nxt
+    return p, theta, frac
+
+
+class Multigrid:
+    """mr_multigrid.py:139-246 with the red-black or the adaptive smoother."""
+
+    def __init__(self, grid, omega=6.0 / 7.0, pre=3, post=3, coarsest_sweeps=80, min_cells=512, smoother="rbgs",
+                 n_iter_max=12, theta_sigma=2.0):
         self.omega, self.pre, self.post, self.coarsest = omega, pre, post, coarsest_sweeps
+        self.smoother, self.n_iter_max, self.theta_sigma = smoother, n_iter_max, theta_sigma
         self.grids = [grid]
         base = grid.level
         cap = grid.sh0
@@ -248,6 +321,7 @@ class Multigrid:
             L += 1
         self.P = [prolongation(self.grids[i], self.grids[i + 1]) for i in range(len(self.grids) - 1)]
         self.col = [colours(g) for g in self.grids]
+        self.nbrs = [block_neighbours(g) for g in self.grids] if smoother == "adaptive" else None
 
     def build_hierarchy(self, bface0):
         """mr_multigrid.py:187-198."""
@@ -258,6 +332,9 @@ class Multigrid:
         self.diag = [np.asarray(A.diagonal()) for A in self.A]
 
     def _smooth(self, i, p, b, n, reverse=False):
+        if self.smoother == "adaptive" and i < len(self.grids) - 1:
+            return adaptive_smooth(self.grids[i], self.A[i], self.diag[i], self.col[i], p, b, self.omega,
+                                   self.n_iter_max, self.theta_sigma, nbrs=self.nbrs[i])[0]
         for _ in range(n):
             for c in ((1, 0) if reverse else (0, 1)):
                 p = rbgs_half_sweep(self.A[i], self.diag[i], self.col[i], p, b, c, self.omega)
@@ -285,7 +362,9 @@ def solve(grid, bface, rhs, tol=1e-4, max_iter=2000, preconditioner="mg", **mg_k
     """mr_solver.py:118-165: flexible Polak-Ribiere PCG -> (p, it, rel)."""
     A = assemble(grid, bface)
     diag = np.asarray(A.diagonal())
-    if preconditioner == "mg":
+    if preconditioner in ("mg", "mg-adaptive"):
+        if preconditioner == "mg-adaptive":
+            mg_kwargs.setdefault("smoother", "adaptive")
         mg = Multigrid(grid, **mg_kwargs)
         mg.build_hierarchy(bface)
         pre = mg.precondition

@@ -195,3 +195,124 @@ def test_solver_on_the_manufactured_problem(mods, precond):
     sol = np_(p).astype(np.float64)
     sol -= (sol * V).sum() / V.sum()
     assert np.abs(sol - ref).max() / np.abs(ref).max() < 0.25
+
+
+@pytest.mark.parametrize("nb,n0,pattern", [((2, 2, 2), 4, "slab"), ((3, 3, 3), 2, "checker")])
+def test_coarsest_lds_smoother_equals_launched_sweeps(mods, nb, n0, pattern):
+    """The one-workgroup coarsest smoother gives the launched half-sweeps'
+    bits, on graded grids with 2:1 jumps (288 and 125 cells), and so does a
+    whole V-cycle."""
+    say(f"coarsest LDS smoother {pattern}")
+    g, _ = pair(mods, nb, n0, 1.0 / (nb[0] * n0), 1, pattern)
+    assert g.ncells <= int(mods["_lib"].load().pf_msbg_coarsest_max_cells())
+    mg = mods["mr_multigrid"].MRMultigrid(g, min_cells=10 ** 6)  # one level: the grid itself
+    rng = np.random.default_rng(4)
+    mg.build_hierarchy(t32(rng.uniform(0.001, 1.0, g.ncells * 3)))
+    lv = mg.levels[0]
+    lv.b.copy_(t32(rng.standard_normal(g.ncells)))
+    lv.p.copy_(t32(rng.standard_normal(g.ncells)))
+    ref_p = lv.p.clone()
+    _lib = mods["_lib"]
+    _lib.call("pf_msbg_coarsest_smooth", g.ref(), _lib.ptr(lv.p), _lib.ptr(lv.b), _lib.ptr(lv.bface),
+              _lib.ptr(lv.diag), 7, 6.0 / 7.0, _lib.stream())
+    fused = lv.p.clone()
+    lv.p.copy_(ref_p)
+    mg._smooth(lv, 7)
+    assert torch.equal(fused, lv.p)
+    r = t32(rng.standard_normal(g.ncells))
+    z1, z2 = g.zeros(), g.zeros()
+    mg2 = mods["mr_multigrid"].MRMultigrid(g, min_cells=16)
+    mg2.build_hierarchy(t32(rng.uniform(0.001, 1.0, g.ncells * 3)))
+    assert mg2.coarsest_in_lds
+    mg2.precondition(r, z1)
+    mg2.coarsest_in_lds = False
+    mg2.precondition(r, z2)
+    assert torch.equal(z1, z2)
+
+
+# ---------------------------------------------------------------- adaptive relaxation
+def _adaptive_level(mods, theta_sigma, n_iter_max=5):
+    g, o = pair(mods, (3, 3, 3), 4, 1 / 12, 1, "checker")
+    rng = np.random.default_rng(7)
+    bface = rng.uniform(0.001, 1.0, g.ncells * 3).astype(np.float32)
+    b = rng.standard_normal(g.ncells).astype(np.float32)
+    mg = mods["mr_multigrid"].MRMultigrid(g, min_cells=16, smoother="adaptive", n_iter_max=n_iter_max,
+                                          theta_sigma=theta_sigma)
+    mg.build_hierarchy(t32(bface))
+    lv = mg.levels[0]
+    lv.b.copy_(t32(b))
+    lv.p.zero_()
+    return g, o, mg, lv, bface, b
+
+
+def test_adaptive_smoother_matches_oracle(mods):
+    say("adaptive smoother vs oracle")
+    g, o, mg, lv, bface, b = _adaptive_level(mods, 2.0)
+    sm = mg.adaptive[0]
+    sm.smooth(lv, record=True)
+    A = M.assemble(o, bface.astype(np.float64))
+    ref, theta, frac = M.adaptive_smooth(o, A, np.asarray(A.diagonal()), M.colours(o), np.zeros(g.ncells),
+                                         b.astype(np.float64), 6.0 / 7.0, n_iter_max=5, theta_sigma=2.0)
+    assert sm.theta.item() == pytest.approx(theta, rel=1e-5)
+    assert sm.active_fraction == pytest.approx(frac)
+    assert min(frac) < 1.0  # blocks were deactivated
+    assert np.abs(np_(lv.p) - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+def test_adaptive_smoother_with_all_blocks_active_is_the_plain_one(mods):
+    """theta below every residual: the active-set half-sweeps equal the plain
+    ones bit for bit (warm-up + n sweeps, colour order alternating)."""
+    say("adaptive smoother, all active")
+    g, o, mg, lv, bface, b = _adaptive_level(mods, -1e3, n_iter_max=4)
+    mg.adaptive[0].smooth(lv, record=True)
+    assert mg.adaptive[0].active_fraction == [1.0] * 4
+    out = lv.p.clone()
+    lv.p.zero_()
+    for rev in [False, True, False, True, False, True]:
+        mg._smooth(lv, 1, reverse=rev)
+    assert torch.equal(out, lv.p)
+
+
+def test_adaptive_vcycle_matches_oracle(mods):
+    say("adaptive V-cycle vs oracle")
+    g, o = pair(mods, (4, 4, 4), 4, 1 / 16, 1, "slab")
+    rng = np.random.default_rng(8)
+    bface = rng.uniform(0.5, 2.0, g.ncells * 3).astype(np.float32)
+    r = rng.standard_normal(g.ncells).astype(np.float32)
+    r -= r.mean()
+    mg = mods["mr_multigrid"].MRMultigrid(g, min_cells=64, smoother="adaptive", n_iter_max=6)
+    mg.build_hierarchy(t32(bface))
+    z = g.zeros()
+    mg.precondition(t32(r), z)
+    om = M.Multigrid(o, min_cells=64, smoother="adaptive", n_iter_max=6)
+    om.build_hierarchy(bface.astype(np.float64))
+    zr = om.precondition(r.astype(np.float64))
+    assert np.abs(np_(z) - zr).max() < 1e-4 * np.abs(zr).max()
+
+
+def test_adaptive_smoother_is_correct_if_not_faster(mods):
+    """tests/test_mr_sim.py:265-309: mg and mg-adaptive reach the same
+    solution at a 1000:1 coefficient jump."""
+    say("mg vs mg-adaptive")
+    nb, n0 = 4, 16
+    g = mods["msbg"].MSBG((nb,) * 3, n0=n0, h0=1.0 / (nb * n0), max_level=1)
+    bx, _, _ = np.meshgrid(*[np.arange(nb)] * 3, indexing="ij")
+    g.build(np.where(bx < nb // 2, 0, 1).ravel().astype(np.int32))
+    c, v, _ = g.geometry()
+    x, V = np_(c).astype(np.float64), np_(v).astype(np.float64)
+    bf = np.ones(g.ncells * 3, np.float32)
+    bf[np.repeat(x[:, 1] < 0.5, 3)] = 1e-3
+    k = 2 * np.pi
+    b = 3 * k * k * np.cos(k * x[:, 0]) * np.cos(k * x[:, 1]) * np.cos(k * x[:, 2]) * V
+    b -= b.mean()
+    sols = {}
+    for kind in ("mg", "mg-adaptive"):
+        p = g.zeros()
+        kw = {"n_iter_max": 6} if kind == "mg-adaptive" else {}
+        it, rel = mods["mr_solver"].MRSolver(g, preconditioner=kind, **kw).solve(t32(bf), t32(b), p, tol=1e-4,
+                                                                                   max_iter=300)
+        assert rel < 1e-4, (kind, it, rel)
+        s = np_(p).astype(np.float64)
+        sols[kind] = s - s.mean()
+    diff = np.abs(sols["mg"] - sols["mg-adaptive"]).max() / np.abs(sols["mg"]).max()
+    assert diff < 2e-3, diff

@@ -205,3 +205,66 @@ def test_multigrid_solver_beats_jacobi_on_the_manufactured_problem():
     for x in (x_mg, x_j):
         x = x - (x * V).sum() / V.sum()
         assert np.abs(x - ref).max() / np.abs(ref).max() < 0.25
+
+
+# ---------------------------------------------------------------- adaptive relaxation
+def _adaptive_case():
+    g = make((3, 3, 3), 4, 1 / 12, 1, "checker")
+    rng = np.random.default_rng(7)
+    A = M.assemble(g, rng.uniform(0.001, 1.0, g.ncells * 3))
+    diag = np.asarray(A.diagonal())
+    col = M.colours(g)
+    b = rng.standard_normal(g.ncells)
+    return g, A, diag, col, b
+
+
+def _plain(A, diag, col, p, b, omega, orders):
+    for rev in orders:
+        for c in ((1, 0) if rev else (0, 1)):
+            p = M.rbgs_half_sweep(A, diag, col, p, b, c, omega)
+    return p
+
+
+def test_adaptive_smoother_reduces_to_the_plain_one():
+    """theta below every residual keeps every block active: warm-up + n
+    sweeps of the plain smoother with the colour order alternating; theta
+    above every residual leaves warm-up + 1 sweep."""
+    g, A, diag, col, b = _adaptive_case()
+    p0 = np.zeros(g.ncells)
+    n = 5
+    all_on, theta, frac = M.adaptive_smooth(g, A, diag, col, p0, b, 0.8, n_iter_max=n, theta_sigma=-1e3)
+    assert theta < 1e-100 and all(f == 1.0 for f in frac)
+    orders = [w % 2 == 1 for w in range(2)] + [it % 2 == 1 for it in range(n)]
+    assert np.allclose(all_on, _plain(A, diag, col, p0, b, 0.8, orders), rtol=0, atol=1e-14)
+    none_on, theta, frac = M.adaptive_smooth(g, A, diag, col, p0, b, 0.8, n_iter_max=n, theta_sigma=50.0)
+    assert frac[1:] == [0.0] * (n - 1)
+    assert np.array_equal(none_on, _plain(A, diag, col, p0, b, 0.8, orders[:3]))
+
+
+def test_adaptive_reactivation_is_blocks_above_theta_and_their_face_neighbours():
+    g = make((3, 3, 3), 2, 1 / 6, 0, levels=np.zeros(27))
+    nbrs = M.block_neighbours(g)
+    centre = g.slot[(1 * 3 + 1) * 3 + 1]
+    assert sorted(nbrs[centre]) == sorted(g.slot[[4, 10, 12, 14, 16, 22]].tolist())
+    corner = g.slot[0]
+    assert sorted(nbrs[corner]) == sorted(g.slot[[1, 3, 9]].tolist())
+
+
+def test_adaptive_preconditioner_reaches_the_plain_solution():
+    """tests/test_mr_sim.py:265-309 on the restatement: mg and mg-adaptive
+    converge to the same solution at a 1000:1 coefficient jump."""
+    nb, n0 = (2, 2, 2), 8
+    g = make(nb, n0, 1.0 / 16, 1, "slab")
+    x, V = g.geometry()
+    bf = np.ones(g.ncells * 3)
+    bf[np.repeat(x[:, 1] < 0.5, 3)] = 1e-3
+    k = 2 * math.pi
+    b = 3 * k * k * np.cos(k * x[:, 0]) * np.cos(k * x[:, 1]) * np.cos(k * x[:, 2]) * V
+    b -= b.mean()
+    sols = {}
+    for kind in ("mg", "mg-adaptive"):
+        kw = {"n_iter_max": 6} if kind == "mg-adaptive" else {}
+        p, it, rel = M.solve(g, bf, b, tol=1e-6, max_iter=300, preconditioner=kind, min_cells=64, **kw)
+        assert rel < 1e-6, (kind, it, rel)
+        sols[kind] = p - p.mean()
+    assert np.abs(sols["mg"] - sols["mg-adaptive"]).max() / np.abs(sols["mg"]).max() < 1e-4
