@@ -41,8 +41,8 @@ class Config:
     mg_min_dim: int = 4
     # ---- velocity extrapolation (config.py:77)
     extrap_cells: int = 3
-    # ---- escaped particles / spray (config.py:82-110).  The droplet stage is
-    # not part of this implementation; see Simulator.step.
+    # ---- escaped particles / spray (config.py:82-110): the droplet stage of
+    # Simulator.step (csrc/pf_droplets.hip)
     droplets: bool = True
     delta_phi: float = 0.2
     droplet_radius_mean: float = 0.25
