@@ -409,6 +409,77 @@ def run_surge256(pf, torch, args):
                              "adjoint_pcg": list(adj.last_solve), "grad_norm": grad_norm}}
 
 
+def msbg_sweep_bytes(n_cells):
+    """Algorithmic bytes of one MSBG red-black half-sweep: p read and written,
+    b, the diagonal, the three owned face coefficients and the cell's block
+    slot read once per cell (32 B/cell); block tables are a few KB."""
+    return 32 * n_cells
+
+
+def run_adaptive128(pf, torch, args):
+    """The adaptive path: a two-phase dam break (y up, liquid x < 0.4,
+    y < 0.6) on a 128^3 MSBG (8^3 blocks of 16^3 cells, levels 0-2), 8 ppc
+    seeded at level 0, alpha_phi 0.25 (the measured-safe noise margin at 8
+    ppc), beta_bw 0.5, MG-PCG to the reference's 1e-4; particles refine and
+    coarsen every step.  Warm-up steps let the resampling settle."""
+    n = 128
+    cfg = pf.Config(res=(n, n, n), dx=1.0 / n, ppc=8, alpha_phi=0.25, droplets=False, pcg_tol=1e-4,
+                    pcg_max_iter=400)
+    sim = pf.AdaptiveSimulator(cfg, (8, 8, 8), n0=16, max_level=2, beta_bw=0.5)
+    pos, vel, pt = pf.scenes.dam_break(cfg)
+    sim.set_particles(pos, vel, pt)
+    n0 = len(pos)
+    del pos, vel, pt
+    for w in range(args.warmup):
+        r = sim.step()
+        print(f"[bench] warmup {w}: particles {r['particles']} cells {r['cells']} cg {r['cg_iter']}",
+              file=sys.stderr, flush=True)
+    recs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        recs.append(sim.step())
+        print(f"[bench] step {k}: particles {recs[-1]['particles']} cells {recs[-1]['cells']} "
+              f"cg {recs[-1]['cg_iter']} wall {recs[-1]['wall_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # particles entering each timed step (the last record holds the count after it)
+    n_in = [sim.stats[-args.steps - 1]["particles"] if len(sim.stats) > args.steps else n0] + \
+        [r["particles"] for r in recs[:-1]]
+    # dominant kernel: the level-0 half-sweep of the MSBG V-cycle, timed with
+    # HIP events on the stream the kernels run on
+    lib = pf._lib
+    lv = sim.solver.mg.levels[0]
+    g = lv.grid
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for i in range(reps):
+        lib.call("pf_msbg_rbgs", g.ref(), lib.ptr(lv.p), lib.ptr(lv.p2), lib.ptr(lv.b), lib.ptr(lv.bface),
+                 lib.ptr(lv.diag), i & 1, 6.0 / 7.0, lib.stream())
+    e1.record()
+    torch.cuda.synchronize()
+    sweep_ms = e0.elapsed_time(e1) / reps
+    rl = roofline(f"k_msbg_rbgs level-0 half-sweep ({g.ncells} cells)", msbg_sweep_bytes(g.ncells), sweep_ms)
+    return {
+        "metric": "adaptive PF-FLIP step particle-steps/s, 128^3 MSBG (levels 0-2)",
+        "value": sum(n_in) / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded jittered lattice, both phases)",
+        "config": {"workload": "adaptive dam break y-up 128^3 MSBG n0=16 levels 0-2, 8 ppc, 1000:1, "
+                               "MG-PCG tol 1e-4", "particles_seeded": n0,
+                   "particles": [r["particles"] for r in recs], "cells": [r["cells"] for r in recs],
+                   "compression": recs[-1]["compression"], "levels": recs[-1]["levels"],
+                   "parallelism": "single GPU"},
+        "pcg_iters": [r["cg_iter"] for r in recs],
+        "roofline": rl,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,9 +491,10 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-solve-compare", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
-    ap.add_argument("--workload", default="dambreak128", choices=["dambreak128", "rt128", "surge256"],
-                    help="dambreak128: the headline benchmark (default); rt128 / surge256: the "
-                         "Rayleigh-Taylor and large surge configurations (their own JSON line)")
+    ap.add_argument("--workload", default="dambreak128",
+                    choices=["dambreak128", "rt128", "surge256", "adaptive128"],
+                    help="dambreak128: the headline benchmark (default); rt128 / surge256 / adaptive128: the "
+                         "Rayleigh-Taylor, large surge and adaptive (MSBG) configurations (their own JSON line)")
     ap.add_argument("--rt-steps", type=int, default=60)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy three times (the known-size "
@@ -452,6 +524,9 @@ def main():
         return
     if args.workload == "surge256":
         print(json.dumps(run_surge256(pf, torch, args)), flush=True)
+        return
+    if args.workload == "adaptive128":
+        print(json.dumps(run_adaptive128(pf, torch, args)), flush=True)
         return
 
     n = args.res

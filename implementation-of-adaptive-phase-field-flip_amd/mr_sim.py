@@ -137,6 +137,9 @@ class AdaptiveSimulator:
             self._level_sig = None
         if sig != self._level_sig:
             self._level_sig = sig
+            # release the old hierarchy first: its memory then serves the new
+            # one from the allocator's cache instead of fresh device mappings
+            self.solver = None
             self.solver = MRSolver(self.grid, preconditioner="mg")
         f = _lib.MrFields()
         f.acc = self._acc.data_ptr()
