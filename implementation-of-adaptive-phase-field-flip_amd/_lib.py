@@ -177,6 +177,9 @@ SIGNATURES = {
     "pf_mr_max_speed": (ctypes.c_int, [P, I64, P, P]),
     "pf_mr_p2g": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParticles), ctypes.POINTER(MrParams),
                                  ctypes.POINTER(MrFields), P]),
+    "pf_mr_p2g_scratch_ints": (I64, [I64, I32]),
+    "pf_mr_p2g_sorted": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParticles),
+                                        ctypes.POINTER(MrParams), ctypes.POINTER(MrFields), P, P]),
     "pf_mr_finish": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParams), ctypes.POINTER(MrFields), P]),
     "pf_mr_face_prepare": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParams),
                                           ctypes.POINTER(MrFields), P]),

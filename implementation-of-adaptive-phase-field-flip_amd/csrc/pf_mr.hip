@@ -72,10 +72,6 @@ __device__ inline float phase_field(float rho, float rho_min, float alpha_phi, f
     return fminf(sqrtf(fmaxf(rho - rho_min, 0.0f) / (alpha_phi * rho_0l)), 1.0f);
 }
 
-__device__ inline void acc_add(unsigned long long* p, float v, float scale) {
-    atomicAdd(p, (unsigned long long)(long long)__float2ll_rn(v * scale));
-}
-
 // ------------------------------------------------------------------ 1. refinement map
 // M[block] = min(l_p, M[block]) over the blocks a particle's box of half
 // width cfl_eps touches (particles.py:49-85).  Neighbouring particles touch
@@ -142,12 +138,14 @@ __global__ __launch_bounds__(256) void k_mr_max_speed(const float* __restrict__ 
 // Mass (eq. 3) and momentum (eq. 4) on the low faces, mass and the bare
 // weight sum at the centres, over the cells of every block the support
 // sphere touches (mr_kernels.py:48-133).  m_p = rho 8^l_p, r_p = r0 2^l_p
-// (particles.py:234-248).
-__global__ __launch_bounds__(256) void k_mr_p2g(MGrid g, const float* __restrict__ x, const float* __restrict__ v,
-                                                const int* __restrict__ type, const int* __restrict__ lev, int64_t n,
-                                                MRConst c, unsigned long long* __restrict__ acc) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+// (particles.py:234-248).  `add(channel, cell, value)` receives the
+// fixed-point value of every contribution.
+__device__ inline long long fixed(float v, float scale) { return __float2ll_rn(v * scale); }
+
+template <class Add>
+__device__ inline void splat_particle(const MGrid& g, const float* __restrict__ x, const float* __restrict__ v,
+                                      const int* __restrict__ type, const int* __restrict__ lev, int64_t p,
+                                      const MRConst& c, Add&& add) {
     const float px = x[3 * p], py = x[3 * p + 1], pz = x[3 * p + 2];
     const float vel[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
     const int lp = lev[p];
@@ -155,7 +153,6 @@ __global__ __launch_bounds__(256) void k_mr_p2g(MGrid g, const float* __restrict
     const float rp = c.r0 * (float)(1 << lp);
     const float r2 = rp * rp, inv_r2 = 1.0f / r2;
     const float bext = (float)g.n0 * g.h0;
-    const int64_t nc = g.ncells;
     const int bi0 = max((int)floorf((px - rp) / bext), 0), bi1 = min((int)floorf((px + rp) / bext), g.nbx - 1);
     const int bj0 = max((int)floorf((py - rp) / bext), 0), bj1 = min((int)floorf((py + rp) / bext), g.nby - 1);
     const int bk0 = max((int)floorf((pz - rp) / bext), 0), bk1 = min((int)floorf((pz + rp) / bext), g.nbz - 1);
@@ -168,12 +165,18 @@ __global__ __launch_bounds__(256) void k_mr_p2g(MGrid g, const float* __restrict
                 const int nn = g.n0 >> l;
                 const float h = cell_size(g, l), half = 0.5f * h;
                 const float ox = (float)bi * bext, oy = (float)bj * bext, oz = (float)bk * bext;
-                const int i0 = max((int)floorf((px - rp - ox) / h) - 1, 0);
-                const int i1 = min((int)floorf((px + rp - ox) / h) + 1, nn - 1);
-                const int j0 = max((int)floorf((py - rp - oy) / h) - 1, 0);
-                const int j1 = min((int)floorf((py + rp - oy) / h) + 1, nn - 1);
-                const int k0 = max((int)floorf((pz - rp - oz) / h) - 1, 0);
-                const int k1 = min((int)floorf((pz + rp - oz) / h) + 1, nn - 1);
+                // cells whose centre ((i + 1/2) h) or low face (i h) lies
+                // within r: i in [(x - r)/h - 1/2, (x + r)/h].  A cell this
+                // range misses by rounding has w < 2^-49 of a full weight,
+                // which is 0 in fixed point (every scale keeps the largest
+                // single contribution below 2^48), so the sums equal those
+                // of the reference's wider walk (index range +-1).
+                const int i0 = max((int)floorf((px - rp - ox) / h - 0.5f), 0);
+                const int i1 = min((int)floorf((px + rp - ox) / h), nn - 1);
+                const int j0 = max((int)floorf((py - rp - oy) / h - 0.5f), 0);
+                const int j1 = min((int)floorf((py + rp - oy) / h), nn - 1);
+                const int k0 = max((int)floorf((pz - rp - oz) / h - 0.5f), 0);
+                const int k1 = min((int)floorf((pz + rp - oz) / h), nn - 1);
                 const int base = g.coff[s];
                 for (int i = i0; i <= i1; ++i) {
                     const float dxc = ox + ((float)i + 0.5f) * h - px, dxf = dxc - half;
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(256) void k_mr_p2g(MGrid g, const float* __restrict
                         for (int k = k0; k <= k1; ++k) {
                             const float dzc = oz + ((float)k + 0.5f) * h - pz, dzf = dzc - half;
                             const float zc2 = dzc * dzc, zf2 = dzf * dzf;
-                            const int64_t addr = base + (i * nn + j) * nn + k;
+                            const int addr = base + (i * nn + j) * nn + k;
                             const float d2[4] = {xc2 + yc2 + zc2, xf2 + yc2 + zc2, xc2 + yf2 + zc2, xc2 + yc2 + zf2};
 #pragma unroll
                             for (int q = 0; q < 4; ++q) {
@@ -195,17 +198,101 @@ __global__ __launch_bounds__(256) void k_mr_p2g(MGrid g, const float* __restrict
                                 if (!(w > 0.0f)) continue;
                                 const float wm = w * mp;
                                 if (q == 0) {
-                                    acc_add(acc + addr, wm, c.mass_scale);
-                                    acc_add(acc + nc + addr, w, c.w_scale);
+                                    add(0, addr, fixed(wm, c.mass_scale));
+                                    add(1, addr, fixed(w, c.w_scale));
                                 } else {
-                                    acc_add(acc + (int64_t)(1 + q) * nc + addr, wm, c.mass_scale);
-                                    acc_add(acc + (int64_t)(4 + q) * nc + addr, wm * vel[q - 1], c.mom_scale);
+                                    add(1 + q, addr, fixed(wm, c.mass_scale));
+                                    add(4 + q, addr, fixed(wm * vel[q - 1], c.mom_scale));
                                 }
                             }
                         }
                     }
                 }
             }
+}
+
+// one thread per particle in index order, every contribution a global atomic
+__global__ __launch_bounds__(256) void k_mr_p2g(MGrid g, const float* __restrict__ x, const float* __restrict__ v,
+                                                const int* __restrict__ type, const int* __restrict__ lev, int64_t n,
+                                                MRConst c, unsigned long long* __restrict__ acc) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int64_t nc = g.ncells;
+    splat_particle(g, x, v, type, lev, p, c, [&](int ch, int addr, long long val) {
+        atomicAdd(acc + ch * nc + addr, (unsigned long long)val);
+    });
+}
+
+// Visit order for the cached P2G: particles sorted by the address of the
+// cell they sit in (key ncells outside the grid); the order inside a cell
+// is whatever the counting atomics give, which the fixed-point sums do not
+// see.  The particle arrays themselves keep their order.
+__global__ __launch_bounds__(256) void k_mr_cell_key(MGrid g, const float* __restrict__ x, int64_t n,
+                                                     int* __restrict__ key, int* __restrict__ rank,
+                                                     int* __restrict__ count) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const MCell cl = locate_at(g, x[3 * p], x[3 * p + 1], x[3 * p + 2]);
+    const int k = cl.addr >= 0 ? cl.addr : g.ncells;
+    key[p] = k;
+    rank[p] = atomicAdd(count + k, 1);
+}
+
+__global__ __launch_bounds__(256) void k_mr_cell_scatter(const int* __restrict__ key, const int* __restrict__ rank,
+                                                         const int* __restrict__ start, int64_t n,
+                                                         int* __restrict__ perm) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    perm[start[key[p]] + rank[p]] = (int)p;
+}
+
+// P2G over the cell-sorted order.  A workgroup's particles are neighbours,
+// so their contributions land on a few hundred cells: each workgroup keeps
+// a hash of kP2GSlots cells in LDS (tag + the 8 fixed-point channels), adds
+// there with LDS atomics, and flushes every cached cell with one global
+// atomic per non-zero channel at the end.  A contribution whose slot holds
+// another cell goes to global memory directly.  Integer sums, so the result
+// equals the uncached kernel's bit for bit.
+constexpr int kP2GSlots = 1024;
+constexpr int kP2GPerBlock = 512;  // particles per workgroup (2 per thread)
+
+__global__ __launch_bounds__(256) void k_mr_p2g_cached(MGrid g, const float* __restrict__ x,
+                                                       const float* __restrict__ v, const int* __restrict__ type,
+                                                       const int* __restrict__ lev, const int* __restrict__ perm,
+                                                       int64_t n, MRConst c, unsigned long long* __restrict__ acc) {
+    __shared__ int tag[kP2GSlots];
+    __shared__ unsigned long long lacc[kP2GSlots * kAcc];
+    for (int s = threadIdx.x; s < kP2GSlots; s += blockDim.x) tag[s] = -1;
+    for (int s = threadIdx.x; s < kP2GSlots * kAcc; s += blockDim.x) lacc[s] = 0ull;
+    __syncthreads();
+    const int64_t nc = g.ncells;
+    const int64_t p0 = (int64_t)blockIdx.x * kP2GPerBlock;
+    for (int q = threadIdx.x; q < kP2GPerBlock; q += blockDim.x) {
+        if (p0 + q >= n) break;
+        const int64_t p = perm[p0 + q];
+        splat_particle(g, x, v, type, lev, p, c, [&](int ch, int addr, long long val) {
+            const int s = (int)(((unsigned)addr * 2654435761u) >> 22);  // 10-bit multiplicative hash
+            int t = tag[s];
+            if (t != addr && t == -1) {
+                t = atomicCAS(tag + s, -1, addr);
+                if (t == -1) t = addr;
+            }
+            if (t == addr)
+                atomicAdd(lacc + s * kAcc + ch, (unsigned long long)val);
+            else
+                atomicAdd(acc + ch * nc + addr, (unsigned long long)val);
+        });
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < kP2GSlots; s += blockDim.x) {
+        const int a = tag[s];
+        if (a < 0) continue;
+#pragma unroll
+        for (int ch = 0; ch < kAcc; ++ch) {
+            const unsigned long long val = lacc[s * kAcc + ch];
+            if (val != 0ull) atomicAdd(acc + ch * nc + a, val);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ 5. cell finish
@@ -673,6 +760,44 @@ PF_API int pf_mr_p2g(const pf_msbg_grid* g, const pf_mr_particles* ps, const pf_
     if (ps->n == 0) return PF_OK;
     k_mr_p2g<<<pgrid(ps->n), 256, 0, S(stream)>>>(mgrid(*g), ps->x, ps->v, ps->type, ps->level, ps->n, mrconst(*prm),
                                                   reinterpret_cast<unsigned long long*>(f->acc));
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int64_t pf_mr_p2g_scratch_ints(int64_t n, int32_t ncells) {
+    // key, rank, perm (n each); count (ncells + 1); start (ncells + 2); scan
+    return 3 * (n > 0 ? n : 1) + 2 * ((int64_t)ncells + 2) + scan_tmp_ints((int64_t)ncells + 1);
+}
+
+PF_API int pf_mr_p2g_sorted(const pf_msbg_grid* g, const pf_mr_particles* ps, const pf_mr_params* prm,
+                            pf_mr_fields* f, int32_t* scratch, pf_stream_t stream) {
+    int rc = check_mgrid(g, "pf_mr_p2g_sorted");
+    if (rc) return rc;
+    rc = check_parts(ps, "pf_mr_p2g_sorted");
+    if (rc) return rc;
+    PF_REQUIRE(prm && f && f->acc && scratch, "pf_mr_p2g_sorted: null params / accumulators / scratch");
+    PF_REQUIRE(prm->mass_scale > 0.0 && prm->mom_scale > 0.0 && prm->w_scale > 0.0, "pf_mr_p2g_sorted: bad scales");
+    const int64_t n = ps->n;
+    if (n == 0) return PF_OK;
+    const int64_t nk = (int64_t)g->ncells + 1;
+    int* key = scratch;
+    int* rank = key + n;
+    int* perm = rank + n;
+    int* count = perm + n;
+    int* start = count + (nk + 1);
+    int* tmp = start + (nk + 1);
+    hipStream_t s = S(stream);
+    rc = fill_u32(count, nk, 0u, s);
+    if (rc) return rc;
+    const MGrid m = mgrid(*g);
+    k_mr_cell_key<<<pgrid(n), 256, 0, s>>>(m, ps->x, n, key, rank, count);
+    PF_CHECK_LAUNCH();
+    rc = exclusive_scan(count, nk, start, tmp, s);
+    if (rc) return rc;
+    k_mr_cell_scatter<<<pgrid(n), 256, 0, s>>>(key, rank, start, n, perm);
+    PF_CHECK_LAUNCH();
+    k_mr_p2g_cached<<<cdiv(n, kP2GPerBlock), 256, 0, s>>>(m, ps->x, ps->v, ps->type, ps->level, perm, n,
+                                                           mrconst(*prm), reinterpret_cast<unsigned long long*>(f->acc));
     PF_CHECK_LAUNCH();
     return PF_OK;
 }

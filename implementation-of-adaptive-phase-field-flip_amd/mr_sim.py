@@ -86,6 +86,10 @@ class AdaptiveSimulator:
         self._edt_tmp = i32(2 * nblk)
         self._dist = torch.zeros(nblk, dtype=torch.float32, device=self.dev)
         self._scalars = torch.zeros(2, dtype=torch.float32, device=self.dev)
+        # P2G over a cell-sorted order with per-workgroup LDS caches (same
+        # sums as the per-particle atomics, bit for bit)
+        self.p2g_sorted = True
+        self._p2g_scratch = torch.zeros(0, dtype=torch.int32, device=self.dev)
         self.time = 0.0
         self.step_index = 0
         self.stats: list[dict] = []
@@ -229,7 +233,14 @@ class AdaptiveSimulator:
             self._acc.zero_()
         src = self.particles.struct()
         self._acc_dirty = True
-        _lib.call("pf_mr_p2g", g, ctypes.byref(src), ctypes.byref(prm), ctypes.byref(f), s)
+        if self.p2g_sorted:
+            need = int(_lib.load().pf_mr_p2g_scratch_ints(self.particles.n, self.grid.ncells))
+            if self._p2g_scratch.numel() < need:
+                self._p2g_scratch = torch.empty(int(need * 1.25), dtype=torch.int32, device=self.dev)
+            _lib.call("pf_mr_p2g_sorted", g, ctypes.byref(src), ctypes.byref(prm), ctypes.byref(f),
+                      _lib.ptr(self._p2g_scratch), s)
+        else:
+            _lib.call("pf_mr_p2g", g, ctypes.byref(src), ctypes.byref(prm), ctypes.byref(f), s)
         _lib.call("pf_mr_finish", g, ctypes.byref(prm), ctypes.byref(f), s)
         self._acc_dirty = False
 

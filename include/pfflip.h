@@ -488,6 +488,13 @@ int pf_mr_max_speed(const float* v, int64_t n, float* out, pf_stream_t stream);
  * (particles.py:234-248) folded in: adds into acc. */
 int pf_mr_p2g(const pf_msbg_grid* g, const pf_mr_particles* ps, const pf_mr_params* prm, pf_mr_fields* f,
               pf_stream_t stream);
+/* pf_mr_p2g over a cell-sorted visit order with a per-workgroup LDS cache
+ * of cells (a counting sort by cell address, then the splat); the same
+ * sums as pf_mr_p2g bit for bit.  scratch holds
+ * pf_mr_p2g_scratch_ints(n, ncells) int32. */
+int64_t pf_mr_p2g_scratch_ints(int64_t n, int32_t ncells);
+int pf_mr_p2g_sorted(const pf_msbg_grid* g, const pf_mr_particles* ps, const pf_mr_params* prm, pf_mr_fields* f,
+                     int32_t* scratch, pf_stream_t stream);
 /* Step 5, face_velocity + cell_phase (mr_kernels.py:136-165): cwsum, phi,
  * is_liquid, fmass, fvel from acc; clears acc. */
 int pf_mr_finish(const pf_msbg_grid* g, const pf_mr_params* prm, pf_mr_fields* f, pf_stream_t stream);
