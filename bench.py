@@ -476,8 +476,31 @@ def run_adaptive128(pf, torch, args):
                    "parallelism": "single GPU"},
         "pcg_iters": [r["cg_iter"] for r in recs],
         "roofline": rl,
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.skip_cpu else adaptive_cpu_baseline(pf),
     }
+
+
+def adaptive_cpu_baseline(pf, steps=3):
+    """The FP64 numpy restatement of the adaptive step (oracle/ref_mr_sim.py)
+    on one core, on a bounded sample of the same workload: the dam break on
+    a 16^3 MSBG (2^3 blocks of 8^3, levels 0-2), 8 ppc, both phases."""
+    import numpy as np
+    from oracle import ref_mr_sim as R
+
+    n = 16
+    cfg = pf.Config(res=(n, n, n), dx=1.0 / n, ppc=8, alpha_phi=0.25, droplets=False, pcg_tol=1e-4,
+                    pcg_max_iter=400)
+    pos, vel, pt = pf.scenes.dam_break(cfg)
+    st = R.State(cfg, (2, 2, 2), 8, 2, 0.5, pos, vel, pt)
+    t0 = time.perf_counter()
+    done = 0
+    for _ in range(steps):
+        done += len(st.x)
+        R.step(st)
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/ref_mr_sim.py FP64 adaptive step, dam break 16^3 MSBG x 8 ppc "
+                      f"({len(pos)} particles), {steps} steps in {el:.1f} s"}
 
 
 def main():

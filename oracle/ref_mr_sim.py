@@ -324,17 +324,25 @@ def advect(g, x, v, fvel, c: Consts):
 
 # ---------------------------------------------------------------- refinement map
 def refine_map(g, x, plev, cfl_eps, max_level):
-    """atomic_min_refinement_map (particles.py:49-85)."""
+    """atomic_min_refinement_map (particles.py:49-85): min(l_p) over the
+    blocks each particle's box of half width cfl_eps touches."""
     bext = g.n0 * g.h0
-    M_ = np.full(g.nb, max_level, np.int64)
-    for p in range(len(x)):
-        lo = np.floor((x[p] - cfl_eps) / bext).astype(np.int64)
-        hi = np.floor((x[p] + cfl_eps) / bext).astype(np.int64)
-        lo = np.maximum(lo, 0)
-        hi = np.minimum(hi, np.array(g.nb) - 1)
-        sub = M_[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1]
-        np.minimum(sub, plev[p], out=sub)
-    return M_.ravel()
+    x = np.asarray(x, np.float64)
+    plev = np.asarray(plev, np.int64)
+    nb = np.array(g.nb)
+    lo = np.maximum(np.floor((x - cfl_eps) / bext).astype(np.int64), 0)
+    hi = np.minimum(np.floor((x + cfl_eps) / bext).astype(np.int64), nb - 1)
+    M_ = np.full(g.nblk, max_level, np.int64)
+    # every box spans at most `span` blocks per axis; visit offsets within it
+    span = int(max((hi - lo).max(initial=0), 0)) + 1
+    for di in range(span):
+        for dj in range(span):
+            for dk in range(span):
+                q = lo + np.array([di, dj, dk])
+                ok = (q <= hi).all(axis=1)
+                idx = (q[ok, 0] * nb[1] + q[ok, 1]) * nb[2] + q[ok, 2]
+                np.minimum.at(M_, idx, plev[ok])
+    return M_
 
 
 # ---------------------------------------------------------------- resample
@@ -425,3 +433,58 @@ def emit(x, v, ptype, plev, pdefer, lnew, nout, threshold, r0, seed):
             outs["defer"].append(0)
     return (np.array(outs["x"]).reshape(-1, 3), np.array(outs["v"]).reshape(-1, 3), np.array(outs["type"]),
             np.array(outs["level"]), np.array(outs["defer"]))
+
+
+# ---------------------------------------------------------------- one step
+class State:
+    """The adaptive simulator's state (mr_sim.py:40-71): particles as FP64
+    arrays, the grid, time and step index."""
+
+    def __init__(self, cfg, nblocks, n0, max_level, beta_bw, x, v, ptype, level=None):
+        self.cfg, self.nb, self.n0, self.max_level, self.beta_bw = cfg, tuple(nblocks), n0, max_level, beta_bw
+        self.x = np.asarray(x, np.float64).copy()
+        self.v = np.asarray(v, np.float64).copy()
+        self.type = np.asarray(ptype).astype(np.int64)
+        self.level = np.zeros(len(self.x), np.int64) if level is None else np.asarray(level, np.int64).copy()
+        self.defer = np.zeros(len(self.x), np.int64)
+        self.rho_tilde_0 = calibrate(cfg.ppc, cfg.dx, cfg.r_p)
+        self.bext = n0 * cfg.dx
+        self.domain = tuple(n * self.bext for n in self.nb)
+        self.time, self.step_index = 0.0, 0
+
+
+def dt_from_cfl(st):
+    """mr_sim.py:193-201."""
+    cfg = st.cfg
+    vmax = float(np.linalg.norm(st.v, axis=1).max()) if len(st.v) else 0.0
+    g = float(np.linalg.norm(cfg.gravity))
+    vmax = max(vmax, math.sqrt(2.0 * g * cfg.dx) if g > 0 else 0.0, 1e-8)
+    return float(np.clip(cfg.cfl * cfg.dx / vmax, cfg.dt_min, cfg.dt_max))
+
+
+def step(st, dt=None, defer_threshold=3):
+    """mr_sim.py:203-310 on the restated stages -> the step record."""
+    cfg = st.cfg
+    dt = dt_from_cfl(st) if dt is None else dt
+    g = M.Grid(st.nb, st.n0, cfg.dx, st.max_level)
+    g.build(refine_map(g, st.x, st.level, max(cfg.cfl, 1.0) * cfg.dx, st.max_level))
+    c = Consts(cfg, st.rho_tilde_0, dt, st.domain, st.max_level)
+    cm, cw, fm, fp = p2g(g, st.x, st.v, st.type, st.level, c.r0, c.rho_l, c.rho_g)
+    phi, tag, u = finish(g, cm, fm, fp, c)
+    beta, u, star = face_prepare(g, tag, fm, u, c)
+    b = rhs(g, u, cw, c)
+    p, it, rel = M.solve(g, beta, b, tol=cfg.pcg_tol, max_iter=cfg.pcg_max_iter, preconditioner="mg")
+    u, du = project(g, p, beta, u, star, dt)
+    div = max_divergence(g, u)
+    st.v = g2p(g, st.x, st.v, st.type, u, du, c)
+    n_before = len(st.x)
+    dist = block_distance(block_mixed(g, tag), g.nb)
+    ln, nout, defer = classify(st.x, st.level, st.defer, dist, st.bext, st.beta_bw, st.max_level, defer_threshold,
+                               st.step_index)
+    st.x, st.v, st.type, st.level, st.defer = emit(st.x, st.v, st.type, st.level, defer, ln, nout, defer_threshold,
+                                                   cfg.r_p, st.step_index + 7919)
+    st.x, st.v = advect(g, st.x, st.v, u, c)
+    st.time += dt
+    st.step_index += 1
+    return {"step": st.step_index, "dt": dt, "cells": g.ncells, "particles": len(st.x),
+            "dn": len(st.x) - n_before, "cg_iter": it, "cg_res": rel, "max_div": div, "grid": g}

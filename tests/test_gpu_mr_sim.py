@@ -255,6 +255,38 @@ def test_sorted_cached_p2g_equals_per_particle_atomics(pk):
     assert np.abs(out[0]["fmass"]).max() > 0
 
 
+def test_whole_step_matches_the_oracle_step(pk):
+    """One full adaptive step (two levels, moving two-phase pool, resampling
+    on) against ``ref_mr_sim.step`` from the same state and dt: the same
+    grid, particle count, levels and types; positions and velocities within
+    FP32 tolerances (the GPU solves in FP32 to the same 1e-6)."""
+    say("whole step vs oracle step")
+    cfg = pk["config"].Config(res=(8, 24, 8), dx=1 / 16, cfl=1.0, pcg_max_iter=400, pcg_tol=1e-6)
+    sim = pk["mr_sim"].AdaptiveSimulator(cfg, (2, 6, 2), n0=4, max_level=1, beta_bw=0.45)
+    seed_pool(sim, cfg, fill=0.3, vel=0.4)
+    p = sim.particles.numpy()
+    rng = np.random.default_rng(5)
+    y = p["x"][:, 1]
+    up = (y > 1.18) | ((y > 0.55) & (y < 0.7))
+    keep = ~up | (rng.random(len(up)) < 0.125)
+    lev = np.where(up, 1, 0)[keep].astype(np.int32)
+    sim.set_particles(p["x"][keep], p["v"][keep], p["type"][keep], lev)
+    x0, v0, t0 = p["x"][keep].copy(), p["v"][keep].copy(), p["type"][keep].copy()
+    dt = sim.dt_from_cfl()
+    rec = sim.step(dt)
+    st = R.State(cfg, (2, 6, 2), 4, 1, 0.45, x0, v0, t0, lev)
+    orec = R.step(st, dt)
+    out = sim.particles.numpy()
+    assert rec["cells"] == orec["cells"] and np.array_equal(sim.grid.level_h, orec["grid"].level)
+    assert rec["particles"] == orec["particles"], (rec["particles"], orec["particles"])
+    assert np.array_equal(out["level"], st.level) and np.array_equal(out["type"], st.type)
+    vs = np.abs(st.v).max()
+    dv = np.abs(out["v"] - st.v).max(axis=1)
+    assert np.quantile(dv, 0.999) < 1e-3 * vs and dv.max() < 2e-2 * vs, (np.quantile(dv, 0.999), dv.max(), vs)
+    assert np.abs(out["x"] - st.x).max() < 1e-3 * cfg.dx
+    assert abs(rec["cg_iter"] - orec["cg_iter"]) <= 2
+
+
 def test_step_is_deterministic(pk):
     """Two runs of the same adaptive steps give bitwise equal particles (P2G
     sums are order independent)."""

@@ -204,3 +204,23 @@ def test_block_distance_is_the_euclidean_distance_to_mixed_blocks():
         ref = np.sqrt(((pts - np.array(idx)) ** 2).sum(axis=1)).min()
         assert d[idx] == pytest.approx(min(ref, 64.0), rel=1e-6)
     assert (R.block_distance(np.zeros(nb), nb) == 64.0).all()
+
+
+def test_composed_step_holds_a_pool_at_rest():
+    """tests/test_mr_sim.py:42-62 on the restatement: with max_level = 0 two
+    steps keep a pool at rest (speeds below 0.6 g dt, centroid fixed), and
+    nothing is resampled."""
+    dx = 1 / 16
+    cfg = _cfg(res=(16,) * 3, dx=dx, cfl=1.0, pcg_max_iter=400, pcg_tol=1e-6)
+    rng = np.random.default_rng(0)
+    x = _lattice((2, 2, 2), 8, dx, 2)
+    x = x + (rng.random(x.shape) - 0.5) * dx * 0.4
+    liq = (x[:, 1] < 0.5).astype(np.int64)
+    st = R.State(cfg, (2, 2, 2), 8, 0, 1.0, x, np.zeros_like(x), liq)
+    y0 = st.x[liq == 1, 1].mean()
+    for _ in range(2):
+        rec = R.step(st)
+        assert rec["particles"] == len(x) and rec["cells"] == 16 ** 3
+    speed = np.linalg.norm(st.v[st.type == 1], axis=1).max()
+    assert speed < 0.6 * 9.81 * rec["dt"], speed
+    assert abs(st.x[st.type == 1, 1].mean() - y0) < 1e-3
