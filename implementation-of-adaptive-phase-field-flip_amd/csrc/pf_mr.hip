@@ -335,13 +335,14 @@ __global__ __launch_bounds__(256) void k_mr_face_prepare(MGrid g, MRConst c, con
                                                          float* __restrict__ fvel, float* __restrict__ fvel_star) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= g.ncells) return;
-    const int4 o = origin(g, a);
+    BlockRef br;
+    const int4 o = origin_ref(g, a, br);
     const float norm = (float)(1 << (3 * o.w));
     const int hi = is_liquid[a];
     const float gv[3] = {c.gx, c.gy, c.gz};
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const MCell nb = locate(g, o.x - comp(ax, 0), o.y - comp(ax, 1), o.z - comp(ax, 2));
+        const MCell nb = locate_near(g, br, o.x - comp(ax, 0), o.y - comp(ax, 1), o.z - comp(ax, 2));
         float beta = 0.0f;
         if (nb.addr >= 0) {
             const int lo = is_liquid[nb.addr];
@@ -364,10 +365,11 @@ __global__ __launch_bounds__(256) void k_mr_face_prepare(MGrid g, MRConst c, con
 
 // flux-weighted velocity through the high face: a finer neighbour backs the
 // face with a 2x2 patch whose mean is taken (mr_kernels.py:230-283)
-__device__ inline float face_velocity_high(const MGrid& g, const float* __restrict__ fvel, int4 o, int ax) {
+__device__ inline float face_velocity_high(const MGrid& g, const BlockRef& br, const float* __restrict__ fvel, int4 o,
+                                          int ax) {
     const int ext = 1 << o.w;
     const int fx = o.x + comp(ax, 0) * ext, fy = o.y + comp(ax, 1) * ext, fz = o.z + comp(ax, 2) * ext;
-    const MCell nb = locate(g, fx, fy, fz);
+    const MCell nb = locate_near(g, br, fx, fy, fz);
     if (nb.addr < 0) return 0.0f;
     if (nb.level >= o.w) return fvel[nb.addr * 3 + ax];
     const int sub = 1 << nb.level;
@@ -376,8 +378,8 @@ __device__ inline float face_velocity_high(const MGrid& g, const float* __restri
     for (int u = 0; u < 2; ++u)
         for (int v = 0; v < 2; ++v) {
             const int du = u * sub, dv = v * sub;
-            const MCell nc = locate(g, fx + comp(a1, 0) * du + comp(a2, 0) * dv,
-                                    fy + comp(a1, 1) * du + comp(a2, 1) * dv, fz + comp(a1, 2) * du + comp(a2, 2) * dv);
+            const MCell nc = locate_near(g, br, fx + comp(a1, 0) * du + comp(a2, 0) * dv,
+                                         fy + comp(a1, 1) * du + comp(a2, 1) * dv, fz + comp(a1, 2) * du + comp(a2, 2) * dv);
             if (nc.addr >= 0) {
                 acc += fvel[nc.addr * 3 + ax];
                 cnt += 1.0f;
@@ -392,11 +394,12 @@ __global__ __launch_bounds__(256) void k_mr_rhs(MGrid g, MRConst c, const float*
                                                 const float* __restrict__ cwsum, float* __restrict__ rhs) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= g.ncells) return;
-    const int4 o = origin(g, a);
+    BlockRef br;
+    const int4 o = origin_ref(g, a, br);
     const float h = cell_size(g, o.w), area = h * h;
     float flux = 0.0f;
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) flux += (face_velocity_high(g, fvel, o, ax) - fvel[a * 3 + ax]) * area;
+    for (int ax = 0; ax < 3; ++ax) flux += (face_velocity_high(g, br, fvel, o, ax) - fvel[a * 3 + ax]) * area;
     float r = -flux / c.dt;
     if (c.k_div > 0.0f) {
         const float excess = (cwsum[a] / (float)(1 << (3 * o.w)) - c.rho_tilde_0) / c.rho_tilde_0;
@@ -412,7 +415,8 @@ __global__ __launch_bounds__(256) void k_mr_project(MGrid g, MRConst c, const fl
                                                     const float* __restrict__ fvel_star, float* __restrict__ fdvel) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= g.ncells) return;
-    const int4 o = origin(g, a);
+    BlockRef br;
+    const int4 o = origin_ref(g, a, br);
     const float hA = cell_size(g, o.w);
     const float pa = pres[a];
 #pragma unroll
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(256) void k_mr_project(MGrid g, MRConst c, const fl
         float u = fvel[a * 3 + ax];
         const float beta = fbeta[a * 3 + ax];
         if (beta != 0.0f) {
-            const MCell nb = locate(g, o.x - comp(ax, 0), o.y - comp(ax, 1), o.z - comp(ax, 2));
+            const MCell nb = locate_near(g, br, o.x - comp(ax, 0), o.y - comp(ax, 1), o.z - comp(ax, 2));
             if (nb.addr >= 0) {
                 const float dist = 0.5f * (hA + cell_size(g, nb.level));
                 u -= c.dt * beta * (pa - pres[nb.addr]) / dist;
@@ -436,10 +440,11 @@ __global__ __launch_bounds__(256) void k_mr_max_div(MGrid g, const float* __rest
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     float m = 0.0f;
     if (a < g.ncells) {
-        const int4 o = origin(g, a);
+        BlockRef br;
+        const int4 o = origin_ref(g, a, br);
         float d = 0.0f;
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) d += face_velocity_high(g, fvel, o, ax) - fvel[a * 3 + ax];
+        for (int ax = 0; ax < 3; ++ax) d += face_velocity_high(g, br, fvel, o, ax) - fvel[a * 3 + ax];
         m = fabsf(d / cell_size(g, o.w));
     }
     __shared__ float sh[4];
@@ -451,9 +456,25 @@ __global__ __launch_bounds__(256) void k_mr_max_div(MGrid g, const float* __rest
 // trilinear face sample at the resolution of the cell containing x; taps
 // resolve through locate, so a coarser or finer neighbour contributes its own
 // face (mr_kernels.py:359-405)
-__device__ inline float sample_face(const MGrid& g, const float* __restrict__ f, float x, float y, float z, int ax) {
-    const MCell c = locate_at(g, x, y, z);
+// the cell containing a sample point and its block, resolved once for the
+// three staggered components
+struct SamplePoint {
+    MCell c;
+    BlockRef br;
+};
+
+__device__ inline SamplePoint sample_point(const MGrid& g, float x, float y, float z) {
+    SamplePoint s;
+    s.c = locate_at(g, x, y, z);
+    if (s.c.addr >= 0) s.br = block_ref(g, s.c.slot);
+    return s;
+}
+
+__device__ inline float sample_face(const MGrid& g, const float* __restrict__ f, float x, float y, float z, int ax,
+                                    const SamplePoint& sp) {
+    const MCell& c = sp.c;
     if (c.addr < 0) return 0.0f;
+    const BlockRef& br = sp.br;
     const int ext = 1 << c.level;
     const float h = cell_size(g, c.level);
     const float sx = x / h - (ax == 0 ? 0.0f : 0.5f);
@@ -471,7 +492,7 @@ __device__ inline float sample_face(const MGrid& g, const float* __restrict__ f,
                 const float wk = dk == 0 ? 1.0f - tz : tz;
                 const float w = wi * wj * wk;
                 if (w <= 0.0f) continue;
-                const MCell nc = locate(g, (i0 + di) * ext, (j0 + dj) * ext, (k0 + dk) * ext);
+                const MCell nc = locate_near(g, br, (i0 + di) * ext, (j0 + dj) * ext, (k0 + dk) * ext);
                 if (nc.addr < 0) continue;
                 acc += w * f[nc.addr * 3 + ax];
                 wsum += w;
@@ -489,15 +510,16 @@ __global__ __launch_bounds__(256) void k_mr_g2p(MGrid g, const float* __restrict
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const float px = x[3 * p], py = x[3 * p + 1], pz = x[3 * p + 2];
-    const MCell cl = locate_at(g, px, py, pz);
+    const SamplePoint sp = sample_point(g, px, py, pz);
+    const MCell& cl = sp.c;
     const int lv = cl.addr >= 0 ? cl.level : 0;
     const float dx = cell_size(g, lv);
     const float nu = type[p] == 1 ? c.nu_l : c.nu_g;
     const float alpha = fminf(fmaxf(1.0f - 6.0f * nu * c.dt / (dx * dx), 0.0f), c.alpha_cap);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const float delta = sample_face(g, fdvel, px, py, pz, ax);
-        const float pic = sample_face(g, fvel, px, py, pz, ax);
+        const float delta = sample_face(g, fdvel, px, py, pz, ax, sp);
+        const float pic = sample_face(g, fvel, px, py, pz, ax, sp);
         v[3 * p + ax] = alpha * (v[3 * p + ax] + delta) + (1.0f - alpha) * pic;
     }
 }
@@ -511,16 +533,19 @@ __global__ __launch_bounds__(256) void k_mr_advect(MGrid g, float* __restrict__ 
     if (p >= n) return;
     float q[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
     float k1[3], k2[3], k3[3], t[3];
+    SamplePoint sp = sample_point(g, q[0], q[1], q[2]);
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) k1[ax] = sample_face(g, fvel, q[0], q[1], q[2], ax);
+    for (int ax = 0; ax < 3; ++ax) k1[ax] = sample_face(g, fvel, q[0], q[1], q[2], ax, sp);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) t[ax] = q[ax] + 0.5f * c.dt * k1[ax];
+    sp = sample_point(g, t[0], t[1], t[2]);
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) k2[ax] = sample_face(g, fvel, t[0], t[1], t[2], ax);
+    for (int ax = 0; ax < 3; ++ax) k2[ax] = sample_face(g, fvel, t[0], t[1], t[2], ax, sp);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) t[ax] = q[ax] + 0.75f * c.dt * k2[ax];
+    sp = sample_point(g, t[0], t[1], t[2]);
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) k3[ax] = sample_face(g, fvel, t[0], t[1], t[2], ax);
+    for (int ax = 0; ax < 3; ++ax) k3[ax] = sample_face(g, fvel, t[0], t[1], t[2], ax, sp);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         float xa = q[ax] + c.dt * ((2.0f / 9.0f) * k1[ax] + (3.0f / 9.0f) * k2[ax] + (4.0f / 9.0f) * k3[ax]);

@@ -27,7 +27,8 @@ int fill_u32(void* dst, int64_t n, uint32_t v, hipStream_t s);
 // the 2x2 patch behind it.
 template <class F>
 __device__ inline void for_each_coupling(const MGrid& g, const float* __restrict__ bface, int addr, F&& f) {
-    const int4 o = origin(g, addr);
+    BlockRef br;
+    const int4 o = origin_ref(g, addr, br);
     const int la = o.w;
     const int ext = 1 << la;
     const float hA = g.h0 * (float)ext;
@@ -44,7 +45,7 @@ __device__ inline void for_each_coupling(const MGrid& g, const float* __restrict
                 fy = o.y + ey * ext;
                 fz = o.z + ez * ext;
             }
-            const MCell nb = locate(g, fx, fy, fz);
+            const MCell nb = locate_near(g, br, fx, fy, fz);
             if (nb.addr < 0) continue;  // domain wall: no flux
             const float hB = g.h0 * (float)(1 << nb.level);
             if (nb.level >= la) {
@@ -63,9 +64,9 @@ __device__ inline void for_each_coupling(const MGrid& g, const float* __restrict
                 for (int u = 0; u < 2; ++u)
                     for (int v = 0; v < 2; ++v) {
                         const int du = u * sub, dv = v * sub;
-                        const MCell nc = locate(g, fx + comp(a1, 0) * du + comp(a2, 0) * dv,
-                                                fy + comp(a1, 1) * du + comp(a2, 1) * dv,
-                                                fz + comp(a1, 2) * du + comp(a2, 2) * dv);
+                        const MCell nc = locate_near(g, br, fx + comp(a1, 0) * du + comp(a2, 0) * dv,
+                                                     fy + comp(a1, 1) * du + comp(a2, 1) * dv,
+                                                     fz + comp(a1, 2) * du + comp(a2, 2) * dv);
                         if (nc.addr < 0) continue;
                         const float beta = side == 0 ? bface[addr * 3 + axis] : bface[nc.addr * 3 + axis];
                         f(nc.addr, beta * area / dist);

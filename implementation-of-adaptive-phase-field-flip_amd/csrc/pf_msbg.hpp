@@ -74,6 +74,50 @@ __device__ inline int4 origin(const MGrid& g, int addr) {
 
 __device__ inline int comp(int axis, int c) { return axis == c ? 1 : 0; }
 
+// A cell's block: slot, level, cells per axis, first address and the
+// block's low corner in finest coordinates.
+struct BlockRef {
+    int slot, level, n, base, bx0, by0, bz0;
+};
+
+__device__ inline BlockRef block_ref(const MGrid& g, int s) {
+    BlockRef r;
+    r.slot = s;
+    r.level = g.blev[s];
+    r.n = g.n0 >> r.level;
+    r.base = g.coff[s];
+    const int b = g.blk[s];
+    r.bz0 = (b % g.nbz) << g.sh0;
+    r.by0 = ((b / g.nbz) % g.nby) << g.sh0;
+    r.bx0 = (b / (g.nbz * g.nby)) << g.sh0;
+    return r;
+}
+
+// locate() for a finest coordinate near block `r`: inside that block the
+// address follows from the local indices without the lookup chain
+// (slot -> level, offset); elsewhere locate().  Same result as locate().
+__device__ inline MCell locate_near(const MGrid& g, const BlockRef& r, int fx, int fy, int fz) {
+    const unsigned ux = (unsigned)(fx - r.bx0), uy = (unsigned)(fy - r.by0), uz = (unsigned)(fz - r.bz0);
+    if ((ux | uy | uz) < (unsigned)g.n0) {
+        MCell c;
+        c.addr = r.base + (((int)ux >> r.level) * r.n + ((int)uy >> r.level)) * r.n + ((int)uz >> r.level);
+        c.slot = r.slot;
+        c.level = r.level;
+        return c;
+    }
+    return locate(g, fx, fy, fz);
+}
+
+// origin() and the cell's block in one pass
+__device__ inline int4 origin_ref(const MGrid& g, int addr, BlockRef& r) {
+    r = block_ref(g, g.cslot[addr]);
+    int q = addr - r.base;
+    const int i = q / (r.n * r.n);
+    q -= i * r.n * r.n;
+    const int j = q / r.n, k = q - j * r.n;
+    return make_int4(r.bx0 + (i << r.level), r.by0 + (j << r.level), r.bz0 + (k << r.level), r.level);
+}
+
 // the cell containing a world position (msbg.py:154-161)
 __device__ inline MCell locate_at(const MGrid& g, float x, float y, float z) {
     return locate(g, (int)floorf(x / g.h0), (int)floorf(y / g.h0), (int)floorf(z / g.h0));
