@@ -138,9 +138,16 @@ __global__ __launch_bounds__(256) void k_mr_max_speed(const float* __restrict__ 
 // Mass (eq. 3) and momentum (eq. 4) on the low faces, mass and the bare
 // weight sum at the centres, over the cells of every block the support
 // sphere touches (mr_kernels.py:48-133).  m_p = rho 8^l_p, r_p = r0 2^l_p
-// (particles.py:234-248).  `add(channel, cell, value)` receives the
-// fixed-point value of every contribution.
-__device__ inline long long fixed(float v, float scale) { return __float2ll_rn(v * scale); }
+// (particles.py:234-248).  `add(cell, values, mask)` receives the
+// fixed-point contributions to one cell (bit ch of mask: channel ch set).
+// round(v * scale) to the nearest integer (ties to even), for |v * scale| <
+// 2^51: adding 1.5 * 2^52 in double leaves the rounded integer in the low
+// mantissa bits (3 instructions; the float-to-int64 conversion takes 11).
+// The host keeps every single contribution below 2^48.
+__device__ inline long long fixed(float v, float scale) {
+    const double d = (double)(v * scale) + 6755399441055744.0;
+    return __double_as_longlong(d) - 0x4338000000000000LL;
+}
 
 template <class Add>
 __device__ inline void splat_particle(const MGrid& g, const float* __restrict__ x, const float* __restrict__ v,
@@ -191,6 +198,8 @@ __device__ inline void splat_particle(const MGrid& g, const float* __restrict__ 
                             const float zc2 = dzc * dzc, zf2 = dzf * dzf;
                             const int addr = base + (i * nn + j) * nn + k;
                             const float d2[4] = {xc2 + yc2 + zc2, xf2 + yc2 + zc2, xc2 + yf2 + zc2, xc2 + yc2 + zf2};
+                            long long val[kAcc];
+                            unsigned mask = 0u;
 #pragma unroll
                             for (int q = 0; q < 4; ++q) {
                                 const float t = fmaxf(1.0f - d2[q] * inv_r2, 0.0f);
@@ -198,13 +207,16 @@ __device__ inline void splat_particle(const MGrid& g, const float* __restrict__ 
                                 if (!(w > 0.0f)) continue;
                                 const float wm = w * mp;
                                 if (q == 0) {
-                                    add(0, addr, fixed(wm, c.mass_scale));
-                                    add(1, addr, fixed(w, c.w_scale));
+                                    val[0] = fixed(wm, c.mass_scale);
+                                    val[1] = fixed(w, c.w_scale);
+                                    mask |= 3u;
                                 } else {
-                                    add(1 + q, addr, fixed(wm, c.mass_scale));
-                                    add(4 + q, addr, fixed(wm * vel[q - 1], c.mom_scale));
+                                    val[1 + q] = fixed(wm, c.mass_scale);
+                                    val[4 + q] = fixed(wm * vel[q - 1], c.mom_scale);
+                                    mask |= (1u << (1 + q)) | (1u << (4 + q));
                                 }
                             }
+                            if (mask) add(addr, val, mask);
                         }
                     }
                 }
@@ -218,8 +230,10 @@ __global__ __launch_bounds__(256) void k_mr_p2g(MGrid g, const float* __restrict
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int64_t nc = g.ncells;
-    splat_particle(g, x, v, type, lev, p, c, [&](int ch, int addr, long long val) {
-        atomicAdd(acc + ch * nc + addr, (unsigned long long)val);
+    splat_particle(g, x, v, type, lev, p, c, [&](int addr, const long long* val, unsigned mask) {
+#pragma unroll
+        for (int ch = 0; ch < kAcc; ++ch)
+            if (mask & (1u << ch)) atomicAdd(acc + ch * nc + addr, (unsigned long long)val[ch]);
     });
 }
 
@@ -270,17 +284,22 @@ __global__ __launch_bounds__(256) void k_mr_p2g_cached(MGrid g, const float* __r
     for (int q = threadIdx.x; q < kP2GPerBlock; q += blockDim.x) {
         if (p0 + q >= n) break;
         const int64_t p = perm[p0 + q];
-        splat_particle(g, x, v, type, lev, p, c, [&](int ch, int addr, long long val) {
+        splat_particle(g, x, v, type, lev, p, c, [&](int addr, const long long* val, unsigned mask) {
             const int s = (int)(((unsigned)addr * 2654435761u) >> 22);  // 10-bit multiplicative hash
             int t = tag[s];
             if (t != addr && t == -1) {
                 t = atomicCAS(tag + s, -1, addr);
                 if (t == -1) t = addr;
             }
-            if (t == addr)
-                atomicAdd(lacc + s * kAcc + ch, (unsigned long long)val);
-            else
-                atomicAdd(acc + ch * nc + addr, (unsigned long long)val);
+            if (t == addr) {
+#pragma unroll
+                for (int ch = 0; ch < kAcc; ++ch)
+                    if (mask & (1u << ch)) atomicAdd(lacc + s * kAcc + ch, (unsigned long long)val[ch]);
+            } else {
+#pragma unroll
+                for (int ch = 0; ch < kAcc; ++ch)
+                    if (mask & (1u << ch)) atomicAdd(acc + ch * nc + addr, (unsigned long long)val[ch]);
+            }
         });
     }
     __syncthreads();
