@@ -424,3 +424,82 @@ class _nullcontext:
 
     def __exit__(self, *a):
         return False
+
+
+# ---------------------------------------------------------------- edge cases
+def test_empty_particle_set(pk):
+    """No particles: the grid stays at max_level, every cell is air (gravity
+    on the open faces against the walls still asks for a pressure, which
+    converges), and the step completes with nothing to move."""
+    say("edge: empty particle set")
+    cfg = pk["config"].Config(res=(16,) * 3, dx=1 / 16, cfl=1.0)
+    sim = pk["mr_sim"].AdaptiveSimulator(cfg, (2, 2, 2), n0=8, max_level=2)
+    sim.set_particles(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32), np.zeros(0, np.int32))
+    rec = sim.step()
+    assert rec["particles"] == 0 and rec["dn"] == 0 and rec["cg_res"] < cfg.pcg_tol
+    assert int(np_(sim.is_liquid).sum()) == 0
+    assert set(np.unique(sim.grid.level_h)) == {2}
+
+
+def test_particles_outside_the_domain_are_clamped_in(pk):
+    """Particles seeded outside the box splat nothing, sample zero velocity
+    and are clamped to [eps, L - eps] by the advection (mr_kernels.py:421-461)."""
+    say("edge: particles outside the domain")
+    cfg = pk["config"].Config(res=(16,) * 3, dx=1 / 16, cfl=1.0, pcg_max_iter=200)
+    sim = pk["mr_sim"].AdaptiveSimulator(cfg, (2, 2, 2), n0=8, max_level=1, resample=False)
+    seed_pool(sim, cfg)
+    p = sim.particles.numpy()
+    out = np.array([[-0.3, 0.5, 0.5], [1.4, 0.2, 0.7], [0.5, -2.0, 3.0]], np.float32)
+    x = np.concatenate([p["x"], out])
+    sim.set_particles(x, np.zeros_like(x), np.concatenate([p["type"], [1, 0, 1]]).astype(np.int32))
+    sim.step()
+    q = sim.particles.numpy()
+    eps = 1e-4 * cfg.dx
+    assert q["x"].min() >= eps * 0.999 and q["x"].max() <= 1.0 - eps * 0.999
+    assert np.isfinite(q["v"]).all()
+
+
+def test_deep_hierarchy_four_levels(pk):
+    """n0 = 16 blocks with levels 0-3 (the reference's default max_level) on
+    a domain 8 blocks tall, graded 2:1, resampling on: particles coarsen one
+    level per three deferred steps, the grid follows them to level 2 within
+    eight steps, the liquid centroid stays put, every particle level stays
+    in [0, 3] and the grading stays 2:1 after each regrid."""
+    say("edge: four levels, n0 = 16")
+    cfg = pk["config"].Config(res=(64, 128, 64), dx=1 / 64, cfl=1.0, pcg_max_iter=400, alpha_phi=0.25)
+    sim = pk["mr_sim"].AdaptiveSimulator(cfg, (4, 8, 4), n0=16, max_level=3, beta_bw=0.5)
+    seed_pool(sim, cfg, fill=0.2)
+    ys = []
+    for _ in range(8):
+        rec = sim.step()
+        lv = sim.grid.level_h.reshape(sim.grid.nb)
+        for ax in range(3):
+            assert np.abs(np.diff(lv, axis=ax)).max() <= 1
+        ys.append(liquid_stats(sim)[1])
+        q = sim.particles.numpy()
+        assert q["level"].min() >= 0 and q["level"].max() <= 3
+    assert max(sim.grid.level_histogram()) >= 2
+    assert max(ys) - min(ys) < 0.02
+    assert rec["compression"] > 1.5
+
+
+def test_particle_capacity_grows_through_splits(pk):
+    """Coarse particles next to the interface split 8 ways; the particle
+    arrays grow past their initial capacity and keep every survivor."""
+    say("edge: capacity growth")
+    cfg = pk["config"].Config(res=(32,) * 3, dx=1 / 32, cfl=1.0, pcg_max_iter=200)
+    staging = pk["mr_sim"].AdaptiveSimulator(cfg, (4, 4, 4), n0=8, max_level=1, beta_bw=0.34)
+    seed_pool(staging, cfg)
+    p = staging.particles.numpy()
+    del staging
+    rng = np.random.default_rng(3)
+    sel = rng.random(len(p["x"])) < 0.125  # a level-1 cloud everywhere
+    sim = pk["mr_sim"].AdaptiveSimulator(cfg, (4, 4, 4), n0=8, max_level=1, beta_bw=0.34)
+    sim.set_particles(p["x"][sel], p["v"][sel], p["type"][sel], np.ones(sel.sum(), np.int32))
+    cap0 = sim.particles.capacity
+    n0 = sim.particles.n
+    rec = sim.step()
+    assert rec["dn"] > 0 and sim.particles.n == n0 + rec["dn"]
+    assert sim.particles.n > cap0 >= n0 and sim.particles.capacity >= sim.particles.n
+    q = sim.particles.numpy()
+    assert (q["level"] == 0).sum() % 8 == 0 and (q["level"] == 0).any()
