@@ -173,6 +173,9 @@ SIGNATURES = {
     "pf_vec_reduce": (ctypes.c_int, [I32, P, P, I64, P, P]),
     "pf_vec_axpby": (ctypes.c_int, [F32, P, F32, P, I64, P]),
     "pf_vec_jacobi": (ctypes.c_int, [P, P, P, I64, P]),
+    "pf_vec_axpby_dev": (ctypes.c_int, [P, I32, F32, P, I32, P, I64, P, P]),
+    "pf_vec_remove_mean": (ctypes.c_int, [P, I64, P, P]),
+    "pf_mrcg_scalar": (ctypes.c_int, [I32, P, P, P, P, F64, P]),
     "pf_mr_refine_map": (ctypes.c_int, [ctypes.POINTER(MrParticles), P, I32, I32, I32, F32, F32, I32, P]),
     "pf_mr_max_speed": (ctypes.c_int, [P, I64, P, P]),
     "pf_mr_p2g": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MrParticles), ctypes.POINTER(MrParams),

@@ -438,6 +438,67 @@ __global__ __launch_bounds__(256) void k_vec_axpby(float a, const float* __restr
         y[i] = x ? a * x[i] + b * y[i] : b * y[i] + a;
 }
 
+// y = a x + b y with a = a_sign * st[ia] and b = st[ib] (or 1 when ib < 0)
+// rounded to float, the scalars of the flexible PCG kept on the device; a
+// no-op once flag[0] is set
+__global__ __launch_bounds__(256) void k_vec_axpby_dev(const double* __restrict__ st, int ia, float a_sign,
+                                                       const float* __restrict__ x, int ib,
+                                                       float* __restrict__ y, int64_t n,
+                                                       const int* __restrict__ flag) {
+    if (flag && flag[0]) return;
+    const float a = ia >= 0 ? (float)((double)a_sign * st[ia]) : a_sign;
+    const float b = ib >= 0 ? (float)st[ib] : 1.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = a * x[i] + b * y[i];
+}
+
+// y = 1 y + (float)(-sum / n): the mean removal of mr_multigrid.py:181-184
+// with the sum left on the device by pf_vec_reduce
+__global__ __launch_bounds__(256) void k_vec_sub_mean(float* __restrict__ y, int64_t n,
+                                                      const double* __restrict__ sum) {
+    const float a = (float)(-*sum / (double)n);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = 1.0f * y[i] + a;
+}
+
+// The scalar steps of the flexible Polak-Ribiere PCG (mr_solver.py:118-165)
+// on one thread.  st: 0 rz, 1 alpha, 2 beta, 3 rel, 4 bnorm, 5 tol, 6 qAq,
+// 7 a copy of flag[0]; flag[0]: 0 running, 1 stopped at qAq <= 0, 2 converged, 3 stopped at
+// rz == 0 or rz_new <= 0.
+enum MrcgOp { MRCG_INIT = 0, MRCG_RZ, MRCG_ALPHA, MRCG_REL, MRCG_BETA };
+
+__global__ void k_mrcg_scalar(int op, double* st, int* flag, const double* a, const double* b, double tol) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (op == MRCG_INIT) {  // a = max|b|
+        st[4] = *a;
+        st[5] = tol;
+        st[3] = 1.0;
+        flag[0] = 0;
+    } else if (op == MRCG_RZ) {
+        st[0] = *a;
+    } else if (flag[0]) {
+        // stopped: the scalars keep their values
+    } else if (op == MRCG_ALPHA) {
+        st[6] = *a;
+        if (*a <= 0.0)
+            flag[0] = 1;
+        else
+            st[1] = st[0] / *a;
+    } else if (op == MRCG_REL) {
+        st[3] = *a / st[4];
+        if (st[3] < st[5]) flag[0] = 2;
+    } else if (op == MRCG_BETA) {  // a = <r, z>, b = <r_prev, z>
+        const double rz_new = *a;
+        if (st[0] == 0.0 || rz_new <= 0.0) {
+            flag[0] = 3;
+        } else {
+            st[2] = fmax((rz_new - *b) / st[0], 0.0);
+            st[0] = rz_new;
+        }
+    }
+    st[7] = (double)flag[0];  // the host reads st[0..7] in one copy
+}
+
 // z = r / diag where diag > 0, else 0 (mr_solver.py:64-69)
 __global__ __launch_bounds__(256) void k_vec_jacobi(const float* __restrict__ r, const float* __restrict__ diag,
                                                     float* __restrict__ z, int64_t n) {
@@ -630,6 +691,37 @@ PF_API int pf_vec_axpby(float a, const float* x, float b, float* y, int64_t n, p
     const int64_t nb64 = cdiv(n, 256 * 4);
     const int nb = (int)(nb64 < kVecBlocks ? nb64 : kVecBlocks);
     k_vec_axpby<<<nb, 256, 0, S(stream)>>>(a, x, b, y, n);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_vec_axpby_dev(const double* st, int32_t ia, float a_sign, const float* x, int32_t ib, float* y,
+                            int64_t n, const int32_t* flag, pf_stream_t stream) {
+    PF_REQUIRE(st && x && y && n > 0, "pf_vec_axpby_dev: null or empty vector");
+    PF_REQUIRE(ia < 8 && ib < 8, "pf_vec_axpby_dev: scalar slot out of range");
+    const int64_t nb64 = cdiv(n, 256 * 4);
+    const int nb = (int)(nb64 < kVecBlocks ? nb64 : kVecBlocks);
+    k_vec_axpby_dev<<<nb, 256, 0, S(stream)>>>(st, ia, a_sign, x, ib, y, n, flag);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_vec_remove_mean(float* x, int64_t n, double* work, pf_stream_t stream) {
+    int rc = pf_vec_reduce(0, x, nullptr, n, work, stream);
+    if (rc) return rc;
+    const int64_t nb64 = cdiv(n, 256 * 4);
+    const int nb = (int)(nb64 < kVecBlocks ? nb64 : kVecBlocks);
+    k_vec_sub_mean<<<nb, 256, 0, S(stream)>>>(x, n, work);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_mrcg_scalar(int32_t op, double* st, int32_t* flag, const double* a, const double* b, double tol,
+                          pf_stream_t stream) {
+    PF_REQUIRE(st && flag && (op < MRCG_ALPHA || a), "pf_mrcg_scalar: null pointer");
+    PF_REQUIRE(op >= MRCG_INIT && op <= MRCG_BETA && (op != MRCG_BETA || b) && (op > MRCG_RZ || a),
+               "pf_mrcg_scalar: bad op");
+    k_mrcg_scalar<<<1, 64, 0, S(stream)>>>(op, st, flag, a, b, tol);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }

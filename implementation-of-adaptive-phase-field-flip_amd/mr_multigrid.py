@@ -45,7 +45,8 @@ def axpby(a, x, b, y):
 
 
 def remove_mean(a, work):
-    axpby(-reduce(0, a, None, work) / a.numel(), None, 1.0, a)
+    """a -= mean(a) (mr_multigrid.py:181-184), the FP64 sum kept on the device."""
+    _lib.call("pf_vec_remove_mean", _lib.ptr(a), a.numel(), _lib.ptr(work), _lib.stream())
 
 
 class MRMultigrid:

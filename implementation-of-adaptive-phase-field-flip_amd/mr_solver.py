@@ -1,8 +1,9 @@
 """Flexible Polak-Ribiere PCG on an MSBG (drop-in for pfflip/mr_solver.py).
 
-Host-driven like the reference (one device-to-host read per reduction);
-the vector kernels and reductions (FP64, fixed combination order) run in
-``pf_msbg.hip``.  Preconditioners: the multigrid V-cycle (``"mg"``), the
+Host-driven like the reference, but the CG scalars stay on the device: the
+host reads one state record per iteration (the reference reads every
+reduction).  The vector kernels, reductions (FP64, fixed combination order)
+and scalar steps run in ``pf_msbg.hip``.  Preconditioners: the multigrid V-cycle (``"mg"``), the
 same with the adaptive relaxation smoother (``"mg-adaptive"``) or Jacobi
 (``"jacobi"``).
 """
@@ -13,7 +14,7 @@ import torch
 
 from . import _lib
 from . import mr_operator as mro
-from .mr_multigrid import MRMultigrid, axpby, reduce, remove_mean
+from .mr_multigrid import MRMultigrid, remove_mean
 
 
 class MRSolver:
@@ -29,8 +30,12 @@ class MRSolver:
         self.n = n
         z = lambda: torch.zeros(n, dtype=torch.float32, device=grid.dev)  # noqa: E731
         self.diag, self.r, self.zv, self.q, self.Ap, self.b, self.r_prev = (z() for _ in range(7))
-        self._work = torch.zeros(int(_lib.load().pf_vec_reduce_bytes()) // 8, dtype=torch.float64,
-                                 device=grid.dev)
+        nw = int(_lib.load().pf_vec_reduce_bytes()) // 8
+        self._work = torch.zeros(nw, dtype=torch.float64, device=grid.dev)
+        # one reduction buffer per scalar of an iteration; the state and flag
+        self._works = [torch.zeros(nw, dtype=torch.float64, device=grid.dev) for _ in range(6)]
+        self._st = torch.zeros(8, dtype=torch.float64, device=grid.dev)
+        self._flag = torch.zeros(2, dtype=torch.int32, device=grid.dev)
 
     def prepare(self, bface):
         mro.compute_diag(self.grid, bface, self.diag)
@@ -45,41 +50,55 @@ class MRSolver:
             remove_mean(z, self._work)
 
     def solve(self, bface, rhs, p_out, *, tol=1e-4, max_iter=2000, verbose=False):
-        """mr_solver.py:118-165; returns (iterations, relative residual)."""
-        w = self._work
+        """mr_solver.py:118-165; returns (iterations, relative residual).
+
+        The CG scalars (rz, alpha, beta, rel) stay on the device; the host
+        reads the state once per iteration to stop (the reference reads
+        every reduction).  The arithmetic is the reference's: FP64 scalars,
+        each rounded to float where it scales a vector."""
+        s = _lib.stream()
+        W = self._works
+        st, flag = self._st, self._flag
         self.prepare(bface)
         self.b.copy_(rhs)
-        remove_mean(self.b, w)
-        bnorm = reduce(2, self.b, None, w)
-        if bnorm == 0.0:
-            p_out.zero_()
-            return 0, 0.0
+        remove_mean(self.b, self._work)
+        _lib.call("pf_vec_reduce", 2, _lib.ptr(self.b), None, self.n, _lib.ptr(W[0]), s)
+        _lib.call("pf_mrcg_scalar", 0, _lib.ptr(st), _lib.ptr(flag), _lib.ptr(W[0]), None, float(tol), s)
         p_out.zero_()
+        if float(W[0][0].item()) == 0.0:
+            return 0, 0.0
         self.r.copy_(self.b)
         self._apply_preconditioner(self.r, self.zv)
         self.q.copy_(self.zv)
-        rz = reduce(1, self.r, self.zv, w)
+        _lib.call("pf_vec_reduce", 1, _lib.ptr(self.r), _lib.ptr(self.zv), self.n, _lib.ptr(W[1]), s)
+        _lib.call("pf_mrcg_scalar", 1, _lib.ptr(st), _lib.ptr(flag), _lib.ptr(W[1]), None, 0.0, s)
         it, rel = 0, 1.0
+        host = torch.empty(8, dtype=torch.float64, pin_memory=True)
         for it in range(1, max_iter + 1):
             mro.apply_A(self.grid, self.q, bface, self.Ap)
-            qAq = reduce(1, self.q, self.Ap, w)
-            if qAq <= 0.0:
-                break
-            alpha = rz / qAq
-            axpby(alpha, self.q, 1.0, p_out)
+            _lib.call("pf_vec_reduce", 1, _lib.ptr(self.q), _lib.ptr(self.Ap), self.n, _lib.ptr(W[2]), s)
+            _lib.call("pf_mrcg_scalar", 2, _lib.ptr(st), _lib.ptr(flag), _lib.ptr(W[2]), None, 0.0, s)
+            _lib.call("pf_vec_axpby_dev", _lib.ptr(st), 1, 1.0, _lib.ptr(self.q), -1, _lib.ptr(p_out), self.n,
+                      _lib.ptr(flag), s)
             self.r_prev.copy_(self.r)
-            axpby(-alpha, self.Ap, 1.0, self.r)
-            rel = reduce(2, self.r, None, w) / bnorm
+            _lib.call("pf_vec_axpby_dev", _lib.ptr(st), 1, -1.0, _lib.ptr(self.Ap), -1, _lib.ptr(self.r), self.n,
+                      _lib.ptr(flag), s)
+            _lib.call("pf_vec_reduce", 2, _lib.ptr(self.r), None, self.n, _lib.ptr(W[3]), s)
+            _lib.call("pf_mrcg_scalar", 3, _lib.ptr(st), _lib.ptr(flag), _lib.ptr(W[3]), None, 0.0, s)
+            host.copy_(st)  # one read per iteration: the scalars and the stop flag (st[7])
+            rel = float(host[3])
+            stop = int(host[7])
+            if stop == 3:  # stopped at the previous iteration's beta
+                it -= 1
+            if stop:
+                break
             if verbose and it % 50 == 0:
                 print(f"    cg {it:5d}  {rel:.3e}")
-            if rel < tol:
-                break
             self._apply_preconditioner(self.r, self.zv)
-            rz_new = reduce(1, self.r, self.zv, w)
-            if rz == 0.0 or rz_new <= 0.0:
-                break
-            beta = max((rz_new - reduce(1, self.r_prev, self.zv, w)) / rz, 0.0)
-            axpby(1.0, self.zv, beta, self.q)
-            rz = rz_new
-        remove_mean(p_out, w)
+            _lib.call("pf_vec_reduce", 1, _lib.ptr(self.r), _lib.ptr(self.zv), self.n, _lib.ptr(W[4]), s)
+            _lib.call("pf_vec_reduce", 1, _lib.ptr(self.r_prev), _lib.ptr(self.zv), self.n, _lib.ptr(W[5]), s)
+            _lib.call("pf_mrcg_scalar", 4, _lib.ptr(st), _lib.ptr(flag), _lib.ptr(W[4]), _lib.ptr(W[5]), 0.0, s)
+            _lib.call("pf_vec_axpby_dev", _lib.ptr(st), -1, 1.0, _lib.ptr(self.zv), 2, _lib.ptr(self.q), self.n,
+                      _lib.ptr(flag), s)
+        remove_mean(p_out, self._work)
         return it, rel

@@ -424,6 +424,22 @@ int pf_msbg_cover(const pf_msbg_grid* g, int32_t* hits, pf_stream_t stream);
 size_t pf_vec_reduce_bytes(void);
 int pf_vec_reduce(int32_t op, const float* x, const float* y, int64_t n, double* work, pf_stream_t stream);
 int pf_vec_axpby(float a, const float* x, float b, float* y, int64_t n, pf_stream_t stream);
+/* Device-resident scalars of the flexible PCG (mr_solver.py:118-165), so a
+ * solve reads one state record per iteration instead of every reduction.
+ * pf_vec_axpby_dev: y = a x + b y with a = a_sign * st[ia] (a = a_sign
+ * when ia < 0) and b = st[ib] (1 when ib < 0), rounded to float; a no-op
+ * once flag[0] != 0 (flag may be NULL).  pf_vec_remove_mean: x -= mean(x)
+ * as _remove_mean (mr_multigrid.py:181-184), the FP64 sum kept on the
+ * device (work as for pf_vec_reduce).  pf_mrcg_scalar: st = {rz, alpha,
+ * beta, rel, bnorm, tol, qAq, flag[0] as a double}; op 0 (init: bnorm = *a, tol), 1 (rz = *a),
+ * 2 (alpha = rz / *a, or flag 1 when *a <= 0), 3 (rel = *a / bnorm, flag 2
+ * below tol), 4 (beta = max((*a - *b) / rz, 0), rz = *a, or flag 3 when
+ * rz == 0 or *a <= 0); ops 2-4 do nothing once flag[0] != 0. */
+int pf_vec_axpby_dev(const double* st, int32_t ia, float a_sign, const float* x, int32_t ib, float* y, int64_t n,
+                     const int32_t* flag, pf_stream_t stream);
+int pf_vec_remove_mean(float* x, int64_t n, double* work, pf_stream_t stream);
+int pf_mrcg_scalar(int32_t op, double* st, int32_t* flag, const double* a, const double* b, double tol,
+                   pf_stream_t stream);
 /* _jacobi (mr_solver.py:64-69): z = r / diag where diag > 0, else 0. */
 int pf_vec_jacobi(const float* r, const float* diag, float* z, int64_t n, pf_stream_t stream);
 
