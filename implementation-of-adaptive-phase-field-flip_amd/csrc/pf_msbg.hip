@@ -148,11 +148,13 @@ __global__ __launch_bounds__(kCoarsestThreads) void k_msbg_coarsest(MGrid g, flo
     unsigned short* nbr = reinterpret_cast<unsigned short*>(dd + n);
     unsigned char* cnt = reinterpret_cast<unsigned char*>(nbr + (size_t)n * kRowMax);
     unsigned char* col = cnt + n;
+    // entry k of every row is contiguous (k * n + a): a wave's loads of one
+    // entry hit consecutive LDS words, free of bank conflicts
     for (int a = threadIdx.x; a < n; a += blockDim.x) {
         int k = 0;
         for_each_coupling(g, bface, a, [&](int nb, float t) {
-            coef[a * kRowMax + k] = t;
-            nbr[a * kRowMax + k] = (unsigned short)nb;
+            coef[k * n + a] = t;
+            nbr[k * n + a] = (unsigned short)nb;
             ++k;
         });
         cnt[a] = (unsigned char)k;
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(kCoarsestThreads) void k_msbg_coarsest(MGrid g, flo
                         const float pa = pin[a];
                         float acc = 0.0f;
                         const int m = cnt[a];
-                        for (int k = 0; k < m; ++k) acc += coef[a * kRowMax + k] * (pa - pin[nbr[a * kRowMax + k]]);
+                        for (int k = 0; k < m; ++k) acc += coef[k * n + a] * (pa - pin[nbr[k * n + a]]);
                         v = v + omega * (bb[a] - acc) / d;
                     }
                 }
