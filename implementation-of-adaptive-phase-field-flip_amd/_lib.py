@@ -163,6 +163,9 @@ SIGNATURES = {
     "pf_msbg_rbgs_active": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, P, P, I32, F32, P, P]),
     "pf_msbg_relax_update": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, P, I32, F32, P, P]),
     "pf_msbg_coarsest_max_cells": (I32, []),
+    "pf_msbg_coarse_op_max_cells": (I32, []),
+    "pf_msbg_coarse_op": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, I32, F32, P, P]),
+    "pf_msbg_coarse_apply": (ctypes.c_int, [P, P, P, I32, P]),
     "pf_msbg_coarsest_smooth": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), P, P, P, P, I32, F32, P]),
     "pf_msbg_coarsen_beta": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MsbgGrid), P, P, P]),
     "pf_msbg_restrict": (ctypes.c_int, [ctypes.POINTER(MsbgGrid), ctypes.POINTER(MsbgGrid), P, P, P]),

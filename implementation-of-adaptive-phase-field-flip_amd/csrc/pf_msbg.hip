@@ -191,6 +191,99 @@ __global__ __launch_bounds__(kCoarsestThreads) void k_msbg_coarsest(MGrid g, flo
     for (int a = threadIdx.x; a < n; a += blockDim.x) p[a] = pin[a];
 }
 
+// The coarsest level's smoothing as a matrix: `sweeps` red-black sweeps
+// from zero followed by mean removal are a linear map b -> p, so column j
+// is that map applied to the unit vector e_j.  One workgroup per column,
+// the sweeps in FP64 from LDS (couplings gathered as in k_msbg_coarsest);
+// op[j * n + t] is entry (t, j).  A V-cycle then applies it with one
+// matrix-vector product (k_msbg_coarse_apply) instead of the sweeps.
+constexpr int kMsbgCoarseOpMax = 512;
+
+__host__ __device__ inline size_t coarse_op_lds_bytes(int n) {
+    return (size_t)n * (kRowMax * (sizeof(float) + sizeof(unsigned short)) + 2 * sizeof(double) + sizeof(float) + 2);
+}
+
+__global__ __launch_bounds__(kCoarsestThreads) void k_msbg_coarse_op(MGrid g, const float* __restrict__ bface,
+                                                                     const float* __restrict__ diag, int sweeps,
+                                                                     double omega, double* __restrict__ op) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const int n = g.ncells, j = blockIdx.x;
+    double* p0 = reinterpret_cast<double*>(lds);
+    double* p1 = p0 + n;
+    float* coef = reinterpret_cast<float*>(p1 + n);
+    float* dd = coef + (size_t)n * kRowMax;
+    unsigned short* nbr = reinterpret_cast<unsigned short*>(dd + n);
+    unsigned char* cnt = reinterpret_cast<unsigned char*>(nbr + (size_t)n * kRowMax);
+    unsigned char* col = cnt + n;
+    __shared__ double sh[16];
+    __shared__ double shift;
+    for (int a = threadIdx.x; a < n; a += blockDim.x) {
+        int k = 0;
+        for_each_coupling(g, bface, a, [&](int nb, float t) {
+            coef[k * n + a] = t;
+            nbr[k * n + a] = (unsigned short)nb;
+            ++k;
+        });
+        cnt[a] = (unsigned char)k;
+        const int4 o = origin(g, a);
+        col[a] = (unsigned char)(((o.x >> o.w) + (o.y >> o.w) + (o.z >> o.w)) & 1);
+        dd[a] = diag[a];
+        p0[a] = 0.0;
+    }
+    __syncthreads();
+    double* pin = p0;
+    double* pout = p1;
+    for (int sw = 0; sw < sweeps; ++sw)
+        for (int c = 0; c < 2; ++c) {
+            for (int a = threadIdx.x; a < n; a += blockDim.x) {
+                double v = pin[a];
+                if (col[a] == c && dd[a] > 0.0f) {
+                    const double pa = pin[a];
+                    double acc = 0.0;
+                    const int m = cnt[a];
+                    for (int k = 0; k < m; ++k) acc += (double)coef[k * n + a] * (pa - pin[nbr[k * n + a]]);
+                    v = pa + omega * ((a == j ? 1.0 : 0.0) - acc) / (double)dd[a];
+                }
+                pout[a] = v;
+            }
+            __syncthreads();
+            double* t = pin;
+            pin = pout;
+            pout = t;
+        }
+    double sm[1] = {0.0};
+    for (int a = threadIdx.x; a < n; a += blockDim.x) sm[0] += pin[a];
+    block_sum<1>(sm, sh);
+    if (threadIdx.x == 0) shift = -(sm[0] / (double)n);
+    __syncthreads();
+    for (int a = threadIdx.x; a < n; a += blockDim.x) op[(int64_t)j * n + a] = pin[a] + shift;
+}
+
+// p = op b in FP64, rounded to float.  A workgroup takes 64 rows; wave w
+// sums the columns w, w + 8, w + 16, ... of them (a wave reads 64
+// consecutive entries of a column), and the 8 partials of a row are added
+// in wave order.
+__global__ __launch_bounds__(512) void k_msbg_coarse_apply(const double* __restrict__ op,
+                                                           const float* __restrict__ b, float* __restrict__ p,
+                                                           int n) {
+    __shared__ double bs[kMsbgCoarseOpMax];
+    __shared__ double part[8][64];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) bs[j] = (double)b[j];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int t = blockIdx.x * 64 + lane;
+    double acc = 0.0;
+    if (t < n)
+        for (int j = w; j < n; j += 8) acc += op[(int64_t)j * n + t] * bs[j];
+    part[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && t < n) {
+        double s = part[0][lane];
+        for (int q = 1; q < 8; ++q) s += part[q][lane];
+        p[t] = (float)s;
+    }
+}
+
 // ------------------------------------------------------------------ adaptive relaxation
 // Algorithm 2 of the paper as the reference runs it on a GPU
 // (adaptive_relax.py:1-241): red-black sweeps restricted to an active block
@@ -611,6 +704,40 @@ PF_API int pf_msbg_coarsest_smooth(const pf_msbg_grid* g, float* p, const float*
         PF_REQUIRE(e == hipSuccess, "pf_msbg_coarsest_smooth: cannot raise the LDS limit to %zu bytes", bytes);
     }
     k_msbg_coarsest<<<1, kCoarsestThreads, bytes, S(stream)>>>(mgrid(*g), p, b, bface, diag, sweeps, omega);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int32_t pf_msbg_coarse_op_max_cells(void) {
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    const int n = (int)(lim / coarse_op_lds_bytes(1));
+    return n < kMsbgCoarseOpMax ? n : kMsbgCoarseOpMax;
+}
+
+PF_API int pf_msbg_coarse_op(const pf_msbg_grid* g, const float* bface, const float* diag, int32_t sweeps,
+                             float omega, double* op, pf_stream_t stream) {
+    int rc = check_grid(g, "pf_msbg_coarse_op");
+    if (rc) return rc;
+    PF_REQUIRE(bface && diag && op && sweeps >= 0, "pf_msbg_coarse_op: bad arguments");
+    PF_REQUIRE(g->ncells <= pf_msbg_coarse_op_max_cells(), "pf_msbg_coarse_op: %d cells exceed the limit",
+               g->ncells);
+    const size_t bytes = coarse_op_lds_bytes(g->ncells);
+    if (bytes > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void*)k_msbg_coarse_op,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        PF_REQUIRE(e == hipSuccess, "pf_msbg_coarse_op: cannot raise the LDS limit to %zu bytes", bytes);
+    }
+    k_msbg_coarse_op<<<g->ncells, kCoarsestThreads, bytes, S(stream)>>>(mgrid(*g), bface, diag, sweeps,
+                                                                         (double)omega, op);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
+PF_API int pf_msbg_coarse_apply(const double* op, const float* b, float* p, int32_t n, pf_stream_t stream) {
+    PF_REQUIRE(op && b && p && n > 0 && n <= kMsbgCoarseOpMax, "pf_msbg_coarse_apply: bad arguments");
+    k_msbg_coarse_apply<<<cdiv(n, 64), 512, 0, S(stream)>>>(op, b, p, n);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }

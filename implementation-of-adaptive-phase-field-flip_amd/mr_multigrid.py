@@ -83,8 +83,20 @@ class MRMultigrid:
             self.adaptive = [AdaptiveSmoother(lv.grid, omega=omega, n_iter_max=n_iter_max, theta_sigma=theta_sigma)
                              for lv in self.levels[:-1]]
         self._record = False
-        # the coarsest sweeps run in one workgroup from LDS when they fit
-        self.coarsest_in_lds = self.levels[-1].n <= int(_lib.load().pf_msbg_coarsest_max_cells())
+        # The coarsest level: its sweeps (from zero) and mean removal as an
+        # FP64 matrix built once per set of coefficients ("matrix"), or the
+        # sweeps in one workgroup from LDS ("lds"), or launched half-sweeps
+        # ("launched"), whichever the level's size allows first.
+        lib = _lib.load()
+        nc = self.levels[-1].n
+        if nc <= int(lib.pf_msbg_coarse_op_max_cells()):
+            self.coarsest_mode = "matrix"
+        elif nc <= int(lib.pf_msbg_coarsest_max_cells()):
+            self.coarsest_mode = "lds"
+        else:
+            self.coarsest_mode = "launched"
+        self._coarse_op = (torch.empty(nc * nc, dtype=torch.float64, device=grid.dev)
+                           if self.coarsest_mode == "matrix" else None)
 
     def build_hierarchy(self, bface0):
         """Level-0 coefficients, then coarsen down the hierarchy (mr_multigrid.py:187-198)."""
@@ -96,6 +108,10 @@ class MRMultigrid:
                 _lib.call("pf_msbg_coarsen_beta", lv.grid.ref(), f.grid.ref(), _lib.ptr(f.bface),
                           _lib.ptr(lv.bface), _lib.stream())
             mro.compute_diag(lv.grid, lv.bface, lv.diag)
+        if self._coarse_op is not None:
+            c = self.levels[-1]
+            _lib.call("pf_msbg_coarse_op", c.grid.ref(), _lib.ptr(c.bface), _lib.ptr(c.diag), self.coarsest_sweeps,
+                      self.omega, _lib.ptr(self._coarse_op), _lib.stream())
 
     def _smooth(self, lv: _MGLevel, n_sweeps, reverse=False, index=None):
         """n_sweeps red-black sweeps, or the adaptive relaxation loop on a
@@ -113,7 +129,10 @@ class MRMultigrid:
         """mr_multigrid.py:213-235."""
         lv = self.levels[i]
         if i == self.nlev - 1:
-            if self.coarsest_in_lds:
+            if self.coarsest_mode == "matrix" and self._coarse_op is not None:
+                _lib.call("pf_msbg_coarse_apply", _lib.ptr(self._coarse_op), _lib.ptr(lv.b), _lib.ptr(lv.p), lv.n,
+                          _lib.stream())
+            elif self.coarsest_mode in ("matrix", "lds"):
                 _lib.call("pf_msbg_coarsest_smooth", lv.grid.ref(), _lib.ptr(lv.p), _lib.ptr(lv.b),
                           _lib.ptr(lv.bface), _lib.ptr(lv.diag), self.coarsest_sweeps, self.omega, _lib.stream())
             else:

@@ -397,6 +397,17 @@ int pf_msbg_relax_update(const pf_msbg_grid* g, const float* delta, const float*
 int32_t pf_msbg_coarsest_max_cells(void);
 int pf_msbg_coarsest_smooth(const pf_msbg_grid* g, float* p, const float* b, const float* bface, const float* diag,
                             int32_t sweeps, float omega, pf_stream_t stream);
+/* The coarsest level's smoothing as a matrix (no counterpart: the reference
+ * sweeps): `sweeps` red-black sweeps from zero plus mean removal are linear
+ * in b, so pf_msbg_coarse_op writes op[j * n + t] = entry (t, j) of that
+ * map (FP64, one workgroup per column, n = g->ncells <=
+ * pf_msbg_coarse_op_max_cells()), once per set of face coefficients; then
+ * pf_msbg_coarse_apply gives p = op b (FP64 sums rounded to float) per
+ * V-cycle. */
+int32_t pf_msbg_coarse_op_max_cells(void);
+int pf_msbg_coarse_op(const pf_msbg_grid* g, const float* bface, const float* diag, int32_t sweeps, float omega,
+                      double* op, pf_stream_t stream);
+int pf_msbg_coarse_apply(const double* op, const float* b, float* p, int32_t n, pf_stream_t stream);
 /* coarsen_beta (mr_multigrid.py:52-89): coarse face = mean of the 2x2 fine
  * faces it covers. */
 int pf_msbg_coarsen_beta(const pf_msbg_grid* coarse, const pf_msbg_grid* fine, const float* bfine, float* bcoarse,

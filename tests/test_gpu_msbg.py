@@ -223,11 +223,41 @@ def test_coarsest_lds_smoother_equals_launched_sweeps(mods, nb, n0, pattern):
     z1, z2 = g.zeros(), g.zeros()
     mg2 = mods["mr_multigrid"].MRMultigrid(g, min_cells=16)
     mg2.build_hierarchy(t32(rng.uniform(0.001, 1.0, g.ncells * 3)))
-    assert mg2.coarsest_in_lds
+    mg2.coarsest_mode = "lds"
     mg2.precondition(r, z1)
-    mg2.coarsest_in_lds = False
+    mg2.coarsest_mode = "launched"
     mg2.precondition(r, z2)
     assert torch.equal(z1, z2)
+
+
+def test_coarsest_matrix_is_the_sweeps(mods):
+    """The FP64 coarsest-level matrix applied to b equals the FP32 sweeps
+    from zero plus mean removal, to FP32 rounding, on a graded grid; the
+    V-cycle built on it matches the launched-sweeps V-cycle to 1e-5."""
+    say("coarsest matrix")
+    g, _ = pair(mods, (2, 2, 2), 4, 1 / 8, 1, "slab")
+    mg = mods["mr_multigrid"].MRMultigrid(g, min_cells=10 ** 6)
+    assert mg.coarsest_mode == "matrix"
+    rng = np.random.default_rng(6)
+    mg.build_hierarchy(t32(rng.uniform(0.001, 1.0, g.ncells * 3)))
+    c = mg.levels[-1]
+    b = rng.standard_normal(c.n).astype(np.float32)
+    c.b.copy_(t32(b))
+    _lib = mods["_lib"]
+    _lib.call("pf_msbg_coarse_apply", _lib.ptr(mg._coarse_op), _lib.ptr(c.b), _lib.ptr(c.p), c.n, _lib.stream())
+    zm = np_(c.p).astype(np.float64)
+    c.p.zero_()
+    mg._smooth(c, mg.coarsest_sweeps)
+    zs = np_(c.p).astype(np.float64)
+    zs -= zs.mean()
+    assert np.abs(zm - zs).max() < 1e-5 * np.abs(zs).max()
+    assert abs(zm.mean()) < 1e-6 * np.abs(zm).max()
+    r = t32(rng.standard_normal(g.ncells))
+    z1, z2 = g.zeros(), g.zeros()
+    mg.precondition(r, z1)
+    mg.coarsest_mode = "launched"
+    mg.precondition(r, z2)
+    assert np.abs(np_(z1) - np_(z2)).max() < 1e-5 * np.abs(np_(z2)).max()
 
 
 # ---------------------------------------------------------------- adaptive relaxation
