@@ -613,6 +613,8 @@ static int check_grid(const pf_msbg_grid* g, const char* who) {
     PF_REQUIRE(g, "%s: null grid", who);
     PF_REQUIRE(g->ncells > 0 && g->nslots > 0 && g->n0 == (1 << g->sh0) && g->h0 > 0.f, "%s: bad grid", who);
     PF_REQUIRE(g->level && g->slot && g->blk && g->blev && g->coff && g->cslot, "%s: null grid array", who);
+    // face fields are indexed 3 * cell + axis in 32-bit arithmetic
+    PF_REQUIRE(g->ncells <= kMsbgMaxCells, "%s: %d cells exceed the limit of %d", who, g->ncells, kMsbgMaxCells);
     return PF_OK;
 }
 

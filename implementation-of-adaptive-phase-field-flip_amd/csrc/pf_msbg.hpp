@@ -7,6 +7,9 @@
 
 namespace pf {
 
+// face fields hold 3 entries per cell, indexed in 32-bit arithmetic
+constexpr int kMsbgMaxCells = 715827882;  // floor((2^31 - 1) / 3)
+
 struct MGrid {
     int nbx, nby, nbz, n0, sh0, nslots, ncells;
     float h0;

@@ -47,6 +47,16 @@ def test_error_path_without_gpu():
     assert b"r_p == dx" in lib.pf_last_error()
     rc = lib.pf_apply_A(0, 4, 4, None, None, None, None, None, None, None)
     assert rc == -1 and b"bad grid" in lib.pf_last_error()
+    # an MSBG whose face fields would overflow 32-bit indexing is refused
+    # before any device call (the pointers are never dereferenced)
+    import ctypes
+    g = L.MsbgGrid()
+    g.nbx = g.nby = g.nbz = 64
+    g.n0, g.sh0, g.nslots, g.ncells, g.h0 = 16, 4, 64 ** 3, 715827883, 1.0
+    for f in ("level", "slot", "blk", "blev", "coff", "cslot"):
+        setattr(g, f, 16)
+    rc = lib.pf_msbg_apply(ctypes.byref(g), None, ctypes.c_void_p(16), ctypes.c_void_p(16), 1, None)
+    assert rc == -1 and b"exceed the limit" in lib.pf_last_error()
 
 
 C_LAYOUT = r"""
