@@ -36,6 +36,7 @@ class MRSolver:
         self._works = [torch.zeros(nw, dtype=torch.float64, device=grid.dev) for _ in range(6)]
         self._st = torch.zeros(8, dtype=torch.float64, device=grid.dev)
         self._flag = torch.zeros(2, dtype=torch.int32, device=grid.dev)
+        self._host = torch.empty(8, dtype=torch.float64, pin_memory=True)  # the per-iteration state read
 
     def prepare(self, bface):
         mro.compute_diag(self.grid, bface, self.diag)
@@ -73,7 +74,7 @@ class MRSolver:
         _lib.call("pf_vec_reduce", 1, _lib.ptr(self.r), _lib.ptr(self.zv), self.n, _lib.ptr(W[1]), s)
         _lib.call("pf_mrcg_scalar", 1, _lib.ptr(st), _lib.ptr(flag), _lib.ptr(W[1]), None, 0.0, s)
         it, rel = 0, 1.0
-        host = torch.empty(8, dtype=torch.float64, pin_memory=True)
+        host = self._host
         for it in range(1, max_iter + 1):
             mro.apply_A(self.grid, self.q, bface, self.Ap)
             _lib.call("pf_vec_reduce", 1, _lib.ptr(self.q), _lib.ptr(self.Ap), self.n, _lib.ptr(W[2]), s)
