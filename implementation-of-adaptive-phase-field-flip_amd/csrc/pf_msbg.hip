@@ -15,6 +15,10 @@
 // resolution jump; reading the old values there makes the half-sweep
 // deterministic.  Restriction gathers each coarse cell's children in a fixed
 // order instead of adding them with atomics.
+//
+// The cell-parallel operator kernels take their cell range through
+// xcd_block: the 16 workgroups of a 16^3 block then run on one XCD and the
+// +-z neighbour rows they read from each other stay in that XCD's L2.
 #include "pf_msbg.hpp"
 
 namespace pf {
@@ -92,7 +96,7 @@ __device__ inline float apply_row(const MGrid& g, const float* __restrict__ p, c
 __global__ __launch_bounds__(256) void k_msbg_apply(MGrid g, const float* __restrict__ p,
                                                     const float* __restrict__ bface, float* __restrict__ out,
                                                     int want_diag) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (a >= g.ncells) return;
     out[a] = apply_row(g, p, bface, a, want_diag != 0);
 }
@@ -100,7 +104,7 @@ __global__ __launch_bounds__(256) void k_msbg_apply(MGrid g, const float* __rest
 __global__ __launch_bounds__(256) void k_msbg_residual(MGrid g, const float* __restrict__ p,
                                                        const float* __restrict__ b,
                                                        const float* __restrict__ bface, float* __restrict__ r) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (a >= g.ncells) return;
     r[a] = b[a] - apply_row(g, p, bface, a, false);
 }
@@ -110,7 +114,7 @@ __global__ __launch_bounds__(256) void k_msbg_residual(MGrid g, const float* __r
 __global__ __launch_bounds__(256) void k_msbg_rbgs(MGrid g, const float* __restrict__ pin, float* __restrict__ pout,
                                                    const float* __restrict__ b, const float* __restrict__ bface,
                                                    const float* __restrict__ diag, int colour, float omega) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (a >= g.ncells) return;
     const int4 o = origin(g, a);
     float v = pin[a];
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(256) void k_msbg_rbgs_active(MGrid g, const float* 
                                                           const float* __restrict__ diag,
                                                           const int* __restrict__ active, int colour, float omega,
                                                           float* __restrict__ delta) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (a >= g.ncells) return;
     float v = pin[a];
     if (active[g.cslot[a]]) {
