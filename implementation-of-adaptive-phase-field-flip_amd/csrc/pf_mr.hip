@@ -267,8 +267,11 @@ __global__ __launch_bounds__(256) void k_mr_cell_scatter(const int* __restrict__
 // atomic per non-zero channel at the end.  A contribution whose slot holds
 // another cell goes to global memory directly.  Integer sums, so the result
 // equals the uncached kernel's bit for bit.
-constexpr int kP2GSlots = 1024;
-constexpr int kP2GPerBlock = 512;  // particles per workgroup (2 per thread)
+// 512 slots (34 KB of LDS) let four workgroups share a CU; 1024 slots with
+// 512 particles per workgroup ran 6 % slower at two workgroups per CU
+constexpr int kP2GSlots = 512;
+constexpr int kP2GPerBlock = 256;  // particles per workgroup (1 per thread)
+static_assert(kP2GSlots == 1 << 9, "the slot hash keeps 9 bits");
 
 __global__ __launch_bounds__(256) void k_mr_p2g_cached(MGrid g, const float* __restrict__ x,
                                                        const float* __restrict__ v, const int* __restrict__ type,
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(256) void k_mr_p2g_cached(MGrid g, const float* __r
         if (p0 + q >= n) break;
         const int64_t p = perm[p0 + q];
         splat_particle(g, x, v, type, lev, p, c, [&](int addr, const long long* val, unsigned mask) {
-            const int s = (int)(((unsigned)addr * 2654435761u) >> 22);  // 10-bit multiplicative hash
+            const int s = (int)(((unsigned)addr * 2654435761u) >> 23);  // 9-bit multiplicative hash
             int t = tag[s];
             if (t != addr && t == -1) {
                 t = atomicCAS(tag + s, -1, addr);
