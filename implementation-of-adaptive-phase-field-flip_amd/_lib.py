@@ -141,10 +141,12 @@ SIGNATURES = {
     "pf_p2g": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32, F32,
                               P, P, P, P, P, P, P, P, P, P, SZ, P]),
     "pf_phase_beta_forces": (ctypes.c_int, [ctypes.POINTER(PhaseArgs), P]),
+    "pf_build_rhs": (ctypes.c_int, [I32, I32, I32, PP, P, P, F32, F32, F32, F32, P]),
     "pf_mg_scalar_bytes": (SZ, []),
     "pf_mg_setup": (ctypes.c_int, [ctypes.POINTER(MG)]),
     "pf_mg_build": (ctypes.c_int, [ctypes.POINTER(MG), P, P, P, P]),
     "pf_mg_precondition": (ctypes.c_int, [ctypes.POINTER(MG), P, P, P]),
+    "pf_mg_vcycle": (ctypes.c_int, [ctypes.POINTER(MG), P, P, P]),
     "pf_mgpcg_solve": (ctypes.c_int, [ctypes.POINTER(MG), P, P, F64, I32,
                                       ctypes.POINTER(I32), ctypes.POINTER(F64), P]),
     "pf_mg_time_smoother": (ctypes.c_int, [ctypes.POINTER(MG), I32, I32, ctypes.POINTER(F32), P]),

@@ -383,6 +383,26 @@ PF_API int pf_phase_beta_forces(const pf_phase_args* args, pf_stream_t stream) {
     return PF_OK;
 }
 
+// build_rhs alone: the grid pass above with only its rhs stage, which reads
+// vel and writes nothing but rhs
+PF_API int pf_build_rhs(int32_t nx, int32_t ny, int32_t nz, const float* const vel[3], const float* cell_wsum,
+                        float* rhs, float dx, float dt, float k_div, float rho_tilde_0, pf_stream_t stream) {
+    PF_REQUIRE(vel, "pf_build_rhs: null vel");
+    pf_phase_args a{};
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.flags = PF_PH_RHS;
+    for (int q = 0; q < 3; ++q) a.vel[q] = const_cast<float*>(vel[q]);
+    a.cell_wsum = cell_wsum;
+    a.rhs = rhs;
+    a.dx = dx;
+    a.dt = dt;
+    a.k_div = k_div;
+    a.rho_tilde_0 = rho_tilde_0;
+    return pf_phase_beta_forces(&a, stream);
+}
+
 PF_API int pf_project(const float* p, float* const beta[3], float* const vel[3],
                       float* const vel_star[3], float* const dvel[3], int nx, int ny, int nz,
                       float dx, float dt, float* max_div, pf_stream_t stream) {

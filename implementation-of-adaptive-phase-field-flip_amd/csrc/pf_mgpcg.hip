@@ -1868,6 +1868,22 @@ PF_API int pf_mg_precondition(const pf_mg* mg, const float* r, float* z, pf_stre
     }
 }
 
+// One V-cycle from a zero guess, without the final mean removal of
+// pf_mg_precondition.  FP32 hierarchies only: the 16-bit V-cycles keep their
+// level-0 iterate in 16-bit storage with a scale that pf_mg_precondition
+// and pf_mgpcg_solve undo.
+PF_API int pf_mg_vcycle(const pf_mg* mg, const float* r, float* z, pf_stream_t stream) {
+    int rc = check_mg(mg);
+    if (rc) return rc;
+    PF_REQUIRE(r && z, "pf_mg_vcycle: null vector");
+    PF_REQUIRE(mg->precision == 0, "pf_mg_vcycle: FP32 hierarchies only (precision %d)", mg->precision);
+    // level 0 ping-pongs between z and the hierarchy's own p
+    PF_REQUIRE((const void*)z != (const void*)r && (const void*)z != (const void*)mg->lev[0].p &&
+                   (const void*)r != (const void*)mg->lev[0].p,
+               "pf_mg_vcycle: r, z and the level-0 p buffer must be distinct");
+    return vcycle<float>(*mg, r, z, nullptr, nullptr, S(stream));
+}
+
 PF_API int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol, int32_t max_iter,
                           int32_t* it_out, double* rel_out, pf_stream_t stream) {
     int rc = check_mg(mg);

@@ -138,6 +138,13 @@ class MGPCG:
         _lib.call("pf_mg_precondition", ctypes.byref(self._mg), _lib.ptr(r_in), _lib.ptr(z_out),
                   _lib.stream())
 
+    def vcycle(self, r_in, z_out):
+        """z = one V-cycle from a zero guess on the right-hand side r, the
+        coarsest level's mean removal included but not the final one
+        (``_vcycle`` on ``levels[0].b = r``, pfflip/poisson.py:323-343).
+        FP32 hierarchies only; ``z_out`` must not be ``levels[0].p``."""
+        _lib.call("pf_mg_vcycle", ctypes.byref(self._mg), _lib.ptr(r_in), _lib.ptr(z_out), _lib.stream())
+
     # ---------------------------------------------------------------- solve
     def solve(self, b, p_out, *, tol=1e-4, max_iter=200, verbose=False):
         """Flexible PCG with Polak-Ribiere beta (pfflip/poisson.py:355-409).
@@ -171,3 +178,16 @@ def apply_A(p, bx, by, bz, diag, out):
     _lib.call("pf_apply_A", nx, ny, nz, _lib.ptr(p), _lib.ptr(bx), _lib.ptr(by), _lib.ptr(bz),
               _lib.ptr(diag), _lib.ptr(out), _lib.stream())
     return out
+
+
+def build_rhs(vel, rhs, dx, dt, cell_wsum=None, k_div=0.0, rho_tilde_0=1.0):
+    """``rhs = -(dx/dt) sum_a (u_a[i+1] - u_a[i])``, minus
+    ``dx^2 k_div max((cell_wsum - rho_tilde_0)/rho_tilde_0, 0)/dt`` when
+    ``k_div > 0`` (the ``build_rhs`` kernel, pfflip/kernels.py:338-363).
+    ``vel`` is the three face arrays, ``rhs`` the cell array it writes."""
+    nx, ny, nz = rhs.shape
+    if k_div > 0.0 and cell_wsum is None:
+        raise ValueError("build_rhs: k_div > 0 needs cell_wsum")
+    _lib.call("pf_build_rhs", nx, ny, nz, _lib.ptr3(vel), _lib.ptr(cell_wsum), _lib.ptr(rhs), float(dx),
+              float(dt), float(k_div), float(rho_tilde_0), _lib.stream())
+    return rhs

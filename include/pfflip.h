@@ -145,6 +145,12 @@ typedef struct pf_phase_args {
 } pf_phase_args;
 
 int pf_phase_beta_forces(const pf_phase_args* args, pf_stream_t stream);
+/* build_rhs (kernels.py:338-363) alone: rhs = -(dx/dt) sum_a (u_a,hi -
+ * u_a,lo), minus dx^2 k_div max((cell_wsum - rho_tilde_0)/rho_tilde_0, 0)/dt
+ * when k_div > 0 (cell_wsum may be NULL otherwise).  The same pass as
+ * pf_phase_beta_forces with flags = PF_PH_RHS; vel is only read. */
+int pf_build_rhs(int32_t nx, int32_t ny, int32_t nz, const float* const vel[3], const float* cell_wsum,
+                 float* rhs, float dx, float dt, float k_div, float rho_tilde_0, pf_stream_t stream);
 
 /* ---------------------------------------------------------------- MG-PCG
  * Replaces MGPCG (pfflip/poisson.py:226-409).  Levels halve while
@@ -203,6 +209,10 @@ int pf_mg_build(const pf_mg* mg, const float* bx, const float* by, const float* 
                 pf_stream_t stream);
 /* precondition (poisson.py:345-352): z = V-cycle(r) minus its mean. */
 int pf_mg_precondition(const pf_mg* mg, const float* r, float* z, pf_stream_t stream);
+/* _vcycle (poisson.py:323-343): z = one V-cycle from a zero guess on the
+ * right-hand side r (the coarsest level's mean removal included, no final
+ * mean removal).  FP32 hierarchies only; r, z and lev[0].p distinct. */
+int pf_mg_vcycle(const pf_mg* mg, const float* r, float* z, pf_stream_t stream);
 /* solve (poisson.py:355-409): flexible PCG, Polak-Ribiere beta clamped at 0,
  * stop when max|r|/max|b| < tol (recursive residual, b mean-removed).
  * Writes x (mean removed); *it_out / *rel_out are host pointers.  Blocks. */

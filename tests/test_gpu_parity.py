@@ -323,6 +323,85 @@ def test_precondition_matches_oracle_vcycle(pf, res):
     assert rel_err(np_(z), zr) < 2e-5
 
 
+@pytest.mark.parametrize("res", [(16, 16, 16), (32, 16, 24), (12, 10, 14)])
+def test_vcycle_matches_oracle(pf, res):
+    """pf_mg_vcycle: one V-cycle without the final mean removal
+    (poisson.py:323-343) against the FP64 restatement; precondition equals it
+    minus its mean."""
+    say(f"pf_mg_vcycle {res}")
+    rng = np.random.default_rng(5)
+    bx, by, bz = random_beta(res, rng, 1000.0)
+    r = rng.standard_normal(res).astype(np.float32)
+    r -= r.mean()
+    solver = pf.MGPCG(res, "cuda:0", pre=3, post=3)
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    solver.build_hierarchy(g(bx), g(by), g(bz))
+    z = torch.zeros(res, device="cuda")
+    zp = torch.zeros(res, device="cuda")
+    solver.vcycle(g(r), z)
+    solver.precondition(g(r), zp)
+    o = O.MGPCGOracle(res, pre=3, post=3)
+    o.build_hierarchy(bx, by, bz)
+    L0 = o.levels[0]
+    L0.b = r.astype(np.float64)
+    L0.p = np.zeros(res)
+    o._vcycle(0)
+    zv = np_(z).astype(np.float64)
+    assert rel_err(np_(z), L0.p) < 2e-5
+    # the V-cycle of a mean-free r is not mean-free in general: the final
+    # mean removal of precondition is a real step
+    assert abs(zv.mean()) > 1e-9 * np.abs(zv).max()
+    np.testing.assert_allclose(np_(zp), zv - zv.mean(), rtol=0, atol=1e-6 * np.abs(zv).max())
+
+
+def test_vcycle_argument_errors(pf):
+    """pf_mg_vcycle refuses 16-bit hierarchies and buffers that alias its own
+    level-0 work array (host checks, before any launch)."""
+    say("pf_mg_vcycle errors")
+    res = (8, 8, 8)
+    r = torch.zeros(res, device="cuda")
+    z = torch.zeros(res, device="cuda")
+    s32 = pf.MGPCG(res, "cuda:0")
+    with pytest.raises(RuntimeError, match="distinct"):
+        s32.vcycle(r, s32.levels[0].p)
+    s16 = pf.MGPCG(res, "cuda:0", precision="bf16")
+    with pytest.raises(RuntimeError, match="FP32 hierarchies only"):
+        s16.vcycle(r, z)
+
+
+def test_build_rhs(pf):
+    """pf_build_rhs (kernels.py:338-363) against the FP64 restatement, with
+    and without the k_div term, plus closed forms: a uniform field has zero
+    divergence exactly, u = x gives rhs = -dx/dt * (dx / dx) per cell."""
+    say("pf_build_rhs")
+    from importlib import import_module
+    poisson = import_module(pf.__name__ + ".poisson")
+    res = (12, 10, 14)
+    rng = np.random.default_rng(9)
+    shapes = O.face_shapes(res)
+    vel = [rng.standard_normal(sh).astype(np.float32) for sh in shapes]
+    wsum = (rng.random(res) * 2.0).astype(np.float32)
+    dx, dt, rho0 = 1.0 / 16, 0.01, 0.9
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    rhs = torch.empty(res, device="cuda")
+    poisson.build_rhs([g(v) for v in vel], rhs, dx, dt)
+    ref = O.build_rhs([v.astype(np.float64) for v in vel], None, dx, dt, 1.0, 0.0)
+    assert rel_err(np_(rhs), ref) < 1e-6
+    poisson.build_rhs([g(v) for v in vel], rhs, dx, dt, cell_wsum=g(wsum), k_div=0.5, rho_tilde_0=rho0)
+    ref = O.build_rhs([v.astype(np.float64) for v in vel], wsum.astype(np.float64), dx, dt, rho0, 0.5)
+    assert rel_err(np_(rhs), ref) < 1e-6
+    # closed forms
+    uni = [np.full(sh, 0.3 + ax, np.float32) for ax, sh in enumerate(shapes)]
+    poisson.build_rhs([g(v) for v in uni], rhs, dx, dt)
+    assert np.abs(np_(rhs)).max() == 0.0
+    ux = np.broadcast_to(np.arange(res[0] + 1, dtype=np.float32)[:, None, None] * dx, shapes[0]).copy()
+    lin = [ux, np.zeros(shapes[1], np.float32), np.zeros(shapes[2], np.float32)]
+    poisson.build_rhs([g(v) for v in lin], rhs, dx, dt)
+    np.testing.assert_allclose(np_(rhs), -(dx / dt) * dx, rtol=1e-5)
+    with pytest.raises(ValueError):
+        poisson.build_rhs([g(v) for v in vel], rhs, dx, dt, k_div=0.5)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("contrast", [1.0, 1000.0])
 def test_mgpcg_solves(pf, contrast, precision):
