@@ -467,14 +467,14 @@ def run_adaptive128(pf, torch, args):
     e1.record()
     torch.cuda.synchronize()
     sweep_ms = e0.elapsed_time(e1) / reps
-    rl = roofline(f"k_msbg_rbgs level-0 half-sweep ({g.ncells} cells)", msbg_sweep_bytes(g.ncells), sweep_ms)
+    rl = roofline(f"k_msbg_rbgs_pairs level-0 half-sweep ({g.ncells} cells)", msbg_sweep_bytes(g.ncells), sweep_ms)
     workload = "adaptive dam break y-up 128^3 MSBG n0=16 levels 0-2, 8 ppc, 1000:1, MG-PCG tol 1e-4"
     tr = measured_traffic(workload, "fp32")
     if tr is not None:
         # HBM bytes per cell of the level-0 half-sweep, measured on this
         # workload's grid; the sweep streams its arrays once, so the figure
         # scales with the cell count
-        m = tr["kernels"]["k_msbg_rbgs level-0 half-sweep"]
+        m = tr["kernels"]["k_msbg_rbgs_pairs level-0 half-sweep"]
         rl["traffic"] = m["bytes_per_cell"] * g.ncells
         rl["traffic_source"] = tr["source"]
     return {

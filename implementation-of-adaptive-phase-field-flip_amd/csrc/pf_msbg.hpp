@@ -65,11 +65,9 @@ __device__ inline MCell locate(const MGrid& g, int fx, int fy, int fz) {
 __device__ inline int4 origin(const MGrid& g, int addr) {
     const int s = g.cslot[addr];
     const int l = g.blev[s];
-    const int n = g.n0 >> l;
-    int r = addr - g.coff[s];
-    const int i = r / (n * n);
-    r -= i * n * n;
-    const int j = r / n, k = r - j * n;
+    const int n = g.n0 >> l, ln = g.sh0 - l;
+    const int r = addr - g.coff[s];
+    const int i = r >> (2 * ln), j = (r >> ln) & (n - 1), k = r & (n - 1);
     const int b = g.blk[s];
     const int bz = b % g.nbz, by = (b / g.nbz) % g.nby, bx = b / (g.nbz * g.nby);
     return make_int4((bx << g.sh0) + (i << l), (by << g.sh0) + (j << l), (bz << g.sh0) + (k << l), l);
@@ -114,10 +112,9 @@ __device__ inline MCell locate_near(const MGrid& g, const BlockRef& r, int fx, i
 // origin() and the cell's block in one pass
 __device__ inline int4 origin_ref(const MGrid& g, int addr, BlockRef& r) {
     r = block_ref(g, g.cslot[addr]);
-    int q = addr - r.base;
-    const int i = q / (r.n * r.n);
-    q -= i * r.n * r.n;
-    const int j = q / r.n, k = q - j * r.n;
+    // n = 2^(sh0 - level): the local indices are bit fields of the offset
+    const int q = addr - r.base, ln = g.sh0 - r.level;
+    const int i = q >> (2 * ln), j = (q >> ln) & (r.n - 1), k = q & (r.n - 1);
     return make_int4(r.bx0 + (i << r.level), r.by0 + (j << r.level), r.bz0 + (k << r.level), r.level);
 }
 

@@ -65,7 +65,7 @@ def test_measured_traffic_records():
     assert head is not None and "k_rbgs_full level-0 sweep (128^3)" in head["kernels"]
     ad = bench.measured_traffic(
         "adaptive dam break y-up 128^3 MSBG n0=16 levels 0-2, 8 ppc, 1000:1, MG-PCG tol 1e-4", "fp32")
-    m = ad["kernels"]["k_msbg_rbgs level-0 half-sweep"]
+    m = ad["kernels"]["k_msbg_rbgs_pairs level-0 half-sweep"]
     # copy-calibrated bytes are at least the algorithmic 32 B/cell
     assert m["bytes_per_cell"] >= bench.msbg_sweep_bytes(1) and abs(m["bytes"] - m["bytes_per_cell"] * m["cells"]) < 1.0
     assert bench.measured_traffic("dam break z-up 128^3 x 27 ppc, 1000:1, MG-PCG tol 1e-6", "bf16") is None

@@ -303,6 +303,36 @@ def test_adaptive_smoother_with_all_blocks_active_is_the_plain_one(mods):
     assert torch.equal(out, lv.p)
 
 
+def test_paired_half_sweep_is_the_per_cell_one(mods):
+    """Grids from 2^17 cells take the paired half-sweep (two cells per
+    thread); it gives the per-cell sweep's bits, here against the active-set
+    kernel with every block active, on a graded grid of 178k cells holding
+    blocks of 8^3, 4^3, 2^3 and 1 cell (so also odd block offsets)."""
+    say("paired half-sweep")
+    g, o = pair(mods, (14, 14, 14), 8, 1.0 / 112, 3, "ball")
+    assert g.ncells >= 1 << 17 and set(np.unique(o.blev)) == {0, 1, 2, 3}
+    mg = mods["mr_multigrid"].MRMultigrid(g, min_cells=10 ** 7)  # one level: the grid itself
+    rng = np.random.default_rng(11)
+    mg.build_hierarchy(t32(rng.uniform(0.001, 1.0, g.ncells * 3)))
+    lv = mg.levels[0]
+    b = t32(rng.standard_normal(g.ncells))
+    p = t32(rng.standard_normal(g.ncells))
+    active = torch.ones(g.nslots, dtype=torch.int32, device="cuda")
+    _lib = mods["_lib"]
+    for colour in (0, 1):
+        out1 = torch.full_like(p, float("nan"))
+        out2 = torch.full_like(p, float("nan"))
+        delta = torch.zeros_like(p)
+        _lib.call("pf_msbg_rbgs", g.ref(), _lib.ptr(p), _lib.ptr(out1), _lib.ptr(b), _lib.ptr(lv.bface),
+                  _lib.ptr(lv.diag), colour, 6.0 / 7.0, _lib.stream())
+        _lib.call("pf_msbg_rbgs_active", g.ref(), _lib.ptr(p), _lib.ptr(out2), _lib.ptr(b), _lib.ptr(lv.bface),
+                  _lib.ptr(lv.diag), _lib.ptr(active), colour, 6.0 / 7.0, _lib.ptr(delta), _lib.stream())
+        assert torch.equal(out1, out2)
+        # half the cells moved, the other half were copied
+        moved = (out1 != p).float().mean().item()
+        assert 0.3 < moved < 0.6
+
+
 def test_adaptive_vcycle_matches_oracle(mods):
     say("adaptive V-cycle vs oracle")
     g, o = pair(mods, (4, 4, 4), 4, 1 / 16, 1, "slab")
