@@ -331,6 +331,14 @@ def test_paired_half_sweep_is_the_per_cell_one(mods):
         # half the cells moved, the other half were copied
         moved = (out1 != p).float().mean().item()
         assert 0.3 < moved < 0.6
+        # pin / pout 4 bytes off an 8-byte boundary: the per-cell kernel, same bits
+        pin_v = torch.empty(g.ncells + 1, device="cuda")[1:]
+        pout_v = torch.full((g.ncells + 1,), float("nan"), device="cuda")[1:]
+        pin_v.copy_(p)
+        assert pin_v.data_ptr() % 8 == 4 and pout_v.data_ptr() % 8 == 4
+        _lib.call("pf_msbg_rbgs", g.ref(), _lib.ptr(pin_v), _lib.ptr(pout_v), _lib.ptr(b), _lib.ptr(lv.bface),
+                  _lib.ptr(lv.diag), colour, 6.0 / 7.0, _lib.stream())
+        assert torch.equal(pout_v, out1)
 
 
 def test_adaptive_vcycle_matches_oracle(mods):
