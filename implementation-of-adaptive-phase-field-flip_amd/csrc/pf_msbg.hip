@@ -394,6 +394,65 @@ __global__ __launch_bounds__(256) void k_msbg_rbgs_active(MGrid g, const float* 
     pout[a] = v;
 }
 
+// k_msbg_rbgs_active with thread t on cells 2t and 2t + 1, paired as in
+// k_msbg_rbgs_pairs (one cell of each colour in one block row, so both share
+// the block's active flag); other cells take the per-cell path.  Same
+// arithmetic and the same delta writes per cell.
+__device__ inline void rbgs_active_cell(const MGrid& g, const float* __restrict__ pin, float* __restrict__ pout,
+                                        const float* __restrict__ b, const float* __restrict__ bface,
+                                        const float* __restrict__ diag, const int* __restrict__ active,
+                                        int colour, float omega, float* __restrict__ delta, int a) {
+    float v = pin[a];
+    if (active[g.cslot[a]]) {
+        BlockRef br;
+        const int4 o = origin_ref(g, a, br);
+        if ((((o.x >> o.w) + (o.y >> o.w) + (o.z >> o.w)) & 1) == colour) {
+            const float d = diag[a];
+            if (d > 0.0f) {
+                const float upd = omega * (b[a] - apply_row_from(g, pin, bface, a, o, br, false)) / d;
+                v = v + upd;
+                delta[a] = fabsf(upd);
+            }
+        }
+    }
+    pout[a] = v;
+}
+
+__global__ __launch_bounds__(256) void k_msbg_rbgs_active_pairs(MGrid g, const float* __restrict__ pin,
+                                                                float* __restrict__ pout,
+                                                                const float* __restrict__ b,
+                                                                const float* __restrict__ bface,
+                                                                const float* __restrict__ diag,
+                                                                const int* __restrict__ active, int colour,
+                                                                float omega, float* __restrict__ delta) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int a0 = 2 * t;
+    if (a0 >= g.ncells) return;
+    BlockRef br;
+    const int4 o0 = origin_ref(g, a0, br);
+    if (br.n >= 2 && ((a0 - br.base) & 1) == 0) {
+        float2 v2 = *reinterpret_cast<const float2*>(pin + a0);
+        if (active[br.slot]) {
+            const bool first = (((o0.x >> o0.w) + (o0.y >> o0.w) + (o0.z >> o0.w)) & 1) == colour;
+            const int act = first ? a0 : a0 + 1;
+            const int4 oa = first ? o0 : make_int4(o0.x, o0.y, o0.z + (1 << o0.w), o0.w);
+            const float d = diag[act];
+            if (d > 0.0f) {
+                const float upd = omega * (b[act] - apply_row_from(g, pin, bface, act, oa, br, false)) / d;
+                if (first)
+                    v2.x = v2.x + upd;
+                else
+                    v2.y = v2.y + upd;
+                delta[act] = fabsf(upd);
+            }
+        }
+        *reinterpret_cast<float2*>(pout + a0) = v2;
+        return;
+    }
+    rbgs_active_cell(g, pin, pout, b, bface, diag, active, colour, omega, delta, a0);
+    if (a0 + 1 < g.ncells) rbgs_active_cell(g, pin, pout, b, bface, diag, active, colour, omega, delta, a0 + 1);
+}
+
 // res[slot] = max over the block's cells of delta * diag, 0 when inactive
 // (adaptive_relax.py:112-129); one workgroup per block
 __global__ __launch_bounds__(256) void k_msbg_block_res(MGrid g, const float* __restrict__ delta,
@@ -739,8 +798,12 @@ PF_API int pf_msbg_rbgs_active(const pf_msbg_grid* g, const float* pin, float* p
     PF_REQUIRE(pin && pout && b && bface && diag && active && delta, "pf_msbg_rbgs_active: null vector");
     PF_REQUIRE(pin != pout, "pf_msbg_rbgs_active: pin and pout must be distinct");
     PF_REQUIRE(colour == 0 || colour == 1, "pf_msbg_rbgs_active: colour must be 0 or 1");
-    k_msbg_rbgs_active<<<cdiv(g->ncells, 256), 256, 0, S(stream)>>>(mgrid(*g), pin, pout, b, bface, diag, active,
-                                                                    colour, omega, delta);
+    if (g->ncells >= kMsbgPairMinCells && ((reinterpret_cast<uintptr_t>(pin) | reinterpret_cast<uintptr_t>(pout)) & 7) == 0)
+        k_msbg_rbgs_active_pairs<<<cdiv(cdiv(g->ncells, 2), 256), 256, 0, S(stream)>>>(
+            mgrid(*g), pin, pout, b, bface, diag, active, colour, omega, delta);
+    else
+        k_msbg_rbgs_active<<<cdiv(g->ncells, 256), 256, 0, S(stream)>>>(mgrid(*g), pin, pout, b, bface, diag, active,
+                                                                        colour, omega, delta);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }

@@ -304,11 +304,13 @@ def test_adaptive_smoother_with_all_blocks_active_is_the_plain_one(mods):
 
 
 def test_paired_half_sweep_is_the_per_cell_one(mods):
-    """Grids from 2^17 cells take the paired half-sweep (two cells per
-    thread); it gives the per-cell sweep's bits, here against the active-set
-    kernel with every block active, on a graded grid of 178k cells holding
-    blocks of 8^3, 4^3, 2^3 and 1 cell (so also odd block offsets)."""
-    say("paired half-sweep")
+    """Grids from 2^17 cells take the paired half-sweeps (two cells per
+    thread, 8-byte aligned pin / pout); buffers 4 bytes off an 8-byte
+    boundary take the per-cell kernels.  Both give the same bits, for the
+    plain sweep and for the active-set sweep (every block active, and a
+    random active set, with its delta), on a graded grid of 178k cells
+    holding blocks of 8^3, 4^3, 2^3 and 1 cell (so also odd block offsets)."""
+    say("paired half-sweeps")
     g, o = pair(mods, (14, 14, 14), 8, 1.0 / 112, 3, "ball")
     assert g.ncells >= 1 << 17 and set(np.unique(o.blev)) == {0, 1, 2, 3}
     mg = mods["mr_multigrid"].MRMultigrid(g, min_cells=10 ** 7)  # one level: the grid itself
@@ -317,28 +319,43 @@ def test_paired_half_sweep_is_the_per_cell_one(mods):
     lv = mg.levels[0]
     b = t32(rng.standard_normal(g.ncells))
     p = t32(rng.standard_normal(g.ncells))
-    active = torch.ones(g.nslots, dtype=torch.int32, device="cuda")
     _lib = mods["_lib"]
+    n = g.ncells
+
+    def off():  # a buffer 4 bytes off an 8-byte boundary
+        v = torch.full((n + 1,), float("nan"), device="cuda")[1:]
+        assert v.data_ptr() % 8 == 4
+        return v
+
+    p_off = off()
+    p_off.copy_(p)
+    all_on = torch.ones(g.nslots, dtype=torch.int32, device="cuda")
+    some_on = t32(rng.random(g.nslots) < 0.5).to(torch.int32)
+    assert 0 < int(some_on.sum()) < g.nslots
     for colour in (0, 1):
-        out1 = torch.full_like(p, float("nan"))
-        out2 = torch.full_like(p, float("nan"))
-        delta = torch.zeros_like(p)
-        _lib.call("pf_msbg_rbgs", g.ref(), _lib.ptr(p), _lib.ptr(out1), _lib.ptr(b), _lib.ptr(lv.bface),
-                  _lib.ptr(lv.diag), colour, 6.0 / 7.0, _lib.stream())
-        _lib.call("pf_msbg_rbgs_active", g.ref(), _lib.ptr(p), _lib.ptr(out2), _lib.ptr(b), _lib.ptr(lv.bface),
-                  _lib.ptr(lv.diag), _lib.ptr(active), colour, 6.0 / 7.0, _lib.ptr(delta), _lib.stream())
-        assert torch.equal(out1, out2)
+        plain_pair = torch.full_like(p, float("nan"))
+        plain_cell = off()
+        for pin, pout in ((p, plain_pair), (p_off, plain_cell)):
+            _lib.call("pf_msbg_rbgs", g.ref(), _lib.ptr(pin), _lib.ptr(pout), _lib.ptr(b), _lib.ptr(lv.bface),
+                      _lib.ptr(lv.diag), colour, 6.0 / 7.0, _lib.stream())
+        assert torch.equal(plain_pair, plain_cell)
         # half the cells moved, the other half were copied
-        moved = (out1 != p).float().mean().item()
-        assert 0.3 < moved < 0.6
-        # pin / pout 4 bytes off an 8-byte boundary: the per-cell kernel, same bits
-        pin_v = torch.empty(g.ncells + 1, device="cuda")[1:]
-        pout_v = torch.full((g.ncells + 1,), float("nan"), device="cuda")[1:]
-        pin_v.copy_(p)
-        assert pin_v.data_ptr() % 8 == 4 and pout_v.data_ptr() % 8 == 4
-        _lib.call("pf_msbg_rbgs", g.ref(), _lib.ptr(pin_v), _lib.ptr(pout_v), _lib.ptr(b), _lib.ptr(lv.bface),
-                  _lib.ptr(lv.diag), colour, 6.0 / 7.0, _lib.stream())
-        assert torch.equal(pout_v, out1)
+        assert 0.3 < (plain_pair != p).float().mean().item() < 0.6
+        for active in (all_on, some_on):
+            outs, deltas = [], []
+            for pin, pout in ((p, torch.full_like(p, float("nan"))), (p_off, off())):
+                delta = torch.zeros_like(p)
+                _lib.call("pf_msbg_rbgs_active", g.ref(), _lib.ptr(pin), _lib.ptr(pout), _lib.ptr(b),
+                          _lib.ptr(lv.bface), _lib.ptr(lv.diag), _lib.ptr(active), colour, 6.0 / 7.0,
+                          _lib.ptr(delta), _lib.stream())
+                outs.append(pout)
+                deltas.append(delta)
+            assert torch.equal(outs[0], outs[1]) and torch.equal(deltas[0], deltas[1])
+            if active is all_on:
+                assert torch.equal(outs[0], plain_pair)
+            else:
+                moved = (outs[0] != p).float().mean().item()
+                assert 0.05 < moved < 0.5
 
 
 def test_adaptive_vcycle_matches_oracle(mods):
