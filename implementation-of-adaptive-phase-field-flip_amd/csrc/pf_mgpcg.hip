@@ -879,6 +879,8 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
         const int n = v.d.nx * v.d.ny * v.d.nz;
         for (int t = tid; t < n; t += nt) {
             double acc = 0.0;
+            // 16 loads in flight per wait (the sum keeps its order)
+#pragma unroll 16
             for (int j = 0; j < n; ++j) acc += a.cop[(int64_t)j * n + t] * (double)v.b[j];
             v.p[t] = (float)acc;
         }
