@@ -3,8 +3,9 @@
 Both phases are seeded everywhere: the air is a fluid with its own
 particles.  ``_lattice`` / ``seed`` reproduce the reference's numpy stream
 (``default_rng(cfg.seed).random``) and formula bit for bit
-(pfflip/scenes.py:17-45); the lattice is generated in row chunks, which
-consumes the PCG64 stream in the same order as one large call, so memory
+(pfflip/scenes.py:17-45); the lattice is generated in row chunks, each
+drawing from the PCG64 stream advanced to its first row, which gives the
+numbers of one large call while chunks run on several threads and memory
 stays bounded at the 256^3 x 27 ppc scale.
 
 The reference scenes are y-up.  The scenes added for the benchmark
@@ -14,27 +15,41 @@ solid column) take their vertical axis from ``up_axis``.
 
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 AIR, LIQUID = 0, 1
 _CHUNK = 1 << 22  # lattice rows per chunk
+_SEED_THREADS = min(8, os.cpu_count() or 1)
+
+
+def _lattice_geometry(res, dx, ppc):
+    per_axis = round(ppc ** (1.0 / 3.0))
+    assert per_axis**3 == ppc, "ppc must be a perfect cube (8, 27, ...)"
+    nx, ny, nz = res
+    return dx / per_axis, (nx * per_axis, ny * per_axis, nz * per_axis)
+
+
+def _lattice_rows(r0, r1, step, dims, rng, jitter):
+    """Rows [r0, r1) of the jittered lattice; ``rng`` must stand at draw 3 r0
+    of the stream."""
+    X, Y, Z = dims
+    r = np.arange(r0, r1, dtype=np.int64)
+    # row r of the 'ij' meshgrid is lattice site (r // (Y Z), (r // Z) % Y, r % Z)
+    idx = np.stack([r // (Y * Z), (r // Z) % Y, r % Z], axis=-1)
+    pos = (idx + 0.5) * step
+    pos += (rng.random(pos.shape) - 0.5) * step * jitter
+    return pos
 
 
 def _lattice_chunks(res, dx, ppc, rng, jitter=0.8):
     """Yield float64 position chunks of the jittered lattice (scenes.py:17-31)."""
-    per_axis = round(ppc ** (1.0 / 3.0))
-    assert per_axis**3 == ppc, "ppc must be a perfect cube (8, 27, ...)"
-    nx, ny, nz = res
-    step = dx / per_axis
-    X, Y, Z = nx * per_axis, ny * per_axis, nz * per_axis
-    n = X * Y * Z
+    step, dims = _lattice_geometry(res, dx, ppc)
+    n = dims[0] * dims[1] * dims[2]
     for r0 in range(0, n, _CHUNK):
-        r = np.arange(r0, min(n, r0 + _CHUNK), dtype=np.int64)
-        # row r of the 'ij' meshgrid is lattice site (r // (Y Z), (r // Z) % Y, r % Z)
-        idx = np.stack([r // (Y * Z), (r // Z) % Y, r % Z], axis=-1)
-        pos = (idx + 0.5) * step
-        pos += (rng.random(pos.shape) - 0.5) * step * jitter
-        yield pos
+        yield _lattice_rows(r0, min(n, r0 + _CHUNK), step, dims, rng, jitter)
 
 
 def _lattice(res, dx, ppc, rng, jitter=0.8):
@@ -49,21 +64,36 @@ def seed(cfg, liquid_mask_fn, velocity_fn=None, jitter=0.8):
     Both see float64 positions, as in the reference.  ``jitter`` is the
     peak-to-peak jitter in sub-cell units (0.8 in the reference).
     """
-    rng = np.random.default_rng(cfg.seed)
-    pa = round(cfg.ppc ** (1.0 / 3.0))
-    n = int(np.prod(cfg.res)) * pa**3
+    step, dims = _lattice_geometry(cfg.res, cfg.dx, cfg.ppc)
+    n = dims[0] * dims[1] * dims[2]
     pos_out = np.empty((n, 3), np.float32)
     vel_out = np.zeros((n, 3), np.float32)
     ptype = np.empty(n, np.int32)
-    r0 = 0
-    for pos in _lattice_chunks(cfg.res, cfg.dx, cfg.ppc, rng, jitter):
-        r1 = r0 + len(pos)
+
+    def chunk(r0):
+        # each chunk draws from its own copy of the stream, advanced to the
+        # chunk's first draw (one 64-bit draw per coordinate): the numbers are
+        # those of one sequential call, and numpy releases the GIL while
+        # filling, so chunks run in parallel threads
+        bg = np.random.PCG64(cfg.seed)
+        bg.advance(3 * r0)
+        r1 = min(n, r0 + _CHUNK)
+        pos = _lattice_rows(r0, r1, step, dims, np.random.Generator(bg), jitter)
         liq = np.asarray(liquid_mask_fn(pos), dtype=bool)
         ptype[r0:r1] = np.where(liq, LIQUID, AIR)
         if velocity_fn is not None:
             vel_out[r0:r1] = np.asarray(velocity_fn(pos, liq), float)
         pos_out[r0:r1] = pos
-        r0 = r1
+
+    starts = range(0, n, _CHUNK)
+    workers = min(len(starts), _SEED_THREADS)
+    if workers <= 1:
+        for r0 in starts:
+            chunk(r0)
+    else:
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            for _ in ex.map(chunk, starts):
+                pass
     return pos_out, vel_out, ptype
 
 
