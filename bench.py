@@ -13,8 +13,9 @@ the dominant kernel (the level-0 smoother sweep) and of P2G and G2P
 BF16 MG-PCG time to 1e-6 on the last step's captured system (with each
 one's pressure error against an FP64 reference solution), per-stage
 milliseconds, the differentiable 64^3 dam break (adjoint vs central finite
-differences), and the CPU baseline (the FP64 numpy restatement of the
-reference step on a bounded sample, one thread).
+differences), and the CPU baseline (the C++/OpenMP FP32 restatement of the
+reference step, oracle/cpu_step.cpp, on the same workload: two steps on
+every host thread and one step on one thread).
 """
 
 from __future__ import annotations
@@ -101,28 +102,40 @@ def measured_traffic(workload, precision):
     return None
 
 
-def cpu_baseline(seconds_budget=30.0):
-    """FP64 numpy restatement of the reference step (oracle/ref_np.py), one
-    thread, on a bounded sample: the same dam break at 32^3 x 27 ppc."""
+def cpu_baseline(pf, n=128, ppc=27, steps_parallel=2, steps_serial=1):
+    """The C++/OpenMP restatement of the reference step (oracle/cpu_step.cpp:
+    FP32, per-particle scatter with float atomics, the same MG-PCG), timed on
+    this host on the benchmark workload itself (the 128^3 x 27 ppc z-up dam
+    break, MG-PCG to 1e-6): ``steps_parallel`` steps with every available
+    thread and ``steps_serial`` with one, from the seeded state."""
+    import subprocess
+    from oracle import cpu_step as cs
     from oracle import ref_np as O
-    n = 32
-    cfg = O.OracleConfig(res=(n, n, n), dx=1.0 / n, ppc=27, gravity=(0.0, 0.0, -9.81),
-                         pcg_tol=1e-6, pcg_max_iter=400, droplets=False)
-    sx, sy, sz = cfg.domain_size
-    pos, vel, pt = O.seed_particles(cfg, lambda p: (p[:, 0] < 0.4 * sx) & (p[:, 2] < 0.6 * sz),
-                                    jitter=0.5)
-    sim = O.OracleSim(cfg)
-    sim.set_particles(pos, vel, pt)
-    steps, t0 = 0, time.perf_counter()
-    while steps < 3:
-        sim.step()
-        steps += 1
-        if time.perf_counter() - t0 > seconds_budget:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": len(pos) * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle/ref_np.py FP64 step, dam break 32^3 x 27 ppc ({len(pos)} particles), "
-                      f"{steps} steps in {dt:.1f} s"}
+    if not os.path.exists(cs.LIB):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    cfg = O.OracleConfig(res=(n, n, n), dx=1.0 / n, ppc=ppc, gravity=(0.0, 0.0, -9.81), pcg_tol=1e-6,
+                         pcg_max_iter=400, droplets=False)
+    pos, vel, pt = pf.scenes.dam_break(cfg, frac=(0.4, 1.0, 0.6), jitter=0.5)
+    n_p = len(pos)
+    out = {}
+    all_threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    for label, threads, steps in (("parallel", all_threads, steps_parallel), ("serial", 1, steps_serial)):
+        used = cs.set_threads(threads)
+        sim = cs.CpuStepSim(cfg, O.derived_densities(cfg))
+        sim.set_particles(pos, vel, pt)
+        t0 = time.perf_counter()
+        recs = [sim.step() for _ in range(steps)]
+        el = time.perf_counter() - t0
+        del sim
+        out[label] = {"value": n_p * steps / el, "unit": "particle-steps/s", "cores": used, "kind": "port",
+                      "sample": f"oracle/cpu_step.cpp (C++/OpenMP FP32 restatement of the reference step), "
+                                f"dam break z-up {n}^3 x {ppc} ppc ({n_p} particles), the first {steps} "
+                                f"step(s) from the seeded state in {el:.1f} s on {used} thread(s), "
+                                f"MG-PCG iterations {[r['pcg_iter'] for r in recs]}"}
+        print(f"[bench] cpu baseline {label}: {out[label]['sample']}", file=sys.stderr, flush=True)
+    res = dict(out["parallel"])
+    res["serial"] = out["serial"]
+    return res
 
 
 def fp64_apply_A(bx, by, bz, p):
@@ -299,18 +312,108 @@ def adjoint_check(pf, torch, n=64, ppc=8, n_steps=16, n_dirs=8, eps=1e-3, tol=1e
             "directions": rows}
 
 
-def stage_rooflines(pf, sim, stages, n_p, n_c, s16):
+def p2g_kernel_ms(torch, sim, reps=5):
+    """Average milliseconds of the P2G launches alone (k_max_count, k_p2g,
+    k_p2g_gather through the current bins), HIP events on the stream they
+    run on around back-to-back replays: the host enqueues faster than the
+    GPU runs them, so no host time falls inside.  The replays rewrite the
+    same grid sums from the same particles."""
+    sim._splat()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        sim._splat()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def g2p_kernel_ms(pf, torch, sim, dt, reps=5):
+    """Average milliseconds of one fused G2P + RK3 + reflection launch
+    through the bins (k_g2p_tiled, leaving the speed bounds as in the step),
+    HIP events around the launch alone.  Each replay starts from the same
+    particle state (restored by a copy outside the event pair) and the state
+    after the step is put back afterwards."""
+    import ctypes
+    lib = pf._lib
+    x1, v1 = sim.px.clone(), sim.pv.clone()
+    x0, v0 = getattr(sim, "_bench_x0"), getattr(sim, "_bench_v0")
+    bins = (lib.ptr(sim.perm), lib.ptr(sim.cell_start), lib.ptr(sim._n_active))
+    a = sim._g2p_args(lib.GA_G2P | lib.GA_ADVECT | lib.GA_REFLECT, dt)
+    total = 0.0
+    for r in range(reps + 1):
+        sim.px.copy_(x0)
+        sim.pv.copy_(v0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        lib.call("pf_g2p_advect_binned", ctypes.byref(a), *bins, lib.ptr(sim.pmass),
+                 lib.ptr(sim._bounds), lib.stream())
+        e1.record()
+        torch.cuda.synchronize()
+        if r > 0:
+            total += e0.elapsed_time(e1)
+    sim.px.copy_(x1)
+    sim.pv.copy_(v1)
+    sim._bounds_token = None
+    torch.cuda.synchronize()
+    return total / reps
+
+
+def particle_kernel_rooflines(pf, torch, sim, n_p, n_c, dt):
+    """P2G and G2P rooflines from kernel-only event timings (see
+    p2g_kernel_ms / g2p_kernel_ms)."""
+    return [roofline("p2g (k_max_count + k_p2g + k_p2g_gather)", p2g_bytes(n_p, n_c), p2g_kernel_ms(torch, sim)),
+            roofline("k_g2p_tiled (G2P + RK3 + reflect + bounds)", g2p_bytes(n_p, n_c),
+                     g2p_kernel_ms(pf, torch, sim, dt))]
+
+
+def stage_rooflines(pf, torch, sim, n_p, n_c, s16, dt):
     """Per-kernel GB/s (algorithmic bytes / measured time) of the step's
     dominant kernels: the level-0 sweep (HIP events via pf_mg_time_smoother),
-    P2G and G2P (per-stage HIP events averaged over the timed steps)."""
+    P2G and G2P (kernel-only HIP events around replays)."""
     out = []
     if sim.solver.small_from > 0:
         ms = pf.poisson.time_smoother(sim.solver, 0, reps=10)
         out.append(roofline(f"k_rbgs_full level-0 sweep ({sim.res[0]}^3)",
                             fused_sweep_bytes(n_c, s16, s16), ms))
-    out.append(roofline("p2g stage (k_max_count + k_p2g + k_p2g_gather)", p2g_bytes(n_p, n_c), stages["p2g"]))
-    out.append(roofline("k_g2p_tiled (G2P + RK3 + reflect + bounds)", g2p_bytes(n_p, n_c), stages["g2p_advect"]))
-    return out
+    return out + particle_kernel_rooflines(pf, torch, sim, n_p, n_c, dt)
+
+
+def timed_steps(torch, sim, steps, label="step"):
+    """Run and time ``steps`` steps: per-step wall times from the step
+    records, the whole region bracketed by device synchronisations.  The
+    state before the last step is kept for the G2P kernel replay."""
+    import gc
+    # a full collection of the interpreter's ~2e5 tracked objects takes tens
+    # of milliseconds and lands inside whatever step allocates the object
+    # that triggers it; freeze the set-up's objects so collections stay cheap
+    gc.collect()
+    gc.freeze()
+    sim.stage_ms = {}
+    recs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step_ms = []
+    for k in range(steps):
+        if k == steps - 1:
+            sim._bench_x0, sim._bench_v0 = sim.px.clone(), sim.pv.clone()
+        ts = time.perf_counter()
+        recs.append(sim.step())        # ends with a device read, so it has finished
+        step_ms.append(1e3 * (time.perf_counter() - ts))
+        print(f"[bench] {label} {k}: pcg_iter={recs[-1]['pcg_iter']} dt={recs[-1]['dt']:.4g} "
+              f"wall={step_ms[-1]:.2f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stages = {k: v / steps for k, v in sim.stage_ms.items()}
+    sim.stage_ms = None
+    return recs, elapsed, stages, step_ms
+
+
+def step_time_summary(step_ms):
+    import statistics
+    return {"ms_per_step_median": statistics.median(step_ms),
+            "ms_per_step_min": min(step_ms), "ms_per_step_max": max(step_ms),
+            "step_ms": [round(t, 3) for t in step_ms]}
 
 
 def run_rt128(pf, torch, args):
@@ -330,22 +433,12 @@ def run_rt128(pf, torch, args):
     sim.set_particles(pos, vel, pt)
     n_p, n_c = len(pos), n ** 3
     del pos, vel, pt
-    sim.stage_ms = {}
-    recs = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
-        recs.append(sim.step())
-        if k % 10 == 9:
-            print(f"[bench] rt step {k + 1}: pcg_iter={recs[-1]['pcg_iter']}", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stages = {k: v / steps for k, v in sim.stage_ms.items()}
-    sim.stage_ms = None
-    kernels = stage_rooflines(pf, sim, stages, n_p, n_c, 2)
+    recs, elapsed, stages, step_ms = timed_steps(torch, sim, steps, "rt step")
+    kernels = stage_rooflines(pf, torch, sim, n_p, n_c, 2, recs[-1]["dt"])
     return {"metric": "Rayleigh-Taylor 128^3 x 27 ppc particle-steps/s, 60 steps, FP16 MG-PCG",
             "value": n_p * steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
-            "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "dtype": "f32",
+            "ms_per_step": 1e3 * elapsed / steps, **step_time_summary(step_ms),
+            "higher_is_better": True, "dtype": "f32",
             "data": "synthetic (seeded jittered lattice)",
             "config": {"workload": f"rayleigh-taylor z-up {n}^3 x 27 ppc, 8 modes amplitude 0.02, "
                                    f"1000:1, alpha_phi 0.5, CFL 0.5, FP16 MG-PCG tol 1e-6",
@@ -375,19 +468,9 @@ def run_surge256(pf, torch, args):
     del pos, vel, pt
     for _ in range(args.warmup):
         sim.step()
-    sim.stage_ms = {}
-    recs = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        recs.append(sim.step())
-        print(f"[bench] surge step {k}: pcg_iter={recs[-1]['pcg_iter']} wall={recs[-1]['wall_s'] * 1e3:.1f} ms",
-              file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stages = {k: v / args.steps for k, v in sim.stage_ms.items()}
-    sim.stage_ms = None
-    kernels = stage_rooflines(pf, sim, stages, n_p, n_c, 2)
+    recs, elapsed, stages, step_ms = timed_steps(torch, sim, args.steps, "surge step")
+    kernels = stage_rooflines(pf, torch, sim, n_p, n_c, 2, recs[-1]["dt"])
+    del sim._bench_x0, sim._bench_v0
     fwd_peak = torch.cuda.max_memory_allocated()
     # one adjoint step with checkpointing: loss = liquid centre of mass
     x0, v0 = sim.px.clone(), sim.pv.clone().requires_grad_(True)
@@ -406,6 +489,7 @@ def run_surge256(pf, torch, args):
     return {"metric": "surge 256^3 x 27 ppc particle-steps/s, BF16 MG-PCG, + one adjoint step",
             "value": n_p * args.steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+            **step_time_summary(step_ms),
             "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded jittered lattice)",
             "config": {"workload": f"surge slab 0.3 (+2.0 in top 0.1) on a solid column, {n}^3 x 27 ppc, "
                                    f"1000:1, BF16 MG-PCG tol 1e-6", "particles": n_p,
@@ -583,29 +667,14 @@ def main():
         rec = sim.step()
         print(f"[bench] warmup {w}: {rec}", file=sys.stderr, flush=True)
 
-    sim.stage_ms = {}
-    recs = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        recs.append(sim.step())
-        print(f"[bench] step {k}: pcg_iter={recs[-1]['pcg_iter']} dt={recs[-1]['dt']:.4g} "
-              f"wall={recs[-1]['wall_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stages = {k: v / args.steps for k, v in sim.stage_ms.items()}
-    sim.stage_ms = None
+    recs, elapsed, stages, step_ms = timed_steps(torch, sim, args.steps)
 
     # the dominant kernel: the level-0 red-black sweep of the V-cycle (one
     # fused kernel per sweep), timed with HIP events on the solver's stream
-    # right after the timed steps
-    smooth_ms = pf.poisson.time_smoother(sim.solver, 0, reps=10)
+    # right after the timed steps; P2G and G2P from kernel-only replays
     s16 = 2 if args.precision != "fp32" else 4
-    kernels = [
-        roofline(f"k_rbgs_full level-0 sweep ({n}^3)", fused_sweep_bytes(n_c, s16, s16), smooth_ms),
-        roofline("p2g stage (k_max_count + k_p2g + k_p2g_gather)", p2g_bytes(n_p, n_c), stages["p2g"]),
-        roofline("k_g2p_tiled (G2P + RK3 + reflect + bounds)", g2p_bytes(n_p, n_c), stages["g2p_advect"]),
-    ]
+    kernels = stage_rooflines(pf, torch, sim, n_p, n_c, s16, recs[-1]["dt"])
+    del sim._bench_x0, sim._bench_v0
     workload = f"dam break z-up {n}^3 x {args.ppc} ppc, 1000:1, MG-PCG tol 1e-6"
     copy_gbs = stream_copy_gbs(torch)
     for k in kernels:
@@ -625,6 +694,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": 1e3 * elapsed / args.steps,
+        **step_time_summary(step_ms),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -646,7 +716,7 @@ def main():
         torch.cuda.empty_cache()
         result["adjoint_64"] = adjoint_check(pf, torch)
     if not args.skip_cpu:
-        result["cpu_baseline"] = cpu_baseline()
+        result["cpu_baseline"] = cpu_baseline(pf, n, args.ppc)
     print(json.dumps(result), flush=True)
 
 
