@@ -54,37 +54,53 @@ def load_frame(path: str) -> dict:
 
 
 def render_slices(sim, path: str, *, axis: int = 2, dpi: int = 110):
-    """Quick-look PNG: a phase-field slice and the particles of a slab
-    through the domain centre (io.py:32-79)."""
+    """Quick-look PNG of the current state, viewed along ``axis``.
+
+    Left: the cell phase field on the grid plane through the domain centre,
+    with the phi = 0.5 contour that separates liquid-tagged from air-tagged
+    cells.  Right: where the liquid particles are, as the number of liquid
+    particles per grid column along the viewing axis, divided by the count a
+    full column holds (ppc * cells along the axis): 1 means a column full of
+    liquid.  Both panels share the same in-plane axes.
+    """
     import matplotlib
 
     matplotlib.use("Agg")
     import matplotlib.pyplot as plt
 
-    phi = _host(sim.phi_cell)
-    sl = [slice(None)] * 3
-    sl[axis] = phi.shape[axis] // 2
-    img = phi[tuple(sl)].T
+    phi = _host(sim.phi_cell).astype(np.float32)
+    h, v = [a for a in range(3) if a != axis]
+    plane = np.take(phi, phi.shape[axis] // 2, axis=axis)      # axes (h, v)
     n = getattr(sim, "n_particles", None)
     px, pt = _host(sim.px), _host(sim.ptype)
     if n is not None:
         px, pt = px[:n], pt[:n]
-    sz = sim.cfg.domain_size
-    band = np.abs(px[:, axis] - 0.5 * sz[axis]) < 1.5 * sim.cfg.dx
-    other = [a for a in range(3) if a != axis]
-    fig, axes = plt.subplots(1, 2, figsize=(11, 5), constrained_layout=True)
-    im = axes[0].imshow(img, origin="lower", extent=[0, sz[other[0]], 0, sz[other[1]]], vmin=0, vmax=1,
-                        cmap="Blues")
-    axes[0].set_title(f"phase field $\\phi$   t = {sim.time:.3f}s")
-    fig.colorbar(im, ax=axes[0], fraction=0.046)
-    liq, air = band & (pt == 1), band & (pt == 0)
-    axes[1].scatter(px[air, other[0]], px[air, other[1]], s=0.3, c="#cccccc", linewidths=0, label="air")
-    axes[1].scatter(px[liq, other[0]], px[liq, other[1]], s=0.4, c="#1f6fb4", linewidths=0, label="liquid")
-    axes[1].set_xlim(0, sz[other[0]])
-    axes[1].set_ylim(0, sz[other[1]])
-    axes[1].set_aspect("equal")
-    axes[1].set_title(f"particles (slab)   step {sim.step_index}")
-    axes[1].legend(loc="upper right", markerscale=12, framealpha=0.9)
+    liq = px[pt == 1]
+    res = phi.shape
+    size = sim.cfg.domain_size
+    counts, _, _ = np.histogram2d(liq[:, h], liq[:, v], bins=(res[h], res[v]),
+                                  range=((0.0, size[h]), (0.0, size[v])))
+    fill = counts / float(max(sim.cfg.ppc * res[axis], 1))
+    names = "xyz"
+    extent = (0.0, size[h], 0.0, size[v])
+
+    fig, (ax_phi, ax_fill) = plt.subplots(1, 2, figsize=(10, 4.4), sharey=True)
+    a = ax_phi.imshow(plane.T, origin="lower", extent=extent, cmap="viridis", vmin=0.0, vmax=1.0,
+                      interpolation="nearest")
+    gh = (np.arange(res[h]) + 0.5) * size[h] / res[h]
+    gv = (np.arange(res[v]) + 0.5) * size[v] / res[v]
+    if plane.min() < 0.5 < plane.max():
+        ax_phi.contour(gh, gv, plane.T, levels=[0.5], colors="w", linewidths=0.8)
+    ax_phi.set_title(f"phi, {names[axis]} = mid-plane (white: phi = 0.5)")
+    b = ax_fill.imshow(fill.T, origin="lower", extent=extent, cmap="cividis", vmin=0.0,
+                       vmax=max(1.0, float(fill.max())), interpolation="nearest")
+    ax_fill.set_title(f"liquid particles per column / full column, along {names[axis]}")
+    for ax in (ax_phi, ax_fill):
+        ax.set_xlabel(names[h])
+    ax_phi.set_ylabel(names[v])
+    fig.colorbar(a, ax=ax_phi, shrink=0.85)
+    fig.colorbar(b, ax=ax_fill, shrink=0.85)
+    fig.suptitle(f"step {sim.step_index}, t = {sim.time:.4f}")
     os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
     fig.savefig(path, dpi=dpi)
     plt.close(fig)

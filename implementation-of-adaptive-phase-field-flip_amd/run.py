@@ -63,8 +63,9 @@ def main(argv=None):
         sim.set_solids(out[3])
     outdir = os.path.join(args.out, f"{args.scene}_{args.res}")
     os.makedirs(outdir, exist_ok=True)
-    print(f"scene={args.scene} res={cfg.res} particles={len(pos):,} (liquid {100 * (ptype == 1).mean():.1f}%)  "
-          f"-> {outdir}", flush=True)
+    n_liq = int((ptype == 1).sum())
+    print(f"[run] {args.scene} on {cfg.res[0]}x{cfg.res[1]}x{cfg.res[2]}: {len(pos)} particles, "
+          f"{n_liq} liquid; writing {args.frames} frames to {outdir}", flush=True)
     frame_dt = 1.0 / args.fps
     t0 = time.perf_counter()
     for frame in range(args.frames):
@@ -73,11 +74,13 @@ def main(argv=None):
         if args.png:
             io.render_slices(sim, os.path.join(outdir, f"frame_{frame:04d}.png"))
         if rec is not None:
-            print(f"frame {frame:4d}  t={sim.time:6.3f}  substeps={nsub:3d}  pcg={rec['pcg_iter']:3d}  "
-                  f"div={rec['max_div']:.2e}  wall={time.perf_counter() - t0:7.1f}s", flush=True)
+            print(f"[run] frame {frame + 1}/{args.frames} at t = {sim.time:.4f}: {nsub} steps, "
+                  f"last solve {rec['pcg_iter']} iterations, max |div u| {rec['max_div']:.3g}, "
+                  f"{time.perf_counter() - t0:.1f} s elapsed", flush=True)
     with open(os.path.join(outdir, "stats.json"), "w") as f:
         json.dump(sim.stats, f)
-    print(f"done in {time.perf_counter() - t0:.1f}s", flush=True)
+    print(f"[run] {len(sim.stats)} steps in {time.perf_counter() - t0:.1f} s; records in "
+          f"{os.path.join(outdir, 'stats.json')}", flush=True)
     return 0
 
 

@@ -20,7 +20,7 @@ def _mods():
 
 def _stub_sim(n=10, res=(4, 5, 6)):
     rng = np.random.default_rng(0)
-    cfg = SimpleNamespace(dx=0.125, domain_size=tuple(r * 0.125 for r in res))
+    cfg = SimpleNamespace(dx=0.125, ppc=8, domain_size=tuple(r * 0.125 for r in res))
     return SimpleNamespace(time=0.375, step_index=7, cfg=cfg, n_particles=n,
                            phi_cell=torch.from_numpy(rng.random(res).astype(np.float32)),
                            px=torch.from_numpy(rng.random((n + 3, 3)).astype(np.float32)),
