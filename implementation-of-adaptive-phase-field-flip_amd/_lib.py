@@ -60,6 +60,7 @@ class MG(ctypes.Structure):
         ("options", ctypes.c_int32),
         ("lev", MGLevel * MAX_LEVELS),
         ("rhs", P), ("res", P), ("z", P), ("q", P), ("Ap", P), ("scalars", P),
+        ("host_state", P),
     ]
 
 
@@ -149,6 +150,8 @@ SIGNATURES = {
     "pf_mg_vcycle": (ctypes.c_int, [ctypes.POINTER(MG), P, P, P]),
     "pf_mgpcg_solve": (ctypes.c_int, [ctypes.POINTER(MG), P, P, F64, I32,
                                       ctypes.POINTER(I32), ctypes.POINTER(F64), P]),
+    "pf_mgpcg_state_create": (ctypes.c_int, [ctypes.POINTER(P)]),
+    "pf_mgpcg_state_destroy": (ctypes.c_int, [P]),
     "pf_mg_time_smoother": (ctypes.c_int, [ctypes.POINTER(MG), I32, I32, ctypes.POINTER(F32), P]),
     "pf_apply_A": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P, P]),
     "pf_project": (ctypes.c_int, [P, PP, PP, PP, PP, ctypes.c_int, ctypes.c_int, ctypes.c_int,

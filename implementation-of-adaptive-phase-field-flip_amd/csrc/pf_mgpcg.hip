@@ -11,6 +11,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "pf_common.hpp"
@@ -1491,22 +1492,24 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
     return PF_OK;
 }
 
-struct HostPoll {
+// Host-side state of one solver (caller-owned through pf_mgpcg_state_*):
+// a pinned ring of two done-flags the host polls, their events, and the HIP
+// graphs captured for this solver's buffers.  Bound to the device current at
+// creation.
+struct SolveGraphs;
+struct SolverState {
+    int device = -1;
     int* flag = nullptr;  // pinned ring of 2 ints
     hipEvent_t ev[2] = {nullptr, nullptr};
-    int device = -1;
+    std::vector<SolveGraphs>* graphs = nullptr;
 };
 
-static int get_poll(HostPoll*& out) {
-    static thread_local HostPoll hp;
-    int dev = 0;
-    PF_CHECK_HIP(hipGetDevice(&dev));
-    if (hp.flag == nullptr || hp.device != dev) {
-        PF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&hp.flag), 2 * sizeof(int), hipHostMallocDefault));
-        for (int q = 0; q < 2; ++q) PF_CHECK_HIP(hipEventCreateWithFlags(&hp.ev[q], hipEventDisableTiming));
-        hp.device = dev;
-    }
-    out = &hp;
+static void release_state(SolverState& st);
+
+static int init_state(SolverState& st) {
+    PF_CHECK_HIP(hipGetDevice(&st.device));
+    PF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.flag), 2 * sizeof(int), hipHostMallocDefault));
+    for (int q = 0; q < 2; ++q) PF_CHECK_HIP(hipEventCreateWithFlags(&st.ev[q], hipEventDisableTiming));
     return PF_OK;
 }
 
@@ -1559,7 +1562,7 @@ static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int*
 // only in the pinned flag slot), cached per solver layout.  A solve issues
 // one graph launch per iteration instead of ~50 kernel launches.
 struct SolveGraphs {
-    std::vector<unsigned char> key;
+    std::vector<uint64_t> key;
     hipGraphExec_t pro = nullptr;
     hipGraphExec_t it[2] = {nullptr, nullptr};
 };
@@ -1598,19 +1601,38 @@ static CGK<ST> slot_args(const CGK<ST>& a, const pf_mg& mg, int slot) {
     return b;
 }
 
+// Everything a captured graph bakes in: the hierarchy's sizes, scalars and
+// buffer addresses, the solve's vectors, the flag slot and the launch shape.
 template <typename ST>
-static int get_graphs(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, HostPoll* hp,
+static std::vector<uint64_t> graph_key(const pf_mg& mg, const CGK<ST>& a, const ST* z, const int* flag,
+                                       unsigned nb) {
+    std::vector<uint64_t> k;
+    auto ptr = [&](const void* p) { k.push_back((uint64_t)(uintptr_t)p); };
+    uint32_t om;
+    std::memcpy(&om, &mg.omega, sizeof(om));
+    k.insert(k.end(), {(uint64_t)mg.nlev, (uint64_t)mg.pre, (uint64_t)mg.post, (uint64_t)mg.coarsest, om,
+                       (uint64_t)mg.precision, (uint64_t)mg.small_from, (uint64_t)mg.options,
+                       (uint64_t)sizeof(ST), (uint64_t)nb});
+    for (int l = 0; l < mg.nlev; ++l) {
+        const pf_mg_level& v = mg.lev[l];
+        k.insert(k.end(), {(uint64_t)v.nx, (uint64_t)v.ny, (uint64_t)v.nz});
+        for (const void* p : {(const void*)v.bx, (const void*)v.by, (const void*)v.bz, (const void*)v.diag,
+                              (const void*)v.p, (const void*)v.b, (const void*)v.r, (const void*)v.bx_h,
+                              (const void*)v.by_h, (const void*)v.bz_h, (const void*)v.p_h})
+            ptr(p);
+    }
+    for (const void* p : {(const void*)mg.rhs, (const void*)mg.res, (const void*)mg.z, (const void*)mg.q,
+                          (const void*)mg.Ap, (const void*)mg.scalars, (const void*)a.b, (const void*)a.x,
+                          (const void*)z, (const void*)flag})
+        ptr(p);
+    return k;
+}
+
+template <typename ST>
+static int get_graphs(SolverState& st, const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done,
                       unsigned nb, SolveGraphs*& out) {
-    static thread_local std::vector<SolveGraphs> cache;
-    int dev = 0;
-    PF_CHECK_HIP(hipGetDevice(&dev));
-    std::vector<unsigned char> key(sizeof(pf_mg) + 6 * sizeof(void*) + 2 * sizeof(int));
-    unsigned char* k = key.data();
-    std::memcpy(k, &mg, sizeof(pf_mg));
-    const void* extra[6] = {a.b, a.x, z, hp->flag, (const void*)(uintptr_t)sizeof(ST), nullptr};
-    std::memcpy(k + sizeof(pf_mg), extra, sizeof(extra));
-    const int ints[2] = {dev, (int)nb};
-    std::memcpy(k + sizeof(pf_mg) + sizeof(extra), ints, sizeof(ints));
+    std::vector<SolveGraphs>& cache = *st.graphs;
+    std::vector<uint64_t> key = graph_key<ST>(mg, a, z, st.flag, nb);
     for (auto& c : cache)
         if (c.key == key) {
             out = &c;
@@ -1619,13 +1641,13 @@ static int get_graphs(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done,
     int rc = ensure_small_lds();
     if (rc) return rc;
     SolveGraphs g;
-    g.key = key;
+    g.key = std::move(key);
     hipStream_t cs = nullptr;
     PF_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     rc = capture(cs, [&](hipStream_t t) { return enqueue_prologue<ST>(mg, a, z, done, nb, t); }, &g.pro);
     for (int q = 0; q < 2 && !rc; ++q) {
         const CGK<ST> aq = slot_args(a, mg, q);
-        rc = capture(cs, [&](hipStream_t t) { return enqueue_iteration<ST>(mg, aq, z, done, &hp->flag[q], nb, t); },
+        rc = capture(cs, [&](hipStream_t t) { return enqueue_iteration<ST>(mg, aq, z, done, &st.flag[q], nb, t); },
                      &g.it[q]);
     }
     (void)hipStreamDestroy(cs);
@@ -1633,13 +1655,27 @@ static int get_graphs(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done,
         destroy_graphs(g);
         return rc;
     }
-    if (cache.size() >= 8) {
+    // a solver is normally solved with one set of vectors; keep a few
+    if (cache.size() >= 4) {
         destroy_graphs(cache.front());
         cache.erase(cache.begin());
     }
     cache.push_back(std::move(g));
     out = &cache.back();
     return PF_OK;
+}
+
+static void release_state(SolverState& st) {
+    if (st.graphs) {
+        for (auto& g : *st.graphs) destroy_graphs(g);
+        delete st.graphs;
+        st.graphs = nullptr;
+    }
+    for (auto& e : st.ev)
+        if (e) (void)hipEventDestroy(e);
+    st.ev[0] = st.ev[1] = nullptr;
+    if (st.flag) (void)hipHostFree(st.flag);
+    st.flag = nullptr;
 }
 
 static bool use_graphs() {
@@ -1696,12 +1732,28 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
         *rel_out = 0.0;
         return PF_OK;
     }
-    HostPoll* hp = nullptr;
-    int rc = get_poll(hp);
-    if (rc) return rc;
+    // the caller's solver state (graphs, flags), or a temporary one for this
+    // call (no graphs)
+    SolverState local;
+    struct LocalGuard {
+        SolverState& s;
+        ~LocalGuard() { release_state(s); }
+    } guard{local};
+    SolverState* hp = static_cast<SolverState*>(mg.host_state);
+    int rc = PF_OK;
+    if (hp) {
+        int dev = 0;
+        PF_CHECK_HIP(hipGetDevice(&dev));
+        PF_REQUIRE(dev == hp->device, "pf_mgpcg_solve: solver state was created on device %d, current is %d",
+                   hp->device, dev);
+    } else {
+        rc = init_state(local);
+        if (rc) return rc;
+        hp = &local;
+    }
     SolveGraphs* gr = nullptr;
-    if (use_graphs() && max_iter > 0) {
-        rc = get_graphs<ST>(mg, a, z, done, hp, nb, gr);
+    if (use_graphs() && max_iter > 0 && hp->graphs) {
+        rc = get_graphs<ST>(*hp, mg, a, z, done, nb, gr);
         if (rc) return rc;
     }
     if (gr) {
@@ -1884,6 +1936,37 @@ PF_API int pf_mg_vcycle(const pf_mg* mg, const float* r, float* z, pf_stream_t s
                    (const void*)r != (const void*)mg->lev[0].p,
                "pf_mg_vcycle: r, z and the level-0 p buffer must be distinct");
     return vcycle<float>(*mg, r, z, nullptr, nullptr, S(stream));
+}
+
+PF_API int pf_mgpcg_state_create(void** out) {
+    PF_REQUIRE(out, "pf_mgpcg_state_create: null argument");
+    *out = nullptr;
+    SolverState* st = new (std::nothrow) SolverState();
+    PF_REQUIRE(st, "pf_mgpcg_state_create: out of host memory");
+    int rc = init_state(*st);
+    if (!rc) {
+        st->graphs = new (std::nothrow) std::vector<SolveGraphs>();
+        if (!st->graphs) rc = set_error(PF_ERR_INVALID, "pf_mgpcg_state_create: out of host memory");
+    }
+    if (rc) {
+        release_state(*st);
+        delete st;
+        return rc;
+    }
+    *out = st;
+    return PF_OK;
+}
+
+PF_API int pf_mgpcg_state_destroy(void* state) {
+    if (!state) return PF_OK;
+    SolverState* st = static_cast<SolverState*>(state);
+    int cur = 0;
+    const bool switched = hipGetDevice(&cur) == hipSuccess && cur != st->device;
+    if (switched) (void)hipSetDevice(st->device);
+    release_state(*st);
+    if (switched) (void)hipSetDevice(cur);
+    delete st;
+    return PF_OK;
 }
 
 PF_API int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol, int32_t max_iter,

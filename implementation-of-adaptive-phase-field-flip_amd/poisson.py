@@ -8,7 +8,9 @@ the work is done by ``pf_mg_*`` / ``pf_mgpcg_solve`` in libpfflip_hip.so.
 Differences from the reference that do not change the algorithm:
 
 * the FPCG scalars stay on the GPU; ``solve`` synchronises with the host
-  about once per iteration (one iteration behind) instead of six times;
+  about once per iteration (one iteration behind) instead of six times; the
+  solver owns its host state (pinned flags, captured HIP graphs) through
+  ``pf_mgpcg_state_create`` and frees it with the solver;
 * dot products and means are deterministic FP64 reductions instead of FP32
   atomics, and the Polak-Ribiere numerator uses r_prev = r + alpha Ap, so no
   r_prev copy is kept (``r_prev`` exists as an attribute but is unused);
@@ -121,7 +123,19 @@ class MGPCG:
                 setattr(lv, name, None if t is None else t.data_ptr())
         mg.rhs, mg.res, mg.z = self.rhs.data_ptr(), self.res_v.data_ptr(), self.z.data_ptr()
         mg.q, mg.Ap, mg.scalars = self.q.data_ptr(), self.Ap.data_ptr(), self._scalars.data_ptr()
+        # host state of this solver (pinned done-flags, the captured graphs),
+        # freed with the solver
+        st = ctypes.c_void_p(None)
+        _lib.call("pf_mgpcg_state_create", ctypes.byref(st))
+        mg.host_state = st.value
+        self._state = st.value
+        self._lib = lib
         self._mg = mg
+
+    def __del__(self):
+        st, self._state = getattr(self, "_state", None), None
+        if st:
+            self._lib.pf_mgpcg_state_destroy(st)
 
     # ---------------------------------------------------------------- hierarchy
     def build_hierarchy(self, bx, by, bz):

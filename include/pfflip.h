@@ -193,6 +193,10 @@ typedef struct pf_mg {
     float* q;
     float* Ap;
     void* scalars; /* pf_mg_scalar_bytes() of device memory */
+    /* solver host state from pf_mgpcg_state_create (pinned done-flags, events
+     * and the HIP graphs captured for these buffers), owned by the caller;
+     * NULL: each pf_mgpcg_solve creates and frees its own, without graphs */
+    void* host_state;
 } pf_mg;
 
 /* Smooth with two half-sweep launches per sweep instead of the fused
@@ -213,9 +217,18 @@ int pf_mg_precondition(const pf_mg* mg, const float* r, float* z, pf_stream_t st
  * right-hand side r (the coarsest level's mean removal included, no final
  * mean removal).  FP32 hierarchies only; r, z and lev[0].p distinct. */
 int pf_mg_vcycle(const pf_mg* mg, const float* r, float* z, pf_stream_t stream);
+/* Host state of one solver for pf_mg.host_state: created on the current
+ * device, destroyed with pf_mgpcg_state_destroy (no reference counterpart;
+ * MGPCG.__init__'s buffers, poisson.py:249-275, hold no host state).  One
+ * state serves one thread at a time. */
+int pf_mgpcg_state_create(void** out);
+int pf_mgpcg_state_destroy(void* state);
 /* solve (poisson.py:355-409): flexible PCG, Polak-Ribiere beta clamped at 0,
  * stop when max|r|/max|b| < tol (recursive residual, b mean-removed).
- * Writes x (mean removed); *it_out / *rel_out are host pointers.  Blocks. */
+ * Writes x (mean removed); *it_out / *rel_out are host pointers.  Blocks:
+ * the host polls a pinned done-flag one iteration behind the GPU to stop
+ * enqueueing iterations.  The library keeps no state of its own between
+ * calls: graphs and flags live in mg->host_state. */
 int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol, int32_t max_iter,
                    int32_t* it_out, double* rel_out, pf_stream_t stream);
 /* Measurement hook (no reference counterpart): average milliseconds of one
