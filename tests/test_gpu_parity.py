@@ -225,14 +225,17 @@ def test_phase_beta_forces_match_oracle(pf):
     sim._phase_field_and_coefficients()
     torch.cuda.synchronize()
     # oracle stage on the GPU's P2G output (isolates the stage)
-    phi, tag = O.cell_phase(np_(sim.cell_mass), osim.rho_min, cfg.alpha_phi, osim.rho_0l)
+    # rho_min and rho_0l as the FP32 kernel arguments the reference's Warp
+    # kernels also take: rho - rho_min is then exact in both (Sterbenz)
+    rho_min, rho_0l = float(np.float32(osim.rho_min)), float(np.float32(osim.rho_0l))
+    phi, tag = O.cell_phase(np_(sim.cell_mass), rho_min, cfg.alpha_phi, rho_0l)
     assert rel_err(np_(sim.phi_cell), phi) < 1e-6
     band = np.abs(phi - 0.5) < 1e-5
     assert np.array_equal(np_(sim.is_liquid)[~band], tag[~band])
     # the GPU's own tags drive the face stage
     gtag = np_(sim.is_liquid)
     for ax in range(3):
-        f, b = O.face_beta(np_(sim.mass[ax]), gtag, ax, osim.rho_min, cfg.alpha_phi, osim.rho_0l,
+        f, b = O.face_beta(np_(sim.mass[ax]), gtag, ax, rho_min, cfg.alpha_phi, rho_0l,
                            cfg.rho_l, cfg.rho_g)
         w = O.wall_mask(b.shape, ax)
         b[w] = 0.0

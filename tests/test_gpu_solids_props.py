@@ -40,6 +40,10 @@ def np_(t):
     return t.detach().cpu().numpy()
 
 
+def f32(x):
+    return float(np.float32(x))
+
+
 def rel_err(got, ref):
     ref = np.asarray(ref, np.float64)
     scale = max(np.abs(ref).max(), 1e-30)
@@ -85,7 +89,12 @@ def test_obstacle_faces_before_and_after_gravity_match_oracle(pf, scene):
     for ax in range(3):
         m, mu = np_(sim.mass[ax]), np_(sim.mom[ax])
         v = O.face_velocity_from_momentum(m, mu, 1e-12 * osim.rho_0l)[0]
-        f, b = O.face_beta(m, gtag, ax, osim.rho_min, cfg.alpha_phi, osim.rho_0l, cfg.rho_l, cfg.rho_g)
+        # the kernel takes rho_min and rho_0l as FP32 arguments, as the
+        # reference's Warp kernels do; with the same FP32 constants the
+        # subtraction rho - rho_min is exact in both (Sterbenz), so phi near
+        # rho_min is not an ill-conditioned comparison
+        f, b = O.face_beta(m, gtag, ax, f32(osim.rho_min), cfg.alpha_phi, f32(osim.rho_0l),
+                           cfg.rho_l, cfg.rho_g)
         O.zero_solid_faces(b, v, ax)
         O.zero_solid_obstacle_faces(solid, b, v, ax)
         blocked = O.solid_face_mask(solid, ax) | O.wall_mask(b.shape, ax)
