@@ -138,6 +138,8 @@ SIGNATURES = {
     "pf_bin_scan_tmp_bytes": (SZ, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pf_bin_sort": (ctypes.c_int, [P, P, I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32,
                                    P, P, P, P, P, P, P, I32, P]),
+    "pf_bin_sort_noalloc": (ctypes.c_int, [P, P, I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32,
+                                           P, P, P, P, P, P, I32, P]),
     "pf_p2g_stage_bytes": (SZ, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pf_p2g": (ctypes.c_int, [P, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32, F32,
                               P, P, P, P, P, P, P, P, P, P, SZ, P]),

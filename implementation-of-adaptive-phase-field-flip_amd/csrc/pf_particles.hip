@@ -1,6 +1,8 @@
 // Particle-side kernels: binning + stable counting sort, tiled P2G with LDS
 // accumulators, fused G2P + RK3 advection + wall reflection (+ solids), and
 // the CFL speed reduction.
+#include <mutex>
+
 #include "pf_common.hpp"
 
 namespace pf {
@@ -1167,19 +1169,26 @@ PF_API size_t pf_bin_scan_tmp_bytes(int nx, int ny, int nz) {
     return (size_t)(nb + 2 * (cdiv(nb + 1, SCAN_TILE) + 2)) * sizeof(int);
 }
 
-PF_API int pf_bin_sort(const float* px, const int32_t* pstate, int64_t n, int nx, int ny, int nz,
-                       float dx, const uint32_t* order, uint32_t* keys, uint32_t* rank,
-                       uint32_t* perm, int32_t* cell_start, void* scan_tmp, int32_t* n_active,
-                       int32_t stable, pf_stream_t stream) {
-    PF_REQUIRE(nx > 0 && ny > 0 && nz > 0 && n >= 0, "pf_bin_sort: bad sizes");
-    PF_REQUIRE(n < (int64_t)1 << 31, "pf_bin_sort: more than 2^31 particles");
-    PF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 31, "pf_bin_sort: grid too large");
-    PF_REQUIRE(!(n > 0 && (!px || !pstate || !keys || !rank || !perm)),
-               "pf_bin_sort: null particle buffer");
-    PF_REQUIRE(cell_start && scan_tmp, "pf_bin_sort: null cell_start / scan_tmp");
-    PF_REQUIRE(dx > 0.f, "pf_bin_sort: dx must be positive");
-    PF_REQUIRE(order != perm || n == 0, "pf_bin_sort: order and perm must be distinct buffers");
-    hipStream_t s = S(stream);
+// Argument checks shared by pf_bin_sort and pf_bin_sort_noalloc (everything
+// but the scan buffer).
+static int bin_sort_check(const char* fn, const float* px, const int32_t* pstate, int64_t n, int nx,
+                          int ny, int nz, float dx, const uint32_t* order, const uint32_t* keys,
+                          const uint32_t* rank, const uint32_t* perm, const int32_t* cell_start) {
+    PF_REQUIRE(nx > 0 && ny > 0 && nz > 0 && n >= 0, "%s: bad sizes", fn);
+    PF_REQUIRE(n < (int64_t)1 << 31, "%s: more than 2^31 particles", fn);
+    PF_REQUIRE((int64_t)nx * ny * nz < (int64_t)1 << 31, "%s: grid too large", fn);
+    PF_REQUIRE(!(n > 0 && (!px || !pstate || !keys || !rank || !perm)), "%s: null particle buffer", fn);
+    PF_REQUIRE(cell_start, "%s: null cell_start", fn);
+    PF_REQUIRE(dx > 0.f, "%s: dx must be positive", fn);
+    PF_REQUIRE(order != perm || n == 0, "%s: order and perm must be distinct buffers", fn);
+    return PF_OK;
+}
+
+// The sort itself, on a scan buffer of pf_bin_scan_tmp_bytes() bytes.
+static int bin_sort_run(const float* px, const int32_t* pstate, int64_t n, int nx, int ny, int nz,
+                        float dx, const uint32_t* order, uint32_t* keys, uint32_t* rank,
+                        uint32_t* perm, int32_t* cell_start, void* scan_tmp, int32_t* n_active,
+                        int32_t stable, hipStream_t s) {
     const TileGeom g = tile_geom(nx, ny, nz);
     const int64_t nkeys = n_tiles(g) * T3;
     const int64_t nbuckets = nkeys + 1;
@@ -1207,6 +1216,72 @@ PF_API int pf_bin_sort(const float* px, const int32_t* pstate, int64_t n, int nx
         }
     }
     return PF_OK;
+}
+
+PF_API int pf_bin_sort(const float* px, const int32_t* pstate, int64_t n, int nx, int ny, int nz,
+                       float dx, const uint32_t* order, uint32_t* keys, uint32_t* rank,
+                       uint32_t* perm, int32_t* cell_start, void* scan_tmp, int32_t* n_active,
+                       int32_t stable, pf_stream_t stream) {
+    int rc = bin_sort_check("pf_bin_sort", px, pstate, n, nx, ny, nz, dx, order, keys, rank, perm,
+                            cell_start);
+    if (rc) return rc;
+    PF_REQUIRE(scan_tmp, "pf_bin_sort: null scan_tmp");
+    return bin_sort_run(px, pstate, n, nx, ny, nz, dx, order, keys, rank, perm, cell_start, scan_tmp,
+                        n_active, stable, S(stream));
+}
+
+// ------------------------------------------------ pf_bin_sort without allocation
+// The scan buffer of pf_bin_sort_noalloc lives in this file-scope device
+// array: one copy per GPU, held until the process ends, sized for
+// PF_BIN_NOALLOC_MAX_KEYS binning keys.  In host code `scratch` names a
+// host-side stand-in, not the device copy: kernels only ever receive the
+// address hipGetSymbolAddress returns for the current device.
+// (the layout of pf_bin_scan_tmp_bytes: counts of nkeys + 1 buckets, then the
+// block sums and offsets of their scan)
+constexpr int64_t kNoallocBuckets = (int64_t)PF_BIN_NOALLOC_MAX_KEYS + 1;
+constexpr int64_t kNoallocScanInts =
+    kNoallocBuckets + 2 * ((kNoallocBuckets + 1 + SCAN_TILE - 1) / SCAN_TILE + 2);
+static __device__ __align__(256) int scratch[kNoallocScanInts];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];  // device address of `scratch`, per device
+
+// The device address of the current device's `scratch`, looked up once per
+// device (with that device current) and kept.
+static int noalloc_scratch(void** out) {
+    int dev = 0;
+    PF_CHECK_HIP(hipGetDevice(&dev));
+    PF_REQUIRE(dev >= 0 && dev < kMaxDevices, "pf_bin_sort_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        PF_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        PF_REQUIRE(ptr, "pf_bin_sort_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return PF_OK;
+}
+
+PF_API int pf_bin_sort_noalloc(const float* px, const int32_t* pstate, int64_t n, int nx, int ny,
+                               int nz, float dx, const uint32_t* order, uint32_t* keys,
+                               uint32_t* rank, uint32_t* perm, int32_t* cell_start,
+                               int32_t* n_active, int32_t stable, pf_stream_t stream) {
+    int rc = bin_sort_check("pf_bin_sort_noalloc", px, pstate, n, nx, ny, nz, dx, order, keys, rank,
+                            perm, cell_start);
+    if (rc) return rc;
+    const int64_t nkeys = pf_bin_nkeys(nx, ny, nz);
+    if (nkeys > PF_BIN_NOALLOC_MAX_KEYS)
+        return set_error(PF_ERR_SIZE,
+                         "pf_bin_sort_noalloc: %lld binning keys for a %dx%dx%d grid exceed the "
+                         "built-in scratch of %d keys",
+                         (long long)nkeys, nx, ny, nz, PF_BIN_NOALLOC_MAX_KEYS);
+    void* tmp = nullptr;
+    rc = noalloc_scratch(&tmp);
+    if (rc) return rc;
+    return bin_sort_run(px, pstate, n, nx, ny, nz, dx, order, keys, rank, perm, cell_start, tmp,
+                        n_active, stable, S(stream));
 }
 
 PF_API size_t pf_p2g_stage_bytes(int nx, int ny, int nz, int with_wsum) {

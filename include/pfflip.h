@@ -76,6 +76,26 @@ int pf_bin_sort(const float* px, const int32_t* pstate, int64_t n, int nx, int n
                 uint32_t* perm, int32_t* cell_start, void* scan_tmp, int32_t* n_active,
                 int32_t stable, pf_stream_t stream);
 
+/* pf_bin_sort without a scan_tmp argument and without any allocation (no
+ * reference counterpart: the reference has no sort).  For callers that
+ * cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  The scan buffer is a
+ * static device array of the library, one per GPU, held until the process
+ * ends, sized for at most PF_BIN_NOALLOC_MAX_KEYS binning keys
+ * (pf_bin_nkeys(); 2^21 keys = a 128^3 grid, about 8.4 MB); a larger grid
+ * returns PF_ERR_SIZE.  Results equal pf_bin_sort's: keys, cell_start and
+ * *n_active bit for bit, and with stable != 0 the FLIP particles' part of
+ * perm too (in both, the particles outside the bins close perm in
+ * atomic-timing order).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (two streams, two threads) race and must be serialised by the
+ * caller.  The array's counts are cleared by a fill kernel at the start of
+ * each call. */
+#define PF_BIN_NOALLOC_MAX_KEYS 2097152
+int pf_bin_sort_noalloc(const float* px, const int32_t* pstate, int64_t n, int nx, int ny, int nz,
+                        float dx, const uint32_t* order, uint32_t* keys, uint32_t* rank,
+                        uint32_t* perm, int32_t* cell_start, int32_t* n_active,
+                        int32_t stable, pf_stream_t stream);
+
 /* ---------------------------------------------------------------- P2G
  * Replaces Simulator._p2g's zeroing + p2g_face x3 + p2g_cell
  * (pfflip/sim.py:205-228, pfflip/kernels.py:100-209): eq. (6) weights
