@@ -28,7 +28,7 @@ constexpr int kCoarseOpMax = 512;
 
 struct CGState {
     double rz, alpha, beta, mu, b_norm, rel, qAq, mu_x, tol;
-    int it, done, pad0, pad1;
+    int it, done, seq, pad1;  // seq: residual tests run in this solve
     // 16-bit V-cycles: the level-0 right-hand side is read times vscale (a
     // power of two putting max|r| in [1/2, 1)) and z is read back times
     // vscale_inv, so the FP16 iterates stay in range as r shrinks
@@ -657,12 +657,13 @@ struct SmallArgs {
     // seg_src[q] to LDS offset seg_dst[q] (the face arrays of every level,
     // then the top level's b)
     int nseg;
-    const float* seg_src[3 * PF_MAX_LEVELS + 1];
-    int seg_dst[3 * PF_MAX_LEVELS + 1];
-    int seg_len[3 * PF_MAX_LEVELS + 1];
+    const float* seg_src[3 * PF_MAX_LEVELS + 2];
+    int seg_dst[3 * PF_MAX_LEVELS + 2];
+    int seg_len[3 * PF_MAX_LEVELS + 2];
     void* p_top;
     const int* done;
     const double* cop;  // non-NULL: the coarsest level's smoother as a matrix
+    int cop_lds;        // >= 0: LDS float offset of a staged copy of that matrix
 };
 
 constexpr int kSmallThreads = 1024;
@@ -704,7 +705,8 @@ __device__ inline LdsLevel lds_level(const SmallArgs& a, float* lds, int l) {
 // (the x low face shares the cell index), its parent on the next coarser
 // level, its right-hand side and omega / diag (0 where diag <= 0: no update).
 struct Slot {
-    int c, n[6], fy, fz, par;
+    int c, n[6], par;
+    float f[6];  // x-low, x-high, y-low, y-high, z-low, z-high faces
     float b, w;
 };
 
@@ -726,23 +728,28 @@ __device__ inline void make_slot(const LdsLevel& v, int colour, int t, float ome
     S.n[3] = j < d.ny - 1 ? c + SJ : c;
     S.n[4] = k > 0 ? c - 1 : c;
     S.n[5] = k < d.nz - 1 ? c + 1 : c;
-    S.fy = c + i * d.nz;        // face_index<1>(d, i, j, k)
-    S.fz = c + (i * d.ny + j);  // face_index<2>(d, i, j, k)
+    const int fy = c + i * d.nz;        // face_index<1>(d, i, j, k)
+    const int fz = c + (i * d.ny + j);  // face_index<2>(d, i, j, k)
     S.par = dc ? cell_index(*dc, i >> 1, j >> 1, k >> 1) : 0;
-    const float dg = v.bx[c] + v.bx[c + SI] + v.by[S.fy] + v.by[S.fy + SJ] + v.bz[S.fz] + v.bz[S.fz + 1];
+    S.f[0] = v.bx[c];
+    S.f[1] = v.bx[c + SI];
+    S.f[2] = v.by[fy];
+    S.f[3] = v.by[fy + SJ];
+    S.f[4] = v.bz[fz];
+    S.f[5] = v.bz[fz + 1];
+    const float dg = S.f[0] + S.f[1] + S.f[2] + S.f[3] + S.f[4] + S.f[5];
     S.b = v.b[c];
     S.w = dg > 0.0f ? omega / dg : 0.0f;
 }
 
 // A p at a slot's cell in the difference form (same term order as stencil_vals)
 __device__ inline float slot_Ap(const LdsLevel& v, const Slot& S, float pc) {
-    const int SI = v.d.ny * v.d.nz, SJ = v.d.nz;
-    float s = v.bx[S.c] * (pc - v.p[S.n[0]]);
-    s += v.bx[S.c + SI] * (pc - v.p[S.n[1]]);
-    s += v.by[S.fy] * (pc - v.p[S.n[2]]);
-    s += v.by[S.fy + SJ] * (pc - v.p[S.n[3]]);
-    s += v.bz[S.fz] * (pc - v.p[S.n[4]]);
-    s += v.bz[S.fz + 1] * (pc - v.p[S.n[5]]);
+    float s = S.f[0] * (pc - v.p[S.n[0]]);
+    s += S.f[1] * (pc - v.p[S.n[1]]);
+    s += S.f[2] * (pc - v.p[S.n[2]]);
+    s += S.f[3] * (pc - v.p[S.n[3]]);
+    s += S.f[4] * (pc - v.p[S.n[4]]);
+    s += S.f[5] * (pc - v.p[S.n[5]]);
     return s;
 }
 
@@ -878,11 +885,13 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
         // the same map, precomputed: p = M b (column j of M contiguous)
         const LdsLevel v = lds_level(a, lds, a.L - 1);
         const int n = v.d.nx * v.d.ny * v.d.nz;
+        // the matrix staged in LDS with the faces, else read from memory
+        const double* M = a.cop_lds >= 0 ? reinterpret_cast<const double*>(lds + a.cop_lds) : a.cop;
         for (int t = tid; t < n; t += nt) {
             double acc = 0.0;
             // 16 loads in flight per wait (the sum keeps its order)
 #pragma unroll 16
-            for (int j = 0; j < n; ++j) acc += a.cop[(int64_t)j * n + t] * (double)v.b[j];
+            for (int j = 0; j < n; ++j) acc += M[(int64_t)j * n + t] * (double)v.b[j];
             v.p[t] = (float)acc;
         }
         __syncthreads();
@@ -1137,8 +1146,18 @@ __device__ inline void set_vscale(CGState* st, float m) {
     st->vscale_inv = ldexpf(1.0f, e);
 }
 
+// host_flag (C_UPDATE only, may be NULL): pinned host memory that receives
+// (number of residual tests so far) * 2 + (done != 0) after every residual
+// test, with a system-scope store the polling host sees without any event.
+__device__ inline void publish_done(CGState* st, int* host_flag) {
+    if (threadIdx.x != 0) return;
+    st->seq += 1;
+    if (host_flag)
+        __hip_atomic_store(host_flag, st->seq * 2 + (st->done != 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, int64_t n,
-                                             double* out, double tol) {
+                                             double* out, double tol, int* host_flag = nullptr) {
     __shared__ double sh[5 * 16];
     __shared__ float shf[4];
     if (op == C_MEAN || op == C_MEAN_RESET) {
@@ -1149,6 +1168,7 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
             if (op == C_MEAN_RESET) {
                 st->done = 0;
                 st->it = 0;
+                st->seq = 0;
                 st->rel = 1.0;
             }
         }
@@ -1168,7 +1188,10 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
         }
         return;
     }
-    if (st->done) return;
+    if (st->done) {
+        if (op == C_UPDATE) publish_done(st, host_flag);
+        return;
+    }
     if (op == C_FIRST) {
         // mu = mean(z); rz = <r, z - mu>
         double t[3];
@@ -1200,6 +1223,7 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
             if (rel < st->tol) st->done = 1;
             set_vscale(st, mm);
         }
+        publish_done(st, host_flag);
     } else if (op == C_BETA) {
         // Polak-Ribiere with the flexible numerator (poisson.py:400-406):
         // z' = z - mean(z); rz_new = <r, z'>; num = rz_new - <r_prev, z'>
@@ -1427,7 +1451,21 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
         seg(B(sm), a.off[sm][4], ncell(mg.lev[sm]));
         a.done = done;
         a.cop = use_coarse_op(mg) ? static_cast<CGState*>(mg.scalars)->coarse_op : nullptr;
-        const size_t lds = (size_t)nf * sizeof(float);
+        a.cop_lds = -1;
+        int64_t nlds = nf;
+        if (a.cop) {
+            // stage the coarsest level's matrix too when it fits the LDS
+            // budget: its loads then overlap the faces' instead of costing
+            // round trips of their own in the middle of the V-cycle
+            const int64_t nc = ncell(mg.lev[L - 1]);
+            const int64_t off = (nf + 3) & ~(int64_t)3;  // 16-byte aligned
+            if ((off + 2 * nc * nc) * (int64_t)sizeof(float) <= kSmallMaxBytes) {
+                a.cop_lds = (int)off;
+                seg(reinterpret_cast<const float*>(a.cop), (int)off, 2 * nc * nc);
+                nlds = off + 2 * nc * nc;
+            }
+        }
+        const size_t lds = (size_t)nlds * sizeof(float);
         if (sm == 0) {
             a.p_top = z0;
             k_small_vcycle<ST><<<1, kSmallThreads, lds, s>>>(a);
@@ -1493,14 +1531,14 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
 }
 
 // Host-side state of one solver (caller-owned through pf_mgpcg_state_*):
-// a pinned ring of two done-flags the host polls, their events, and the HIP
-// graphs captured for this solver's buffers.  Bound to the device current at
+// the pinned int the residual tests publish to (the host polls it, no
+// events), and the HIP graphs captured for this solver's buffers.  Bound to the device current at
 // creation.
 struct SolveGraphs;
 struct SolverState {
     int device = -1;
-    int* flag = nullptr;  // pinned ring of 2 ints
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    int* flag = nullptr;    // pinned int the residual tests publish to (host address)
+    int* flag_d = nullptr;  // the same memory as the device addresses it
     std::vector<SolveGraphs>* graphs = nullptr;
 };
 
@@ -1508,8 +1546,11 @@ static void release_state(SolverState& st);
 
 static int init_state(SolverState& st) {
     PF_CHECK_HIP(hipGetDevice(&st.device));
-    PF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.flag), 2 * sizeof(int), hipHostMallocDefault));
-    for (int q = 0; q < 2; ++q) PF_CHECK_HIP(hipEventCreateWithFlags(&st.ev[q], hipEventDisableTiming));
+    // coherent: the device's stores reach host memory without cache flushes
+    PF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.flag), 2 * sizeof(int),
+                               hipHostMallocMapped | hipHostMallocCoherent));
+    st.flag[0] = st.flag[1] = 0;
+    PF_CHECK_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&st.flag_d), st.flag, 0));
     return PF_OK;
 }
 
@@ -1519,9 +1560,10 @@ static const float* vcycle_scale(const CGK<ST>& a) {
     return a.zscale ? &a.st->vscale : nullptr;
 }
 
-// Kernels of one FPCG iteration; `flag` (pinned host memory) receives the
-// done flag after the residual test.  Every kernel returns at once when the
-// flag is set, so an iteration enqueued past convergence does nothing.
+// Kernels of one FPCG iteration; `flag` (the device address of pinned host
+// memory) receives the done flag from the residual test's combine kernel.
+// Every kernel returns at once when the flag is set, so an iteration
+// enqueued past convergence does nothing.
 template <typename ST>
 static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, int* flag,
                              unsigned nb, hipStream_t s) {
@@ -1532,9 +1574,8 @@ static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int
     PF_CHECK_LAUNCH();
     k_cg_update<ST><<<nb, RT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
-    k_comb<<<1, RT, 0, s>>>(st, C_UPDATE, (int)nb, a.n, nullptr, 0.0);
+    k_comb<<<1, RT, 0, s>>>(st, C_UPDATE, (int)nb, a.n, nullptr, 0.0, flag);
     PF_CHECK_LAUNCH();
-    PF_CHECK_HIP(hipMemcpyAsync(flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
     int rc = vcycle<ST>(mg, a.r, z, done, vcycle_scale(a), s);
     if (rc) return rc;
     k_cg_beta<ST><<<nb, RT, 0, s>>>(a);
@@ -1559,7 +1600,7 @@ static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int*
 }
 
 // HIP graphs of the prologue and of one iteration (two instances that differ
-// only in the pinned flag slot), cached per solver layout.  A solve issues
+// only in which search-direction buffer they write), cached per solver layout.  A solve issues
 // one graph launch per iteration instead of ~50 kernel launches.
 struct SolveGraphs {
     std::vector<uint64_t> key;
@@ -1647,7 +1688,7 @@ static int get_graphs(SolverState& st, const pf_mg& mg, const CGK<ST>& a, ST* z,
     rc = capture(cs, [&](hipStream_t t) { return enqueue_prologue<ST>(mg, a, z, done, nb, t); }, &g.pro);
     for (int q = 0; q < 2 && !rc; ++q) {
         const CGK<ST> aq = slot_args(a, mg, q);
-        rc = capture(cs, [&](hipStream_t t) { return enqueue_iteration<ST>(mg, aq, z, done, &st.flag[q], nb, t); },
+        rc = capture(cs, [&](hipStream_t t) { return enqueue_iteration<ST>(mg, aq, z, done, st.flag_d, nb, t); },
                      &g.it[q]);
     }
     (void)hipStreamDestroy(cs);
@@ -1671,11 +1712,8 @@ static void release_state(SolverState& st) {
         delete st.graphs;
         st.graphs = nullptr;
     }
-    for (auto& e : st.ev)
-        if (e) (void)hipEventDestroy(e);
-    st.ev[0] = st.ev[1] = nullptr;
     if (st.flag) (void)hipHostFree(st.flag);
-    st.flag = nullptr;
+    st.flag = st.flag_d = nullptr;
 }
 
 static bool use_graphs() {
@@ -1684,6 +1722,34 @@ static bool use_graphs() {
         return !(e && e[0] == '0');
     }();
     return on;
+}
+
+// Spin until the published counter (see publish_done) reaches `seq`; *stop
+// = the solve is done.  The stream running dry without that publication is
+// an error, so the loop cannot spin forever.
+static int wait_published(const int* flag, int seq, hipStream_t s, bool* stop) {
+    const volatile int* f = flag;
+    for (unsigned spin = 0;; ++spin) {
+        const int v = *f;
+        if ((v >> 1) >= seq) {
+            *stop = (v & 1) != 0;
+            return PF_OK;
+        }
+        if ((spin & 255) == 255) {
+            const hipError_t q = hipStreamQuery(s);
+            if (q == hipSuccess) {
+                const int w = *f;
+                if ((w >> 1) >= seq) {
+                    *stop = (w & 1) != 0;
+                    return PF_OK;
+                }
+                return set_error(PF_ERR_INVALID, "pf_mgpcg_solve: residual test %d was never published", seq);
+            }
+            if (q != hipErrorNotReady)
+                return set_error((int)q, "pf_mgpcg_solve: %s", hipGetErrorString(q));
+        }
+        __builtin_ia32_pause();
+    }
 }
 
 template <typename ST>
@@ -1751,6 +1817,9 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
         if (rc) return rc;
         hp = &local;
     }
+    // the previous solve's last publication must not be read as this one's
+    // (its kernels have all completed: it ended with a stream synchronize)
+    static_cast<volatile int*>(hp->flag)[0] = 0;
     SolveGraphs* gr = nullptr;
     if (use_graphs() && max_iter > 0 && hp->graphs) {
         rc = get_graphs<ST>(*hp, mg, a, z, done, nb, gr);
@@ -1767,14 +1836,17 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
         if (gr) {
             PF_CHECK_HIP(hipGraphLaunch(gr->it[slot], s));
         } else {
-            rc = enqueue_iteration<ST>(mg, slot_args(a, mg, slot), z, done, &hp->flag[slot], nb, s);
+            rc = enqueue_iteration<ST>(mg, slot_args(a, mg, slot), z, done, hp->flag_d, nb, s);
             if (rc) return rc;
         }
-        PF_CHECK_HIP(hipEventRecord(hp->ev[slot], s));
-        // poll one iteration behind, so the GPU always has queued work
+        // poll one iteration behind, so the GPU always has queued work: wait
+        // for iteration it-1's residual test to be published, then stop if
+        // it (or an earlier one) ended the solve
         if (it >= 2) {
-            PF_CHECK_HIP(hipEventSynchronize(hp->ev[slot ^ 1]));
-            if (hp->flag[slot ^ 1] != 0) break;
+            bool stop = false;
+            rc = wait_published(hp->flag, it - 1, s, &stop);
+            if (rc) return rc;
+            if (stop) break;
         }
     }
     // _remove_mean(x) (poisson.py:408)

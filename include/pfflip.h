@@ -213,8 +213,9 @@ typedef struct pf_mg {
     float* q;
     float* Ap;
     void* scalars; /* pf_mg_scalar_bytes() of device memory */
-    /* solver host state from pf_mgpcg_state_create (pinned done-flags, events
-     * and the HIP graphs captured for these buffers), owned by the caller;
+    /* solver host state from pf_mgpcg_state_create (the pinned flag the
+     * residual tests publish to and the HIP graphs captured for these
+     * buffers), owned by the caller;
      * NULL: each pf_mgpcg_solve creates and frees its own, without graphs */
     void* host_state;
 } pf_mg;
@@ -246,8 +247,9 @@ int pf_mgpcg_state_destroy(void* state);
 /* solve (poisson.py:355-409): flexible PCG, Polak-Ribiere beta clamped at 0,
  * stop when max|r|/max|b| < tol (recursive residual, b mean-removed).
  * Writes x (mean removed); *it_out / *rel_out are host pointers.  Blocks:
- * the host polls a pinned done-flag one iteration behind the GPU to stop
- * enqueueing iterations.  The library keeps no state of its own between
+ * every residual test publishes (tests so far, done) to a pinned host int,
+ * which the host polls one iteration behind the GPU to stop enqueueing
+ * iterations (no events).  The library keeps no state of its own between
  * calls: graphs and flags live in mg->host_state. */
 int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol, int32_t max_iter,
                    int32_t* it_out, double* rel_out, pf_stream_t stream);
