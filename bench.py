@@ -176,14 +176,106 @@ def fp64_reference_pressure(solver, rhs, torch, tol=1e-12, max_refine=10):
     return p, rel
 
 
-def solve_compare(pf, torch, sim, cfg, tol):
+def nested_dissection(m):
+    """Elimination order of an m^3 box of cells by nested dissection
+    (George, "Nested dissection of a regular finite element mesh", SIAM J.
+    Numer. Anal. 10, 1973): both halves of a box, then the separator plane
+    between them, recursively.  For the 7-point operator it keeps the direct
+    factor's fill near O(N^(4/3)), far below a general-purpose ordering."""
+    import numpy as np
+    idx = np.arange(m ** 3).reshape(m, m, m)
+    order = []
+
+    def rec(box):
+        sz = [hi - lo for lo, hi in box]
+        if max(sz) <= 4:
+            order.append(idx[tuple(slice(lo, hi) for lo, hi in box)].ravel())
+            return
+        ax = int(np.argmax(sz))
+        lo, hi = box[ax]
+        mid = (lo + hi) // 2
+        for half in ((lo, mid), (mid + 1, hi)):
+            b = list(box)
+            b[ax] = half
+            rec(b)
+        b = list(box)
+        b[ax] = (mid, mid + 1)
+        order.append(idx[tuple(slice(lo2, hi2) for lo2, hi2 in b)].ravel())
+
+    rec([(0, m)] * 3)
+    return np.concatenate(order)
+
+
+def extraction_direct_solve(bx, by, bz, b0, p_ref, lo, m):
+    """FP64 scipy direct solve of the pressure system restricted to the box
+    of cells [lo, lo+m)^3, with the FP64 reference solution as Dirichlet
+    data outside it: each box row keeps A's diagonal (the sum of its six
+    face coefficients, compute_diag, pfflip/poisson.py:40-51), couples to
+    box neighbours by -beta (poisson.py:54-88), and moves beta * p_ref of a
+    neighbour outside the box to the right-hand side.  That removes the
+    constant null space, so the solve is a plain sparse LU (SuperLU,
+    nested-dissection order).  Arguments are FP64 numpy arrays of the whole
+    grid; returns (p on the box, seconds)."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    sl = (slice(lo, lo + m),) * 3
+    N = m ** 3
+    idx = np.arange(N).reshape(m, m, m)
+    diag = (bx[lo:lo + m + 1, sl[1], sl[2]][:-1] + bx[lo:lo + m + 1, sl[1], sl[2]][1:]
+            + by[sl[0], lo:lo + m + 1, sl[2]][:, :-1] + by[sl[0], lo:lo + m + 1, sl[2]][:, 1:]
+            + bz[sl[0], sl[1], lo:lo + m + 1][:, :, :-1] + bz[sl[0], sl[1], lo:lo + m + 1][:, :, 1:])
+    rhs = b0[sl].copy()
+    rows, cols, vals = [idx.ravel()], [idx.ravel()], [diag.ravel()]
+    for ax, f in enumerate((bx, by, bz)):
+        # faces of this axis from the box's low face to its high face
+        fs = [sl[0], sl[1], sl[2]]
+        fs[ax] = slice(lo, lo + m + 1)
+        fb = f[tuple(fs)]
+        inner = [slice(None)] * 3
+        inner[ax] = slice(1, m)
+        lo_c = [slice(None)] * 3
+        lo_c[ax] = slice(0, m - 1)
+        hi_c = [slice(None)] * 3
+        hi_c[ax] = slice(1, m)
+        w = fb[tuple(inner)].ravel()
+        a, c = idx[tuple(lo_c)].ravel(), idx[tuple(hi_c)].ravel()
+        rows += [a, c]
+        cols += [c, a]
+        vals += [-w, -w]
+        # faces on the box boundary: the outside neighbour is Dirichlet data
+        for side in (0, m):
+            fsel = [slice(None)] * 3
+            fsel[ax] = side
+            csel = [slice(None)] * 3
+            csel[ax] = 0 if side == 0 else m - 1
+            osel = list(sl)
+            o = lo - 1 if side == 0 else lo + m
+            if 0 <= o < p_ref.shape[ax]:
+                osel[ax] = o
+                rhs[tuple(csel)] += fb[tuple(fsel)] * p_ref[tuple(osel)]
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+    perm = nested_dissection(m)
+    t0 = time.perf_counter()
+    lu = spla.splu(A[perm][:, perm].tocsc(), permc_spec="NATURAL", options={"SymmetricMode": True})
+    y = lu.solve(rhs.ravel()[perm])
+    secs = time.perf_counter() - t0
+    x = np.empty(N)
+    x[perm] = y
+    return x.reshape(m, m, m), secs
+
+
+def solve_compare(pf, torch, sim, cfg, tol, extraction=64):
     """Time FP32, BF16 and FP16 MG-PCG on the same captured system (beta,
     rhs) and give each one's pressure error against an FP64 reference
-    solution."""
+    solution (iterative refinement over the whole grid) and, on a central
+    box of `extraction`^3 cells, against an FP64 scipy direct solve with the
+    reference as Dirichlet data (0: no extraction)."""
     out = {}
     beta = [b.clone() for b in sim.beta]
     rhs = sim.rhs.clone()
     p_ref = None
+    sols = {}
     for prec in ("fp32", "bf16", "fp16"):
         s = pf.MGPCG(sim.res, cfg.device, omega=cfg.mg_omega, pre=cfg.mg_pre_sweeps,
                      post=cfg.mg_post_sweeps, coarsest=cfg.mg_coarsest_sweeps,
@@ -206,11 +298,30 @@ def solve_compare(pf, torch, sim, cfg, tol):
         d -= d.mean()
         p_err = float(d.abs().max() / p_ref.abs().max())
         out[prec] = {"ms": 1e3 * min(times), "iters": it, "rel": rel, "p_err_vs_fp64": p_err}
+        sols[prec] = p.double().cpu().numpy()
         del s, d
+    if extraction:
+        n = sim.res[0]
+        lo = n // 2 - extraction // 2
+        b0 = rhs.double() - rhs.double().mean()
+        f64 = [t.double().cpu().numpy() for t in beta]
+        pr = p_ref.cpu().numpy()
+        pd, secs = extraction_direct_solve(*f64, b0.cpu().numpy(), pr, lo, extraction)
+        box = (slice(lo, lo + extraction),) * 3
+        scale = float(abs(pr).max())
+        out["fp64_direct_extraction"] = {
+            "box": [lo, lo + extraction], "cells": extraction ** 3,
+            "method": "scipy.sparse.linalg.splu (SuperLU, nested-dissection order), FP64, Dirichlet "
+                      "data from the FP64 reference outside the box",
+            "seconds": secs,
+            "reference_vs_direct": float(abs(pr[box] - pd).max()) / scale}
+        for prec, sol in sols.items():
+            out[prec]["p_err_vs_direct"] = float(abs(sol[box] - pd).max()) / scale
     return out
 
 
-def adjoint_check(pf, torch, n=64, ppc=8, n_steps=16, n_dirs=8, eps=1e-3, tol=1e-8):
+def adjoint_check(pf, torch, n=64, ppc=8, n_steps=16, n_dirs=8, eps=1e-3, tol=1e-8,
+                  eps_sweep=(2e-3, 5e-4)):
     """Differentiable dam break: 64^3, 8 ppc, 16 steps, FP32.  Control: a
     seeded Gaussian-random initial liquid velocity field (sigma 0.1 after a
     4-cell Gaussian smoothing), sampled trilinearly at the liquid particles.
@@ -218,7 +329,11 @@ def adjoint_check(pf, torch, n=64, ppc=8, n_steps=16, n_dirs=8, eps=1e-3, tol=1e
     (0.6, 0.5, 0.2).  Hand-written adjoint vs central finite differences
     (eps = 1e-3) along n_dirs seeded smooth directions, dt held fixed, MG-PCG
     (forward and adjoint) to `tol` so that the solver's stopping point does
-    not move the loss by more than the finite-difference signal."""
+    not move the loss by more than the finite-difference signal.  The
+    tags-fixed differences are also taken at each step of `eps_sweep`: the
+    step map is only piecewise smooth (clamps, trilinear kinks at cell
+    faces), so a central difference carries an error that shrinks with eps,
+    which the sweep shows apart from the adjoint's own."""
     import numpy as np
     import scipy.ndimage as ndi
     import torch.nn.functional as F
@@ -294,6 +409,9 @@ def adjoint_check(pf, torch, n=64, ppc=8, n_steps=16, n_dirs=8, eps=1e-3, tol=1e
                 fd = (run(sample(u0 + eps * d), frozen) - run(sample(u0 - eps * d), frozen)) / (2.0 * eps)
                 row[name] = fd
                 row["rel_" + name] = abs(fd - ad) / max(abs(fd), 1e-30)
+            for e in eps_sweep:
+                row[f"fd_tags_fixed@{e:g}"] = (run(sample(u0 + e * d), True)
+                                               - run(sample(u0 - e * d), True)) / (2.0 * e)
         rows.append(row)
     def vec_err(name):
         fd = np.array([r[name] for r in rows])
@@ -304,6 +422,9 @@ def adjoint_check(pf, torch, n=64, ppc=8, n_steps=16, n_dirs=8, eps=1e-3, tol=1e
             "particles": N, "loss": float(loss.detach()), "forward_ms": 1e3 * (t1 - t0),
             "backward_ms": 1e3 * (t2 - t1), "eps": eps, "pcg_tol": tol, "pcg_iters": iters,
             "rel_err_tags_fixed": vec_err("fd_tags_fixed"), "rel_err_free": vec_err("fd_free"),
+            "rel_err_tags_fixed_by_eps": {f"{e:g}": vec_err(f"fd_tags_fixed@{e:g}") if e != eps
+                                          else vec_err("fd_tags_fixed")
+                                          for e in sorted(set(eps_sweep) | {eps}, reverse=True)},
             "max_rel_err_tags_fixed": max(r["rel_fd_tags_fixed"] for r in rows),
             "max_rel_err_free": max(r["rel_fd_free"] for r in rows),
             "note": "the adjoint holds the liquid tags fixed; fd_tags_fixed differences the "
@@ -618,6 +739,9 @@ def main():
                     help="dambreak128: the headline benchmark (default); rt128 / surge256 / adaptive128: the "
                          "Rayleigh-Taylor, large surge and adaptive (MSBG) configurations (their own JSON line)")
     ap.add_argument("--rt-steps", type=int, default=60)
+    ap.add_argument("--extraction", type=int, default=64,
+                    help="edge of the central box whose FP64 scipy direct solve checks the solvers' "
+                         "pressure (0: none; the 64^3 box takes about a minute of host time)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy three times (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -710,7 +834,7 @@ def main():
         "stages_ms": stages,
     }
     if not args.skip_solve_compare:
-        result["mgpcg_1e-6"] = solve_compare(pf, torch, sim, cfg, 1e-6)
+        result["mgpcg_1e-6"] = solve_compare(pf, torch, sim, cfg, 1e-6, args.extraction)
     if not args.skip_adjoint:
         del sim
         torch.cuda.empty_cache()

@@ -70,3 +70,35 @@ def test_measured_traffic_records():
     assert m["bytes_per_cell"] >= bench.msbg_sweep_bytes(1) and abs(m["bytes"] - m["bytes_per_cell"] * m["cells"]) < 1.0
     assert bench.measured_traffic("dam break z-up 128^3 x 27 ppc, 1000:1, MG-PCG tol 1e-6", "bf16") is None
     assert bench.measured_traffic("no such workload", "fp32") is None
+
+
+def test_nested_dissection_is_a_permutation():
+    bench = _bench()
+    for m in (1, 5, 8, 13):
+        perm = bench.nested_dissection(m)
+        assert np.array_equal(np.sort(perm), np.arange(m ** 3))
+
+
+def test_extraction_direct_solve_reproduces_the_full_solution():
+    """The box system with Dirichlet data from the full grid's exact
+    solution has that solution as its own: an independent scipy solve of the
+    whole singular system (first unknown pinned, mean removed) restricted to
+    the box, at a 1000:1 coefficient jump.  A dropped boundary term, a sign
+    error or a transposed face array fails it."""
+    from test_oracle import direct_solve
+    bench = _bench()
+    rng = np.random.default_rng(5)
+    res = (12, 11, 13)
+    bx, by, bz = random_beta(res, rng, contrast=1000.0)
+    b = rng.standard_normal(res)
+    b -= b.mean()
+    A = assemble(res, bx, by, bz)
+    p = direct_solve(A, b).reshape(res)
+    for lo, m in ((3, 6), (0, 5), (6, 5)):   # interior, touching the low and the high walls
+        pd, secs = bench.extraction_direct_solve(bx, by, bz, b, p, lo, m)
+        box = (slice(lo, lo + m),) * 3
+        assert np.abs(pd - p[box]).max() < 1e-9 * np.abs(p).max(), (lo, m)
+        assert secs >= 0.0
+    # with wrong Dirichlet data the box solution moves
+    pd, _ = bench.extraction_direct_solve(bx, by, bz, b, p + 1.0 * (rng.random(res) > 0.5), 3, 6)
+    assert np.abs(pd - p[3:9, 3:9, 3:9]).max() > 1e-3

@@ -1,6 +1,14 @@
 """The benchmark configurations at their full size, through size-independent
 properties (MI355X).
 
+* Dam break 128^3 x 27 ppc (56.6M particles): the pressure system of a step
+  solved to 1e-6 by FP32, BF16 and FP16 MG-PCG, each within 1e-5 of an FP64
+  scipy direct solve on the central 32^3 box (Dirichlet data from an FP64
+  reference of the whole grid, which agrees with the direct solve to 1e-9).
+* Differentiable dam break 64^3 x 8 ppc, 16 steps: the hand-written adjoint
+  against central differences along 8 seeded directions, within 1e-3 at
+  eps = 5e-4, with the difference shrinking from eps = 1e-3 (the step map is
+  piecewise smooth, so a central difference's own error is O(eps)).
 * Rayleigh-Taylor 128^3 x 27 ppc (56.6M particles), FP16 MG-PCG: every step
   converges to 1e-6 within a bounded number of V-cycles, phi is exactly 0
   throughout the light phase away from the interface, mass and phase counts
@@ -41,6 +49,60 @@ def pf():
 
 def np_(t):
     return t.detach().cpu().numpy()
+
+
+def _bench():
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.mark.timeout(600)
+def test_dambreak_128_solvers_against_fp64_direct_extraction(pf):
+    bench = _bench()
+    n = 128
+    cfg = pf.Config(res=(n, n, n), dx=1.0 / n, ppc=27, gravity=(0.0, 0.0, -9.81), droplets=False,
+                    pcg_tol=1e-6, pcg_max_iter=400)
+    pos, vel, pt = pf.scenes.dam_break(cfg, frac=(0.4, 1.0, 0.6), jitter=0.5)
+    say(f"dam break {n}^3 x 27: {len(pos)} particles")
+    assert len(pos) == n ** 3 * 27
+    sim = pf.Simulator(cfg)
+    sim.set_particles(pos, vel, pt)
+    del pos, vel, pt
+    for s in range(2):
+        rec = sim.step()
+        say(f"  step {s}: pcg_iter={rec['pcg_iter']} res={rec['pcg_res']:.2e}")
+    out = bench.solve_compare(pf, torch, sim, cfg, 1e-6, extraction=32)
+    ext = out["fp64_direct_extraction"]
+    say(f"  direct 32^3 extraction in {ext['seconds']:.1f} s, reference vs direct "
+        f"{ext['reference_vs_direct']:.2e}")
+    assert out["fp64_reference"]["rel_residual"] < 1e-11
+    assert ext["reference_vs_direct"] < 1e-9
+    for prec in ("fp32", "bf16", "fp16"):
+        r = out[prec]
+        say(f"  {prec}: {r['iters']} it, {r['ms']:.2f} ms, vs direct {r['p_err_vs_direct']:.2e}, "
+            f"vs FP64 reference {r['p_err_vs_fp64']:.2e}")
+        assert r["rel"] < 1e-6
+        assert r["p_err_vs_direct"] < 1e-5 and r["p_err_vs_fp64"] < 1e-5
+    assert out["bf16"]["iters"] <= out["fp32"]["iters"] + 4
+    assert out["fp16"]["iters"] <= out["fp32"]["iters"] + 4
+
+
+@pytest.mark.timeout(600)
+def test_differentiable_dambreak_64_adjoint_matches_finite_differences(pf):
+    bench = _bench()
+    say("differentiable dam break 64^3 x 8 ppc, 16 steps")
+    r = bench.adjoint_check(pf, torch, eps=5e-4, eps_sweep=(1e-3,))
+    by = r["rel_err_tags_fixed_by_eps"]
+    say(f"  loss {r['loss']:.6f}, forward {r['forward_ms']:.0f} ms, backward {r['backward_ms']:.0f} ms, "
+        f"aggregate relative error by eps {by}")
+    assert np.isfinite(r["loss"])
+    assert by["0.0005"] < 1e-3
+    assert by["0.0005"] < by["0.001"]
 
 
 @pytest.mark.timeout(600)
