@@ -107,7 +107,7 @@ class MRMultigrid:
                 f = self.levels[i - 1]
                 _lib.call("pf_msbg_coarsen_beta", lv.grid.ref(), f.grid.ref(), _lib.ptr(f.bface),
                           _lib.ptr(lv.bface), _lib.stream())
-            mro.compute_diag(lv.grid, lv.bface, lv.diag)
+            mro._diag(lv.grid, lv.bface, lv.diag)
         if self._coarse_op is not None:
             c = self.levels[-1]
             _lib.call("pf_msbg_coarse_op", c.grid.ref(), _lib.ptr(c.bface), _lib.ptr(c.diag), self.coarsest_sweeps,
@@ -122,7 +122,7 @@ class MRMultigrid:
         colours = (1, 0) if reverse else (0, 1)
         for _ in range(n_sweeps):
             for c in colours:
-                mro.rbgs_half_sweep(lv.grid, lv.p, lv.p2, lv.b, lv.bface, lv.diag, c, self.omega)
+                mro._rbgs(lv.grid, lv.p, lv.p2, lv.b, lv.bface, lv.diag, c, self.omega)
                 lv.p, lv.p2 = lv.p2, lv.p
 
     def _vcycle(self, i=0):
@@ -140,7 +140,7 @@ class MRMultigrid:
             remove_mean(lv.p, self._work)
             return
         self._smooth(lv, self.pre, index=i)
-        mro.residual(lv.grid, lv.p, lv.b, lv.bface, lv.r)
+        mro._residual(lv.grid, lv.p, lv.b, lv.bface, lv.r)
         c = self.levels[i + 1]
         c.p.zero_()
         _lib.call("pf_msbg_restrict", lv.grid.ref(), c.grid.ref(), _lib.ptr(lv.r), _lib.ptr(c.b), _lib.stream())

@@ -39,7 +39,7 @@ class MRSolver:
         self._host = torch.empty(8, dtype=torch.float64, pin_memory=True)  # the per-iteration state read
 
     def prepare(self, bface):
-        mro.compute_diag(self.grid, bface, self.diag)
+        mro._diag(self.grid, bface, self.diag)
         if self.mg is not None:
             self.mg.build_hierarchy(bface)
 
@@ -76,7 +76,7 @@ class MRSolver:
         it, rel = 0, 1.0
         host = self._host
         for it in range(1, max_iter + 1):
-            mro.apply_A(self.grid, self.q, bface, self.Ap)
+            mro._apply(self.grid, self.q, bface, self.Ap)
             _lib.call("pf_vec_reduce", 1, _lib.ptr(self.q), _lib.ptr(self.Ap), self.n, _lib.ptr(W[2]), s)
             _lib.call("pf_mrcg_scalar", 2, _lib.ptr(st), _lib.ptr(flag), _lib.ptr(W[2]), None, 0.0, s)
             _lib.call("pf_vec_axpby_dev", _lib.ptr(st), 1, 1.0, _lib.ptr(self.q), -1, _lib.ptr(p_out), self.n,

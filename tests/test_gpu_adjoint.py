@@ -209,12 +209,16 @@ def test_rollout_matches_finite_differences(pf, n_steps):
         ad = dot(g, d)
         errs.append(abs(fd - ad) / max(abs(fd), 1e-12))
         say(f"  fd={fd:.6e} ad={ad:.6e} rel={errs[-1]:.2e}")
-    assert max(errs) < 1e-2, errs
+    # measured: at most 1.4e-4 over these directions
+    assert max(errs) < 1e-3, errs
 
 
 def test_rollout_with_solids_matches_finite_differences(pf):
     """The surge-on-a-column scene: solid faces, the push-out of particles
-    entering the column, and the adjoint's treatment of both."""
+    entering the column, and the adjoint's treatment of both.  A push-out is
+    a jump in position, so a central difference straddling one carries an
+    O(1) error for that particle; measured errors are 3e-3 - 8e-3, against
+    the 1e-2 bound."""
     say("adjoint with a solid column, 2 steps")
     n = 24
     cfg = pf.Config(res=(n, n, n), dx=1.0 / n, ppc=8, droplets=False, gravity=(0.0, 0.0, -9.81),
