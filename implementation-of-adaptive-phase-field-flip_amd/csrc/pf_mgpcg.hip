@@ -8,6 +8,7 @@
 // one workgroup with all its data in LDS.  The FPCG scalars live on the
 // device; dot products are deterministic FP64 block reductions, and the
 // host only polls a "done" flag one iteration behind the GPU.
+#include <algorithm>
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -22,19 +23,28 @@ int fill_u32(void* dst, int64_t n, uint32_t v, hipStream_t s);
 
 constexpr int MAXB = 1024;  // blocks of a reduction launch
 constexpr int RT = 256;     // threads of element-wise / reduction kernels
+// k_cg_apply and k_cg_beta: few, wide blocks, so the kernel that combines
+// their partials itself reads few of them
+constexpr int WT = 1024;
+constexpr int WB = 256;
 
 // Largest coarsest level whose smoother is applied as a precomputed matrix.
 constexpr int kCoarseOpMax = 512;
 
 struct CGState {
-    double rz, alpha, beta, mu, b_norm, rel, qAq, mu_x, tol;
-    int it, done, seq, pad1;  // seq: residual tests run in this solve
+    double alpha, beta, mu, b_norm, rel, qAq, mu_x, tol;
+    // <r, z - mu> of the iteration whose search direction parity is the
+    // index: the k_cg_apply of iteration `slot` forms its value from the
+    // previous one's, so the two never share a word
+    double rz[2];
+    int it, done, seq;  // seq: residual tests run in this solve
+    int beta_parts;     // 1: the FPCG beta partials in part[0..5) are this iteration's to combine
     // 16-bit V-cycles: the level-0 right-hand side is read times vscale (a
     // power of two putting max|r| in [1/2, 1)) and z is read back times
     // vscale_inv, so the FP16 iterates stay in range as r shrinks
     float vscale, vscale_inv;
     unsigned ticket[8];
-    double part[6 * MAXB];
+    double part[6 * MAXB];  // [0, 5 MAXB): dot-product partials; [5 MAXB, 6 MAXB): <q, Aq> partials
     float fpart[MAXB];
     // the coarsest level's `coarsest` red-black sweeps from zero followed by
     // mean removal, a linear map of its right-hand side: column j (the image
@@ -1017,6 +1027,9 @@ struct CGK {
     const float* zscale;  // NULL: z as stored; else z times *zscale
     CGState* st;
     double tol;
+    int slot;      // parity of the iteration (selects q / q_old and rz)
+    int np_apply;  // blocks of k_cg_apply (its <q, Aq> partials)
+    int np_beta;   // blocks of k_cg_beta (its dot-product partials)
 };
 
 template <typename ZT>
@@ -1063,12 +1076,43 @@ __global__ __launch_bounds__(RT) void k_cg_first(CGK<ZT> a) {
 // on the first iteration), evaluated at the cell and its six neighbours;
 // then Ap = A q and the partial <q, Ap> (poisson.py:385-386).  Writes q and Ap
 // of the cell; q_old is a different buffer, so neighbours read old values.
+// From the second iteration on it first combines k_cg_beta's partials into
+// the mean of z and the Polak-Ribiere beta (poisson.py:400-406) itself:
+// every block sums them in the same fixed order, so all blocks hold the same
+// scalars, and block 0 records them (no separate combine launch).
 template <typename ZT>
-__global__ __launch_bounds__(RT) void k_cg_apply(CGK<ZT> a) {
-    __shared__ double sh[16];
-    if (a.st->done) return;
-    const float cm = (float)(-a.st->mu);
-    const float be = (float)a.st->beta;
+__global__ __launch_bounds__(WT) void k_cg_apply(CGK<ZT> a) {
+    __shared__ double sh[5 * 16];
+    __shared__ float s_cm, s_be;
+    CGState* st = a.st;
+    if (st->done) return;
+    if (st->beta_parts) {
+        // z' = z - mean(z); rz_new = <r, z'>; num = rz_new - <r_prev, z'> =
+        // -alpha <Ap, z'> because r_prev = r + alpha Ap
+        double t[5];
+        combine_sums<5>(st->part, a.np_beta, t, sh);
+        if (threadIdx.x == 0) {
+            const double mu = t[0] / (double)a.n;
+            const double rz_new = t[3] - mu * t[1];
+            const double apz = t[4] - mu * t[2];
+            const double num = -st->alpha * apz;
+            const double rz = st->rz[a.slot ^ 1];
+            const double beta = (rz != 0.0) ? fmax(num / rz, 0.0) : 0.0;
+            s_cm = (float)(-mu);
+            s_be = (float)beta;
+            if (blockIdx.x == 0) {
+                st->mu = mu;
+                st->beta = beta;
+                st->rz[a.slot] = rz_new;
+            }
+        }
+    } else if (threadIdx.x == 0) {  // the first iteration: k_comb(C_FIRST) set them
+        s_cm = (float)(-st->mu);
+        s_be = (float)st->beta;
+    }
+    __syncthreads();
+    const float cm = s_cm;
+    const float be = s_be;
     const float zs = a.zscale ? *a.zscale : 1.0f;
     double v[1] = {0.0};
     const Dims d = a.d;
@@ -1076,7 +1120,7 @@ __global__ __launch_bounds__(RT) void k_cg_apply(CGK<ZT> a) {
         const float zz = zval(a, zs, c) + cm;
         return zz + be * a.q_old[c];
     };
-    for (int64_t c = (int64_t)blockIdx.x * RT + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * RT) {
+    for (int64_t c = (int64_t)blockIdx.x * WT + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * WT) {
         const int k = (int)(c % d.nz), j = (int)((c / d.nz) % d.ny), i = (int)(c / ((int64_t)d.nz * d.ny));
         const float f[6] = {a.bx[face_index<0>(d, i, j, k)], a.bx[face_index<0>(d, i + 1, j, k)],
                             a.by[face_index<1>(d, i, j, k)], a.by[face_index<1>(d, i, j + 1, k)],
@@ -1092,15 +1136,42 @@ __global__ __launch_bounds__(RT) void k_cg_apply(CGK<ZT> a) {
         v[0] += (double)pc * (double)Ap;
     }
     block_sum<1>(v, sh);
-    if (threadIdx.x == 0) a.st->part[blockIdx.x] = v[0];
+    if (threadIdx.x == 0) st->part[5 * MAXB + blockIdx.x] = v[0];
 }
 
-// x += alpha q; r -= alpha Ap; partial max|r| (poisson.py:390-394)
+// x += alpha q; r -= alpha Ap; partial max|r| (poisson.py:390-394), after
+// combining k_cg_apply's partials into <q, Aq> and alpha = rz / <q, Aq> in
+// every block (fixed order, so all blocks agree); <q, Aq> <= 0 ends the
+// solve before any update (poisson.py:386-389).  Block 0 records the
+// scalars and the iteration count.
 template <typename ZT>
 __global__ __launch_bounds__(RT) void k_cg_update(CGK<ZT> a) {
     __shared__ float sh[4];
-    if (a.st->done) return;
-    const float al = (float)a.st->alpha, nal = (float)(-a.st->alpha);
+    __shared__ double shd[16];
+    __shared__ float s_al;
+    __shared__ int s_ok;
+    CGState* st = a.st;
+    if (st->done) return;
+    double t[1];
+    combine_sums<1>(st->part + 5 * MAXB, a.np_apply, t, shd);
+    if (threadIdx.x == 0) {
+        const double qAq = t[0];
+        const bool ok = qAq > 0.0;
+        const double alpha = ok ? st->rz[a.slot] / qAq : 0.0;
+        s_ok = ok ? 1 : 0;
+        s_al = (float)alpha;
+        if (blockIdx.x == 0) {
+            st->qAq = qAq;
+            st->it += 1;
+            if (ok)
+                st->alpha = alpha;
+            else
+                st->done = 1;
+        }
+    }
+    __syncthreads();
+    if (!s_ok) return;
+    const float al = s_al, nal = -s_al;
     float m = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
         a.x[i] = a.x[i] + al * a.q[i];
@@ -1114,12 +1185,12 @@ __global__ __launch_bounds__(RT) void k_cg_update(CGK<ZT> a) {
 
 // partials of sum(z), sum(r), sum(Ap), <r,z>, <Ap,z> after a V-cycle
 template <typename ZT>
-__global__ __launch_bounds__(RT) void k_cg_beta(CGK<ZT> a) {
+__global__ __launch_bounds__(WT) void k_cg_beta(CGK<ZT> a) {
     __shared__ double sh[5 * 16];
     if (a.st->done) return;
     const float zs = a.zscale ? *a.zscale : 1.0f;
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
+    for (int64_t i = (int64_t)blockIdx.x * WT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * WT) {
         const double zz = (double)zval(a, zs, i), rr = (double)a.r[i], ap = (double)a.Ap[i];
         v[0] += zz;
         v[1] += rr;
@@ -1128,12 +1199,14 @@ __global__ __launch_bounds__(RT) void k_cg_beta(CGK<ZT> a) {
         v[4] += ap * zz;
     }
     block_sum<5>(v, sh);
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         for (int q = 0; q < 5; ++q) a.st->part[q * MAXB + blockIdx.x] = v[q];
+        if (blockIdx.x == 0) a.st->beta_parts = 1;  // read by the next k_cg_apply
+    }
 }
 
 // One-block combine of the previous kernel's partials, then the scalar step.
-enum CombOp { C_MEAN = 0, C_MEAN_RESET, C_INIT, C_FIRST, C_APPLY, C_UPDATE, C_BETA };
+enum CombOp { C_MEAN = 0, C_MEAN_RESET, C_INIT, C_FIRST, C_UPDATE };
 
 // vscale = 2^-e with m = f 2^e, f in [1/2, 1): m * vscale in [1/2, 1)
 __device__ inline void set_vscale(CGState* st, float m) {
@@ -1169,6 +1242,7 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
                 st->done = 0;
                 st->it = 0;
                 st->seq = 0;
+                st->beta_parts = 0;
                 st->rel = 1.0;
             }
         }
@@ -1197,22 +1271,12 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
         double t[3];
         combine_sums<3>(st->part, nparts, t, sh);
         if (threadIdx.x == 0) {
+            // iteration 1 has parity 1: its k_cg_update reads rz[1]
             const double mu = t[0] / (double)n;
             st->mu = mu;
-            st->rz = t[2] - mu * t[1];
+            st->rz[1] = t[2] - mu * t[1];
             st->beta = 0.0;
-        }
-    } else if (op == C_APPLY) {
-        // qAq <= 0 ends the solve before any update (poisson.py:386-389)
-        double t[1];
-        combine_sums<1>(st->part, nparts, t, sh);
-        if (threadIdx.x == 0) {
-            st->qAq = t[0];
-            st->it += 1;
-            if (t[0] <= 0.0)
-                st->done = 1;
-            else
-                st->alpha = st->rz / t[0];
+            st->beta_parts = 0;
         }
     } else if (op == C_UPDATE) {
         // rel = max|r| / max|b| (poisson.py:394-398)
@@ -1224,22 +1288,6 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
             set_vscale(st, mm);
         }
         publish_done(st, host_flag);
-    } else if (op == C_BETA) {
-        // Polak-Ribiere with the flexible numerator (poisson.py:400-406):
-        // z' = z - mean(z); rz_new = <r, z'>; num = rz_new - <r_prev, z'>
-        // = -alpha <Ap, z'> because r_prev = r + alpha Ap.
-        double t[5];
-        combine_sums<5>(st->part, nparts, t, sh);
-        if (threadIdx.x == 0) {
-            const double mu = t[0] / (double)n;
-            const double rz_new = t[3] - mu * t[1];
-            const double apz = t[4] - mu * t[2];
-            const double num = -st->alpha * apz;
-            const double rz = st->rz;
-            st->beta = (rz != 0.0) ? fmax(num / rz, 0.0) : 0.0;
-            st->mu = mu;
-            st->rz = rz_new;
-        }
     }
 }
 
@@ -1568,9 +1616,7 @@ template <typename ST>
 static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, int* flag,
                              unsigned nb, hipStream_t s) {
     CGState* st = a.st;
-    k_cg_apply<ST><<<nb, RT, 0, s>>>(a);
-    PF_CHECK_LAUNCH();
-    k_comb<<<1, RT, 0, s>>>(st, C_APPLY, (int)nb, a.n, nullptr, 0.0);
+    k_cg_apply<ST><<<a.np_apply, WT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
     k_cg_update<ST><<<nb, RT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
@@ -1578,9 +1624,7 @@ static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int
     PF_CHECK_LAUNCH();
     int rc = vcycle<ST>(mg, a.r, z, done, vcycle_scale(a), s);
     if (rc) return rc;
-    k_cg_beta<ST><<<nb, RT, 0, s>>>(a);
-    PF_CHECK_LAUNCH();
-    k_comb<<<1, RT, 0, s>>>(st, C_BETA, (int)nb, a.n, nullptr, 0.0);
+    k_cg_beta<ST><<<a.np_beta, WT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }
@@ -1639,6 +1683,7 @@ static CGK<ST> slot_args(const CGK<ST>& a, const pf_mg& mg, int slot) {
     CGK<ST> b = a;
     b.q = slot ? mg.rhs : mg.q;
     b.q_old = slot ? mg.q : mg.rhs;
+    b.slot = slot;
     return b;
 }
 
@@ -1780,6 +1825,8 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
     a.st = st;
     a.tol = tol;
     const unsigned nb = red_blocks(n);
+    a.slot = 1;
+    a.np_apply = a.np_beta = (int)std::min<int64_t>(WB, (n + WT - 1) / WT);
     const int* done = &st->done;
 
     k_part_sum<float><<<nb, RT, 0, s>>>(b, n, st->part);
