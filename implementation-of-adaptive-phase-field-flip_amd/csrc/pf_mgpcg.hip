@@ -54,6 +54,49 @@ struct CGState {
     double coarse_op[kCoarseOpMax * kCoarseOpMax];
 };
 
+// vscale = 2^-e with m = f 2^e, f in [1/2, 1): m * vscale in [1/2, 1)
+__device__ inline int vscale_exponent(float m) {
+    int e = 0;
+    if (m > 0.0f && m < 3.0e38f) {
+        (void)frexpf(m, &e);
+        e = max(min(e, 120), -120);
+    }
+    return e;
+}
+
+__device__ inline void set_vscale(CGState* st, float m) {
+    const int e = vscale_exponent(m);
+    st->vscale = ldexpf(1.0f, -e);
+    st->vscale_inv = ldexpf(1.0f, e);
+}
+
+// The residual test (poisson.py:394-398) on m = max|r|, recorded by one
+// thread: rel, done, the 16-bit input scale, then (number of tests so far) *
+// 2 + (done != 0) to host_flag (pinned host memory, may be NULL) with a
+// system-scope store the polling host sees without any event.  A solve that
+// is already done only publishes.
+__device__ inline void residual_test_record(CGState* st, float m, int* host_flag) {
+    if (!st->done) {
+        const double rel = (double)m / st->b_norm;
+        st->rel = rel;
+        if (rel < st->tol) st->done = 1;
+        set_vscale(st, m);
+    }
+    st->seq += 1;
+    if (host_flag)
+        __hip_atomic_store(host_flag, st->seq * 2 + (st->done != 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The residual test folded into the V-cycle's first kernel (the zero-start
+// level-0 sweep): every block takes the max of k_cg_update's partials itself
+// (max is exact, so all blocks agree) and so knows whether to store and the
+// 16-bit input scale; block 0 records the test.  st == NULL: no fold.
+struct UpdateFold {
+    CGState* st;
+    int nparts;
+    int* host_flag;
+};
+
 static unsigned red_blocks(int64_t n) {
     const int64_t nb = cdiv(n, (int64_t)RT * 4);
     return (unsigned)(nb < 1 ? 1 : (nb > MAXB ? MAXB : nb));
@@ -331,10 +374,19 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
                                                    const float* __restrict__ b, int c1, float omega,
                                                    const int* __restrict__ done,
                                                    const float* __restrict__ bscale,
-                                                   const CT* __restrict__ pcoarse, Dims dc) {
+                                                   const CT* __restrict__ pcoarse, Dims dc, UpdateFold uf) {
     using T = RbTile<TI, TJ, TK, NTH>;
     __shared__ __attribute__((aligned(16))) float P[T::NP];
+    __shared__ float s_fm[NTH / 64];
     const int tid = threadIdx.x;
+    // the residual test's inputs, loaded with everything else
+    const bool fold = uf.st != nullptr;
+    int was_done = 0;
+    float fm = 0.0f;
+    if (fold) {
+        was_done = uf.st->done;
+        for (int e = tid; e < uf.nparts; e += T::NT) fm = fmaxf(fm, uf.st->fpart[e]);
+    }
     const int ntk = (d.nz + TK - 1) / TK, ntj = (d.ny + TJ - 1) / TJ;
     const int tile = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = (tile / (ntk * ntj)) * TI, j0 = ((tile / ntk) % ntj) * TJ, k0 = (tile % ntk) * TK;
@@ -461,16 +513,10 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
             sb[r] = b[cell_index(d, i, j, k)];
         }
     }
-    const bool skip = done && *done;
-    if (bscale) {
-        const float bs = *bscale;
-#pragma unroll
-        for (int q = 0; q < T::PPT; ++q) {
-            bb[q][0] *= bs;
-            bb[q][1] *= bs;
-        }
-#pragma unroll
-        for (int r = 0; r < T::SHT; ++r) sb[r] *= bs;
+    bool skip = !fold && done && *done;
+    if (fold) {
+        fm = wave_max(fm);
+        if ((tid & 63) == 0) s_fm[tid >> 6] = fm;
     }
 
     if (ZINIT) {
@@ -479,6 +525,27 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
         __builtin_amdgcn_s_waitcnt(0);  // the LDS-destined loads have landed
     }
     __syncthreads();
+    float bs = 1.0f;
+    if (fold) {
+        float mm = s_fm[0];
+#pragma unroll
+        for (int w = 1; w < NTH / 64; ++w) mm = fmaxf(mm, s_fm[w]);
+        const bool conv = (double)mm / uf.st->b_norm < uf.st->tol;
+        skip = was_done || conv;
+        bs = ldexpf(1.0f, -vscale_exponent(mm));
+        if (blockIdx.x == 0 && tid == 0) residual_test_record(uf.st, mm, uf.host_flag);
+    } else if (bscale) {
+        bs = *bscale;
+    }
+    if (bscale) {
+#pragma unroll
+        for (int q = 0; q < T::PPT; ++q) {
+            bb[q][0] *= bs;
+            bb[q][1] *= bs;
+        }
+#pragma unroll
+        for (int r = 0; r < T::SHT; ++r) sb[r] *= bs;
+    }
     if constexpr (!F32) {
         if (!ZINIT) {
             // raw pairs -> floats (LDS index 2 * pair index); past either end
@@ -1208,26 +1275,6 @@ __global__ __launch_bounds__(WT) void k_cg_beta(CGK<ZT> a) {
 // One-block combine of the previous kernel's partials, then the scalar step.
 enum CombOp { C_MEAN = 0, C_MEAN_RESET, C_INIT, C_FIRST, C_UPDATE };
 
-// vscale = 2^-e with m = f 2^e, f in [1/2, 1): m * vscale in [1/2, 1)
-__device__ inline void set_vscale(CGState* st, float m) {
-    int e = 0;
-    if (m > 0.0f && m < 3.0e38f) {
-        (void)frexpf(m, &e);
-        e = max(min(e, 120), -120);
-    }
-    st->vscale = ldexpf(1.0f, -e);
-    st->vscale_inv = ldexpf(1.0f, e);
-}
-
-// host_flag (C_UPDATE only, may be NULL): pinned host memory that receives
-// (number of residual tests so far) * 2 + (done != 0) after every residual
-// test, with a system-scope store the polling host sees without any event.
-__device__ inline void publish_done(CGState* st, int* host_flag) {
-    if (threadIdx.x != 0) return;
-    st->seq += 1;
-    if (host_flag)
-        __hip_atomic_store(host_flag, st->seq * 2 + (st->done != 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
 
 __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, int64_t n,
                                              double* out, double tol, int* host_flag = nullptr) {
@@ -1262,10 +1309,13 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
         }
         return;
     }
-    if (st->done) {
-        if (op == C_UPDATE) publish_done(st, host_flag);
+    if (op == C_UPDATE) {
+        // rel = max|r| / max|b| (poisson.py:394-398)
+        const float mm = st->done ? 0.0f : combine_max(st->fpart, nparts, shf);
+        if (threadIdx.x == 0) residual_test_record(st, mm, host_flag);
         return;
     }
+    if (st->done) return;
     if (op == C_FIRST) {
         // mu = mean(z); rz = <r, z - mu>
         double t[3];
@@ -1278,16 +1328,6 @@ __global__ __launch_bounds__(RT) void k_comb(CGState* st, int op, int nparts, in
             st->beta = 0.0;
             st->beta_parts = 0;
         }
-    } else if (op == C_UPDATE) {
-        // rel = max|r| / max|b| (poisson.py:394-398)
-        const float mm = combine_max(st->fpart, nparts, shf);
-        if (threadIdx.x == 0) {
-            const double rel = (double)mm / st->b_norm;
-            st->rel = rel;
-            if (rel < st->tol) st->done = 1;
-            set_vscale(st, mm);
-        }
-        publish_done(st, host_flag);
     }
 }
 
@@ -1378,20 +1418,20 @@ template <typename ST, bool ZINIT, bool PROL = false, typename CT = float>
 static int launch_full_sweep(const Dims& d, const ST* fx, const ST* fy, const ST* fz, const ST* pin,
                              ST* pout, const float* b, int c1, float omega, const int* done,
                              const float* bscale, hipStream_t s, const CT* pcoarse = nullptr,
-                             Dims dc = Dims{1, 1, 1}) {
+                             Dims dc = Dims{1, 1, 1}, UpdateFold uf = UpdateFold{nullptr, 0, nullptr}) {
     const int64_t n = (int64_t)d.nx * d.ny * d.nz;
     auto tiles = [&](int ti, int tj, int tk) {
         return (unsigned)(cdiv(d.nx, ti) * cdiv(d.ny, tj) * cdiv(d.nz, tk));
     };
     if (n >= (int64_t)1 << 20)
         k_rbgs_full<ST, ZINIT, 8, 8, 64, 512, PROL, CT><<<tiles(8, 8, 64), 512, 0, s>>>(
-            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc);
+            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc, uf);
     else if (n >= (int64_t)1 << 17)
         k_rbgs_full<ST, ZINIT, 8, 8, 16, 256, PROL, CT><<<tiles(8, 8, 16), 256, 0, s>>>(
-            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc);
+            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc, uf);
     else
         k_rbgs_full<ST, ZINIT, 4, 8, 16, 256, PROL, CT><<<tiles(4, 8, 16), 256, 0, s>>>(
-            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc);
+            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc, uf);
     PF_CHECK_LAUNCH();
     return PF_OK;
 }
@@ -1411,7 +1451,7 @@ static bool fused_smoother(const pf_mg& mg, int l) {
 // One V-cycle: z0 = M^{-1} r0 (no mean removal at level 0).
 template <typename ST>
 static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, const float* bscale,
-                  hipStream_t s) {
+                  hipStream_t s, const UpdateFold* uf = nullptr) {
     const int L = mg.nlev;
     const int sm = mg.small_from;             // first LDS level (L if none)
     const int G = sm < L ? sm : L - 1;        // levels 0..G-1 run out of HBM with restriction
@@ -1437,9 +1477,12 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
             for (int sw = 0; sw < sweeps; ++sw) {
                 ST* out = other(l, cur[l]);
                 const int c1 = reverse ? 1 : 0;
+                // the V-cycle's first kernel takes the residual test with it
+                const UpdateFold f = (uf && l == 0 && sw == 0) ? *uf : UpdateFold{nullptr, 0, nullptr};
                 const int rc = (from_zero && sw == 0)
                                    ? launch_full_sweep<ST, true>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done,
-                                                                 bsc, s)
+                                                                 bsc, s, static_cast<const float*>(nullptr),
+                                                                 Dims{1, 1, 1}, f)
                                    : launch_full_sweep<ST, false>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done,
                                                                   bsc, s);
                 if (rc) return rc;
@@ -1612,6 +1655,12 @@ static const float* vcycle_scale(const CGK<ST>& a) {
 // memory) receives the done flag from the residual test's combine kernel.
 // Every kernel returns at once when the flag is set, so an iteration
 // enqueued past convergence does nothing.
+// The residual test rides in the V-cycle's first kernel when that is the
+// fused zero-start sweep of an HBM level 0.
+static bool fold_residual_test(const pf_mg& mg) {
+    return mg.small_from > 0 && mg.pre > 0 && fused_smoother(mg, 0);
+}
+
 template <typename ST>
 static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, int* flag,
                              unsigned nb, hipStream_t s) {
@@ -1620,9 +1669,13 @@ static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int
     PF_CHECK_LAUNCH();
     k_cg_update<ST><<<nb, RT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
-    k_comb<<<1, RT, 0, s>>>(st, C_UPDATE, (int)nb, a.n, nullptr, 0.0, flag);
-    PF_CHECK_LAUNCH();
-    int rc = vcycle<ST>(mg, a.r, z, done, vcycle_scale(a), s);
+    const UpdateFold uf{st, (int)nb, flag};
+    const bool fold = fold_residual_test(mg);
+    if (!fold) {
+        k_comb<<<1, RT, 0, s>>>(st, C_UPDATE, (int)nb, a.n, nullptr, 0.0, flag);
+        PF_CHECK_LAUNCH();
+    }
+    int rc = vcycle<ST>(mg, a.r, z, done, vcycle_scale(a), s, fold ? &uf : nullptr);
     if (rc) return rc;
     k_cg_beta<ST><<<a.np_beta, WT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
