@@ -599,6 +599,72 @@ def test_extrapolation_matches_oracle(pf):
         assert rel_err(np_(sim.vel_liq[ax]), ref[ax]) < 1e-6
 
 
+def _extrap_torch(vel, is_liq, sweeps):
+    """seed_liquid_velocity + extrapolate_sweep (pfflip/kernels.py:406-462)
+    restated with torch FP32 array operations in the kernels' arithmetic and
+    neighbour order (x-, x+, y-, y+, z-, z+)."""
+    out = []
+    for A in range(3):
+        v = vel[A]
+        liq = torch.zeros(v.shape, dtype=torch.int32, device=v.device)
+        n = is_liq.shape[A]
+        hi = [slice(None)] * 3
+        hi[A] = slice(0, n)
+        lo = [slice(None)] * 3
+        lo[A] = slice(1, n + 1)
+        liq[tuple(hi)] = torch.maximum(liq[tuple(hi)], is_liq)
+        liq[tuple(lo)] = torch.maximum(liq[tuple(lo)], is_liq)
+        known = liq == 1
+        val = torch.where(known, v, torch.zeros_like(v))
+        for _ in range(sweeps):
+            acc = torch.zeros_like(v)
+            cnt = torch.zeros(v.shape, dtype=torch.int32, device=v.device)
+            for ax in range(3):
+                for sgn in (-1, 1):
+                    nk = torch.zeros_like(known)
+                    nv = torch.zeros_like(v)
+                    dst = [slice(None)] * 3
+                    src = [slice(None)] * 3
+                    m = v.shape[ax]
+                    if sgn < 0:
+                        dst[ax], src[ax] = slice(1, m), slice(0, m - 1)
+                    else:
+                        dst[ax], src[ax] = slice(0, m - 1), slice(1, m)
+                    nk[tuple(dst)] = known[tuple(src)]
+                    nv[tuple(dst)] = val[tuple(src)]
+                    acc = acc + torch.where(nk, nv, torch.zeros_like(nv))
+                    cnt = cnt + nk.to(torch.int32)
+            upd = (~known) & (cnt > 0)
+            val = torch.where(upd, acc / cnt.clamp(min=1).to(torch.float32), val)
+            known = known | upd
+        out.append(val)
+    return out
+
+
+@pytest.mark.parametrize("sweeps", [0, 1, 3, 4])
+def test_extrapolation_equals_torch_restatement(pf, sweeps):
+    """pf_extrapolate gives the bits of a torch FP32 restatement in the same
+    arithmetic and neighbour order, on an uneven grid with a scattered
+    liquid pattern (the oracle test above compares with FP64 to 1e-6)."""
+    say(f"extrapolation tiled vs torch, sweeps={sweeps}")
+    res = (20, 13, 37)
+    L = sys.modules[pf.__name__ + "._lib"]
+    rng = np.random.default_rng(40 + sweeps)
+    is_liq = torch.from_numpy((rng.random(res) < 0.3).astype(np.int32)).cuda()
+    shapes = [(res[0] + 1, res[1], res[2]), (res[0], res[1] + 1, res[2]), (res[0], res[1], res[2] + 1)]
+    vel = [torch.from_numpy(rng.standard_normal(s).astype(np.float32)).cuda() for s in shapes]
+    out = [torch.full(s, 7.0, device="cuda") for s in shapes]
+    tmp = [torch.empty(s, device="cuda") for s in shapes]
+    kn = [torch.empty(s, dtype=torch.uint8, device="cuda") for s in shapes]
+    kn2 = [torch.empty(s, dtype=torch.uint8, device="cuda") for s in shapes]
+    L.call("pf_extrapolate", L.ptr3(vel), L.ptr(is_liq), L.ptr3(out), L.ptr3(tmp), L.ptr3(kn), L.ptr3(kn2),
+           *res, sweeps, L.stream())
+    torch.cuda.synchronize()
+    ref = _extrap_torch(vel, is_liq, sweeps)
+    for ax in range(3):
+        assert torch.equal(out[ax], ref[ax]), ax
+
+
 # ------------------------------------------------------------------ G2P / advection
 def test_g2p_advect_match_oracle(pf):
     say("g2p + advect 16^3")
