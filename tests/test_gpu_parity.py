@@ -498,12 +498,14 @@ def test_mgpcg_matches_oracle_iterations(pf):
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("res,pre,post", [((32, 32, 32), 3, 3), ((48, 40, 24), 3, 3),
                                           ((64, 64, 64), 2, 1), ((128, 64, 32), 1, 2),
-                                          ((128, 128, 128), 3, 3), ((68, 136, 120), 2, 1)])
+                                          ((128, 128, 128), 3, 3), ((68, 136, 120), 2, 1),
+                                          ((104, 118, 190), 3, 2)])
 def test_fused_smoother_is_bitwise_split(pf, res, pre, post, precision):
     """The fused full-sweep smoother (LDS tile, halo recomputation, ping-pong
     buffers) gives exactly the bits of two in-place half-sweeps, including
     partial tiles and odd sweep counts; 68 x 136 x 120 takes the large-level
-    kernel (from 2^20 cells) with partial tiles along i and k."""
+    kernel (from 2^20 cells) with partial tiles along i and k, and
+    104 x 118 x 190 (585 tiles) with partial tiles along j and k."""
     say(f"fused smoother {res} pre={pre} post={post} {precision}")
     rng = np.random.default_rng(11)
     bx, by, bz = random_beta(res, rng, 1000.0)
