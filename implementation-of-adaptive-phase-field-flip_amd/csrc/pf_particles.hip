@@ -275,6 +275,7 @@ struct P2GArgs {
 // sums do not depend on the order of the adds, so P2G is bit-reproducible.
 struct FixScale {
     float sm, su, sw;        // 2^E for mass, momentum, weight
+    float ru;                // su / sm
     double im, iu, iw;       // 2^-E
 };
 
@@ -291,25 +292,31 @@ __host__ __device__ inline int fix_exponent(double per_item, double n_items) {
 
 __device__ inline FixScale fix_scale(const float* bounds, const int* cmax) {
     const double nsup = 27.0 * (double)max(*cmax, 1);
-    const int em = fix_exponent((double)bounds[1], nsup);
-    const int eu = fix_exponent((double)bounds[2], nsup);
+    int em = fix_exponent((double)bounds[1], nsup);
+    int eu = fix_exponent((double)bounds[2], nsup);
+    // keep su / sm a normal float (a lower exponent only costs precision)
+    if (eu > em + 126) eu = em + 126;
+    if (em > eu + 126) em = eu + 126;
     const int ew = fix_exponent(1.0, nsup);
     FixScale f;
     f.sm = ldexpf(1.0f, em);
     f.su = ldexpf(1.0f, eu);
     f.sw = ldexpf(1.0f, ew);
+    f.ru = ldexpf(1.0f, eu - em);
     f.im = ldexp(1.0, -em);
     f.iu = ldexp(1.0, -eu);
     f.iw = ldexp(1.0, -ew);
     return f;
 }
 
-// round(q * s) as a two's-complement 64-bit integer, |q * s| < 2^51
-// (1.5 * 2^52 aligns the binary point of a double with its last mantissa bit)
-__device__ inline unsigned long long to_fix(float q, float s) {
-    const double d = (double)(q * s) + 6755399441055744.0;
+// round(q) as a two's-complement 64-bit integer, |q| < 2^51 (1.5 * 2^52
+// aligns the binary point of a double with its last mantissa bit)
+__device__ inline unsigned long long fix_round(float q) {
+    const double d = (double)q + 6755399441055744.0;
     return (unsigned long long)(__double_as_longlong(d) - 0x4338000000000000LL);
 }
+// round(q * s)
+__device__ inline unsigned long long to_fix(float q, float s) { return fix_round(q * s); }
 
 constexpr int E = T + 2;  // tangential / cell extent of a tile region (1-cell halo)
 constexpr int F = T + 1;  // normal extent of a face region
@@ -357,9 +364,13 @@ __device__ inline AxisCand axis_cand(float x, float dx, int origin) {
 // Accumulate one particle into the 2x2x2 candidates of one staggered grid.
 // q*[2] are per-axis squared distances; base is the LDS offset of candidate
 // (0,0,0); SX, SY are the LDS strides of the first / second index.
+// ms = m * sm and us = u * su / sm fold the power-of-two fixed-point scales
+// into the particle: scaling by 2^E commutes with rounding, so w * ms is
+// round(w m) 2^Em and (w * ms) * us is round(round(w m) u) 2^Eu, the bits
+// of scaling each rounded product, with two multiplies fewer.
 template <int SX, int SY, bool MOM>
 __device__ inline void splat8(const float (&qx)[2], const float (&qy)[2], const float (&qz)[2],
-                              int base, float m, float u, float inv_r2, const FixScale& fs,
+                              int base, float ms, float us, float inv_r2,
                               unsigned long long* sm, unsigned long long* su) {
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -371,9 +382,9 @@ __device__ inline void splat8(const float (&qx)[2], const float (&qy)[2], const 
                 const float w = cube_w(pxy + qz[c], inv_r2);
                 if (w > 0.0f) {
                     const int addr = base + a * SX + b * SY + c;
-                    const float wm = w * m;
-                    atomicAdd(&sm[addr], to_fix(wm, fs.sm));
-                    if (MOM) atomicAdd(&su[addr], to_fix(wm * u, fs.su));
+                    const float wm = w * ms;
+                    atomicAdd(&sm[addr], fix_round(wm));
+                    if (MOM) atomicAdd(&su[addr], fix_round(wm * us));
                 }
             }
         }
@@ -403,14 +414,15 @@ __device__ inline void splat_particle(const PData& d, float dx, float inv_r2, in
     const float nx_[2] = {cx.n0, cx.n1}, tx_[2] = {cx.t0, cx.t1};
     const float ny_[2] = {cy.n0, cy.n1}, ty_[2] = {cy.t0, cy.t1};
     const float nz_[2] = {cz.n0, cz.n1}, tz_[2] = {cz.t0, cz.t1};
+    const float ms = d.m * fs.sm;
     // x-faces: region [F][E][E], origin (X0, Y0-1, Z0-1)
-    splat8<E * E, E, true>(nx_, ty_, tz_, (cx.ln * E + cy.lt) * E + cz.lt, d.m, d.vx, inv_r2, fs,
+    splat8<E * E, E, true>(nx_, ty_, tz_, (cx.ln * E + cy.lt) * E + cz.lt, ms, d.vx * fs.ru, inv_r2,
                            s + O_XM, s + O_XU);
     // y-faces: region [E][F][E], origin (X0-1, Y0, Z0-1)
-    splat8<F * E, E, true>(tx_, ny_, tz_, (cx.lt * F + cy.ln) * E + cz.lt, d.m, d.vy, inv_r2, fs,
+    splat8<F * E, E, true>(tx_, ny_, tz_, (cx.lt * F + cy.ln) * E + cz.lt, ms, d.vy * fs.ru, inv_r2,
                            s + O_YM, s + O_YU);
     // z-faces: region [E][E][F], origin (X0-1, Y0-1, Z0)
-    splat8<E * F, F, true>(tx_, ty_, nz_, (cx.lt * E + cy.lt) * F + cz.ln, d.m, d.vz, inv_r2, fs,
+    splat8<E * F, F, true>(tx_, ty_, nz_, (cx.lt * E + cy.lt) * F + cz.ln, ms, d.vz * fs.ru, inv_r2,
                            s + O_ZM, s + O_ZU);
     // cells: region [E][E][E], origin (X0-1, Y0-1, Z0-1)
     if (WSUM) {
@@ -424,19 +436,25 @@ __device__ inline void splat_particle(const PData& d, float dx, float inv_r2, in
                     const float w = cube_w(pxy + tz_[k], inv_r2);
                     if (w > 0.0f) {
                         const int addr = ((cx.lt + i) * E + (cy.lt + j)) * E + cz.lt + k;
-                        atomicAdd(&s[O_CM + addr], to_fix(w * d.m, fs.sm));
+                        atomicAdd(&s[O_CM + addr], fix_round(w * ms));
                         atomicAdd(&s[O_CW + addr], to_fix(w, fs.sw));
                     }
                 }
             }
     } else {
-        splat8<E * E, E, false>(tx_, ty_, tz_, (cx.lt * E + cy.lt) * E + cz.lt, d.m, 0.f, inv_r2, fs,
+        splat8<E * E, E, false>(tx_, ty_, tz_, (cx.lt * E + cy.lt) * E + cz.lt, ms, 0.f, inv_r2,
                                 s + O_CM, nullptr);
     }
 }
 
+// Two cells of the tile per lane.  The LDS region (51-59 KB) allows three
+// workgroups per CU; one lane per cell (512 lanes, 6 waves per SIMD instead
+// of 3) measured 18 % slower: more waves of one tile region put more
+// same-address LDS atomics in flight at once.
+constexpr int kP2GThreads = 256;
+
 template <bool WSUM>
-__global__ __launch_bounds__(256) void k_p2g(P2GArgs a) {
+__global__ __launch_bounds__(kP2GThreads) void k_p2g(P2GArgs a) {
     constexpr int R = WSUM ? REG_W : REG;
     __shared__ unsigned long long s[R];
 
@@ -1331,9 +1349,9 @@ PF_API int pf_p2g(const float* px, const float* pv, const float* pmass, const ui
     k_max_count<<<(unsigned)(nbk < 2048 ? nbk : 2048), 256, 0, s>>>(cell_start, nkeys, cmax);
     PF_CHECK_LAUNCH();
     if (cell_wsum)
-        k_p2g<true><<<(unsigned)nt, 256, 0, s>>>(a);
+        k_p2g<true><<<(unsigned)nt, kP2GThreads, 0, s>>>(a);
     else
-        k_p2g<false><<<(unsigned)nt, 256, 0, s>>>(a);
+        k_p2g<false><<<(unsigned)nt, kP2GThreads, 0, s>>>(a);
     PF_CHECK_LAUNCH();
     GatherArgs ga;
     ga.g = a.g;
