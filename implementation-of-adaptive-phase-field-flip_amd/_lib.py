@@ -41,6 +41,7 @@ class PhaseArgs(ctypes.Structure):
 
 
 MG_UNFUSED_SMOOTHER = 1
+MG_SEPARATE_DOTS = 2
 
 
 class MGLevel(ctypes.Structure):

@@ -97,6 +97,17 @@ struct UpdateFold {
     int* host_flag;
 };
 
+// The flexible CG's inner products after the preconditioner (k_cg_beta's
+// work), folded into the V-cycle's last level-0 sweep, which has the final
+// z and the residual r (its right-hand side) of its tile in registers: per
+// workgroup the FP64 partials of sum z, sum r, sum Ap, <r, z>, <Ap, z> of
+// its tile's cells, with z as stored (times zscale for 16-bit iterates).
+struct DotFold {
+    const float* Ap;
+    const float* zscale;
+    CGState* st;
+};
+
 static unsigned red_blocks(int64_t n) {
     const int64_t nb = cdiv(n, (int64_t)RT * 4);
     return (unsigned)(nb < 1 ? 1 : (nb > MAXB ? MAXB : nb));
@@ -367,14 +378,19 @@ struct RbTile {
 // the coarse iterate pc (poisson.py:178-182), formed in LDS while staging
 // (rounded to ST as the separate prolongation stores it): the first
 // post-sweep of a level without a separate prolongation pass.
-template <typename ST, bool ZINIT, int TI, int TJ, int TK, int NTH = 256, bool PROL = false, typename CT = float>
+// DOTS: the CG's inner products of the sweep's output (DotFold), with Ap
+// staged into LDS alongside the iterate and the sums formed after the
+// stores, when the face registers are free.
+template <typename ST, bool ZINIT, int TI, int TJ, int TK, int NTH = 256, bool PROL = false, typename CT = float,
+          bool DOTS = false>
 __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restrict__ bx,
                                                    const ST* __restrict__ by, const ST* __restrict__ bz,
                                                    const ST* __restrict__ pin, ST* __restrict__ pout,
                                                    const float* __restrict__ b, int c1, float omega,
                                                    const int* __restrict__ done,
                                                    const float* __restrict__ bscale,
-                                                   const CT* __restrict__ pcoarse, Dims dc, UpdateFold uf) {
+                                                   const CT* __restrict__ pcoarse, Dims dc, UpdateFold uf,
+                                                   DotFold df) {
     using T = RbTile<TI, TJ, TK, NTH>;
     __shared__ __attribute__((aligned(16))) float P[T::NP];
     __shared__ float s_fm[NTH / 64];
@@ -471,6 +487,21 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
             f[q][9] = to_f32(bz[z + 1]);
             f[q][10] = to_f32(bz[z + 2]);
             ld_pair(b, cell_index(d, i, j, k), bb[q][0], bb[q][1]);
+        }
+    }
+    // (1c) DOTS: Ap of the tile into LDS, cell (il, jl, kl) at its tile-linear
+    // index (every lane loads: coordinates clamped into the grid)
+    constexpr int TC = TI * TJ * TK;
+    __shared__ float A[DOTS ? TC : 1];
+    if constexpr (DOTS) {
+        static_assert(TC % T::NT == 0, "whole passes");
+        for (int e0 = 0; e0 < TC; e0 += T::NT) {
+            const int e = e0 + tid;
+            const int kl = e % TK, jl = (e / TK) % TJ, il = e / (TK * TJ);
+            const int gi = min(i0 + il, d.nx - 1), gj = min(j0 + jl, d.ny - 1), gk = min(k0 + kl, d.nz - 1);
+            __builtin_amdgcn_global_load_lds((const void*)(df.Ap + cell_index(d, gi, gj, gk)),
+                                             (__attribute__((address_space(3))) void*)(A + e0 + (tid & ~63)), 4,
+                                             0, 0);
         }
     }
 
@@ -646,6 +677,39 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
         const float other = P[li[q] + (1 - h)];
         if (!skip)
             st_pair(pout, cell_index(d, i0 + il, j0 + jl, k0 + 2 * kp), h ? other : v, h ? v : other);
+        // DOTS: the cell's value as stored (no other thread reads a
+        // second-colour cell in this half-sweep)
+        if (DOTS) P[c] = round_st<ST>(v);
+    }
+    if constexpr (DOTS) {
+        // k_cg_beta's sums over the tile: z as stored times zscale (its
+        // zval), r = b unscaled (b * bs * zscale, powers of two), Ap
+        const float zsc = df.zscale ? *df.zscale : 1.0f;
+        // (A landed before the first barrier; a pair's two cells were
+        // written by this thread)
+        double dsum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < T::PPT; ++q) {
+            if (!ok[q]) continue;
+            const int pq = tid + q * T::NT;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const double zz = (double)(P[li[q] + e] * zsc), rr = (double)(bb[q][e] * zsc);
+                const double ap = (double)A[2 * pq + e];
+                dsum[0] += zz;
+                dsum[1] += rr;
+                dsum[2] += ap;
+                dsum[3] += rr * zz;
+                dsum[4] += ap * zz;
+            }
+        }
+        __shared__ double s_dot[5 * 16];
+        block_sum<5>(dsum, s_dot);
+        if (tid == 0 && !skip) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) df.st->part[q * MAXB + blockIdx.x] = dsum[q];
+            if (blockIdx.x == 0) df.st->beta_parts = 1;  // read by the next k_cg_apply
+        }
     }
 }
 
@@ -1418,22 +1482,47 @@ template <typename ST, bool ZINIT, bool PROL = false, typename CT = float>
 static int launch_full_sweep(const Dims& d, const ST* fx, const ST* fy, const ST* fz, const ST* pin,
                              ST* pout, const float* b, int c1, float omega, const int* done,
                              const float* bscale, hipStream_t s, const CT* pcoarse = nullptr,
-                             Dims dc = Dims{1, 1, 1}, UpdateFold uf = UpdateFold{nullptr, 0, nullptr}) {
+                             Dims dc = Dims{1, 1, 1}, UpdateFold uf = UpdateFold{nullptr, 0, nullptr},
+                             DotFold df = DotFold{nullptr, nullptr, nullptr}) {
     const int64_t n = (int64_t)d.nx * d.ny * d.nz;
     auto tiles = [&](int ti, int tj, int tk) {
         return (unsigned)(cdiv(d.nx, ti) * cdiv(d.ny, tj) * cdiv(d.nz, tk));
     };
+    if constexpr (!ZINIT && !PROL) {
+        if (df.Ap) {  // the plain sweep with the CG's inner products
+            if (n >= (int64_t)1 << 20)
+                k_rbgs_full<ST, false, 8, 8, 64, 512, false, float, true><<<tiles(8, 8, 64), 512, 0, s>>>(
+                    d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, nullptr, dc, uf, df);
+            else if (n >= (int64_t)1 << 17)
+                k_rbgs_full<ST, false, 8, 8, 16, 256, false, float, true><<<tiles(8, 8, 16), 256, 0, s>>>(
+                    d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, nullptr, dc, uf, df);
+            else
+                k_rbgs_full<ST, false, 4, 8, 16, 256, false, float, true><<<tiles(4, 8, 16), 256, 0, s>>>(
+                    d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, nullptr, dc, uf, df);
+            PF_CHECK_LAUNCH();
+            return PF_OK;
+        }
+    }
+    PF_REQUIRE(!df.Ap, "launch_full_sweep: inner products only on a plain sweep");
     if (n >= (int64_t)1 << 20)
         k_rbgs_full<ST, ZINIT, 8, 8, 64, 512, PROL, CT><<<tiles(8, 8, 64), 512, 0, s>>>(
-            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc, uf);
+            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc, uf, df);
     else if (n >= (int64_t)1 << 17)
         k_rbgs_full<ST, ZINIT, 8, 8, 16, 256, PROL, CT><<<tiles(8, 8, 16), 256, 0, s>>>(
-            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc, uf);
+            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc, uf, df);
     else
         k_rbgs_full<ST, ZINIT, 4, 8, 16, 256, PROL, CT><<<tiles(4, 8, 16), 256, 0, s>>>(
-            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc, uf);
+            d, fx, fy, fz, pin, pout, b, c1, omega, done, bscale, pcoarse, dc, uf, df);
     PF_CHECK_LAUNCH();
     return PF_OK;
+}
+
+// workgroups of launch_full_sweep on a level of dims d (one per tile)
+static int64_t full_sweep_blocks(const Dims& d) {
+    const int64_t n = (int64_t)d.nx * d.ny * d.nz;
+    const int ti = n >= (int64_t)1 << 20 ? 8 : (n >= (int64_t)1 << 17 ? 8 : 4);
+    const int tk = n >= (int64_t)1 << 20 ? 64 : 16;
+    return cdiv(d.nx, ti) * cdiv(d.ny, 8) * cdiv(d.nz, tk);
 }
 
 // The two iterate buffers of an HBM level: the level's own (level 0: the
@@ -1450,8 +1539,11 @@ static bool fused_smoother(const pf_mg& mg, int l) {
 
 // One V-cycle: z0 = M^{-1} r0 (no mean removal at level 0).
 template <typename ST>
+// dfold: the CG's inner products ride on the last level-0 post-sweep
+// (*dot_done tells whether one took them).
 static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, const float* bscale,
-                  hipStream_t s, const UpdateFold* uf = nullptr) {
+                  hipStream_t s, const UpdateFold* uf = nullptr, const DotFold* dfold = nullptr,
+                  bool* dot_done = nullptr) {
     const int L = mg.nlev;
     const int sm = mg.small_from;             // first LDS level (L if none)
     const int G = sm < L ? sm : L - 1;        // levels 0..G-1 run out of HBM with restriction
@@ -1479,12 +1571,16 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
                 const int c1 = reverse ? 1 : 0;
                 // the V-cycle's first kernel takes the residual test with it
                 const UpdateFold f = (uf && l == 0 && sw == 0) ? *uf : UpdateFold{nullptr, 0, nullptr};
+                const bool dots = dfold && l == 0 && !from_zero && sw == sweeps - 1;
                 const int rc = (from_zero && sw == 0)
                                    ? launch_full_sweep<ST, true>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done,
                                                                  bsc, s, static_cast<const float*>(nullptr),
                                                                  Dims{1, 1, 1}, f)
                                    : launch_full_sweep<ST, false>(d, fx, fy, fz, cur[l], out, b, c1, mg.omega, done,
-                                                                  bsc, s);
+                                                                  bsc, s, static_cast<const float*>(nullptr),
+                                                                  Dims{1, 1, 1}, UpdateFold{nullptr, 0, nullptr},
+                                                                  dots ? *dfold : DotFold{nullptr, nullptr, nullptr});
+                if (dots && dot_done) *dot_done = true;
                 if (rc) return rc;
                 cur[l] = out;
             }
@@ -1661,6 +1757,14 @@ static bool fold_residual_test(const pf_mg& mg) {
     return mg.small_from > 0 && mg.pre > 0 && fused_smoother(mg, 0);
 }
 
+// The inner products after the V-cycle ride on its last level-0 sweep when
+// that is a plain fused sweep (at least two post-sweeps on an HBM level 0)
+// whose workgroups fit the partials array.
+static bool fold_dots(const pf_mg& mg) {
+    return !(mg.options & PF_MG_SEPARATE_DOTS) && mg.small_from > 0 && mg.post >= 2 && fused_smoother(mg, 0) &&
+           full_sweep_blocks(ldims(mg.lev[0])) <= MAXB;
+}
+
 template <typename ST>
 static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int* done, int* flag,
                              unsigned nb, hipStream_t s) {
@@ -1675,10 +1779,17 @@ static int enqueue_iteration(const pf_mg& mg, const CGK<ST>& a, ST* z, const int
         k_comb<<<1, RT, 0, s>>>(st, C_UPDATE, (int)nb, a.n, nullptr, 0.0, flag);
         PF_CHECK_LAUNCH();
     }
-    int rc = vcycle<ST>(mg, a.r, z, done, vcycle_scale(a), s, fold ? &uf : nullptr);
+    const DotFold df{a.Ap, a.zscale, st};
+    const bool dots = fold_dots(mg);
+    bool dot_done = false;
+    int rc = vcycle<ST>(mg, a.r, z, done, vcycle_scale(a), s, fold ? &uf : nullptr, dots ? &df : nullptr,
+                        &dot_done);
     if (rc) return rc;
-    k_cg_beta<ST><<<a.np_beta, WT, 0, s>>>(a);
-    PF_CHECK_LAUNCH();
+    PF_REQUIRE(dot_done == dots, "pf_mgpcg_solve: the V-cycle's last level-0 kernel is not a plain fused sweep");
+    if (!dots) {
+        k_cg_beta<ST><<<a.np_beta, WT, 0, s>>>(a);
+        PF_CHECK_LAUNCH();
+    }
     return PF_OK;
 }
 
@@ -1880,6 +1991,7 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
     const unsigned nb = red_blocks(n);
     a.slot = 1;
     a.np_apply = a.np_beta = (int)std::min<int64_t>(WB, (n + WT - 1) / WT);
+    if (fold_dots(mg)) a.np_beta = (int)full_sweep_blocks(d);  // one partial per level-0 tile
     const int* done = &st->done;
 
     k_part_sum<float><<<nb, RT, 0, s>>>(b, n, st->part);

@@ -20,7 +20,11 @@ Differences from the reference that do not change the algorithm:
 * a red-black sweep is one fused kernel that reads the iterate from one
   buffer and writes it to a second one (``levels[l].r``, and ``levels[0].p``
   in FP32); ``fused_smoother=False`` runs the two half-sweeps in place, with
-  the same result bit for bit.
+  the same result bit for bit;
+* the CG's inner products after the preconditioner (sum z, sum r, sum Ap,
+  <r, z>, <Ap, z>) are formed by the V-cycle's last level-0 sweep from the
+  values it holds, one FP64 partial per tile; ``fold_dots=False`` forms
+  them in a kernel of their own (the same sums in another order).
 """
 
 from __future__ import annotations
@@ -75,7 +79,7 @@ class MGPCG:
     """Multigrid-preconditioned flexible CG (pfflip/poisson.py:240-409)."""
 
     def __init__(self, res, device, *, omega=6.0 / 7.0, pre=2, post=2, coarsest=60, min_dim=4,
-                 precision="fp32", fused_smoother=True):
+                 precision="fp32", fused_smoother=True, fold_dots=True):
         self.device = _require_cuda(device)
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
@@ -99,7 +103,8 @@ class MGPCG:
         mg.precision = PRECISIONS[precision]
         # one fused kernel per red-black sweep (default) or one per half-sweep;
         # both give the same bits
-        mg.options = 0 if fused_smoother else _lib.MG_UNFUSED_SMOOTHER
+        mg.options = (0 if fused_smoother else _lib.MG_UNFUSED_SMOOTHER) | \
+            (0 if fold_dots else _lib.MG_SEPARATE_DOTS)
         for i, L in enumerate(self.levels):
             mg.lev[i].nx, mg.lev[i].ny, mg.lev[i].nz = L.res
         _lib.check(lib.pf_mg_setup(ctypes.byref(mg)), "pf_mg_setup")

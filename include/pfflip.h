@@ -223,6 +223,11 @@ typedef struct pf_mg {
 /* Smooth with two half-sweep launches per sweep instead of the fused
  * full-sweep kernel (same results bit for bit; for testing and timing). */
 #define PF_MG_UNFUSED_SMOOTHER 1
+/* Form the flexible CG's inner products after the preconditioner in a
+ * kernel of their own; by default the V-cycle's last level-0 sweep forms
+ * them from the values it holds (the same sums in another order: results
+ * agree to rounding; for testing and timing). */
+#define PF_MG_SEPARATE_DOTS 2
 
 size_t pf_mg_scalar_bytes(void);
 /* Validate the level table and choose small_from. */

@@ -533,7 +533,9 @@ def test_fused_smoother_solve_is_bitwise_split(pf):
     g = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
     out = []
     for fused in (True, False):
-        solver = pf.MGPCG(res, "cuda:0", pre=3, post=3, fused_smoother=fused)
+        # the inner products in their own kernel in both runs (the fused run
+        # would otherwise sum them per tile, in another order)
+        solver = pf.MGPCG(res, "cuda:0", pre=3, post=3, fused_smoother=fused, fold_dots=False)
         solver.build_hierarchy(g(bx), g(by), g(bz))
         p = torch.zeros(res, device="cuda")
         it, rel = solver.solve(g(b), p, tol=1e-6, max_iter=200)
@@ -541,6 +543,37 @@ def test_fused_smoother_solve_is_bitwise_split(pf):
         del solver
     assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
     assert np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("res", [(64, 64, 64), (128, 128, 128), (40, 48, 56)])
+def test_inner_products_on_the_last_sweep_match_their_own_kernel(pf, res, precision):
+    """The CG's inner products formed by the V-cycle's last level-0 sweep
+    (one FP64 partial per tile) against the same sums in a kernel of their
+    own: the two FPCG runs differ only by the order of FP64 additions (which
+    can also move a 16-bit V-cycle's power-of-two input scale), so they take
+    the same iterations to within two and reach pressures equal to 1e-5 of
+    their size, the bound the solver tests hold every precision to.  128^3 takes the 8x8x64 tiles, 64^3 the 8x8x16 ones, 40 x
+    48 x 56 the small ones with partial tiles along i and k."""
+    say(f"inner products on the last sweep {res} {precision}")
+    rng = np.random.default_rng(14)
+    bx, by, bz = random_beta(res, rng, 1000.0)
+    b = rng.standard_normal(res).astype(np.float32)
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
+    out = []
+    for fold in (True, False):
+        solver = pf.MGPCG(res, "cuda:0", pre=3, post=3, precision=precision, fold_dots=fold)
+        assert solver.small_from >= 1
+        solver.build_hierarchy(g(bx), g(by), g(bz))
+        p = torch.zeros(res, device="cuda")
+        it, rel = solver.solve(g(b), p, tol=1e-6, max_iter=300)
+        out.append((it, rel, np_(p).astype(np.float64)))
+        del solver
+    (it0, rel0, p0), (it1, rel1, p1) = out
+    say(f"  folded {it0} it rel {rel0:.3e}, separate {it1} it rel {rel1:.3e}")
+    assert rel0 < 1e-6 and rel1 < 1e-6
+    assert abs(it0 - it1) <= 2
+    assert np.abs(p0 - p1).max() < 1e-5 * np.abs(p1).max()
 
 
 def test_mgpcg_zero_rhs_and_max_iter(pf):
