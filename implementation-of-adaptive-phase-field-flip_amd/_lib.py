@@ -42,6 +42,7 @@ class PhaseArgs(ctypes.Structure):
 
 MG_UNFUSED_SMOOTHER = 1
 MG_SEPARATE_DOTS = 2
+MG_SPLIT_LEVELS = 4
 
 
 class MGLevel(ctypes.Structure):

@@ -3,9 +3,11 @@
 //
 // Hierarchy: unsmoothed-aggregation Galerkin (coarse face = sum of 4 fine
 // faces), red-black weighted Gauss-Seidel (omega = 6/7), V-cycle with
-// pre (0,1) / post (1,0) colour orders.  Fine levels run one kernel per
-// half-sweep out of HBM; every level from `small_from` down is solved inside
-// one workgroup with all its data in LDS.  The FPCG scalars live on the
+// pre (0,1) / post (1,0) colour orders.  Large levels run one kernel per
+// red-black sweep out of HBM (one per half-sweep on request), levels of up
+// to 32^3 cells below level 0 one kernel per smoothing phase; every level
+// from `small_from` down is solved inside one workgroup with all its data
+// in LDS.  The FPCG scalars live on the
 // device; dot products are deterministic FP64 block reductions, and the
 // host only polls a "done" flag one iteration behind the GPU.
 #include <algorithm>
@@ -162,16 +164,19 @@ __device__ inline void stencil_s(const Dims& d, const FT* __restrict__ bx, const
 // 1000:1 coefficient jump, this form to ~1e-9 / ~5e-8.
 // f = (x-low, x-high, y-low, y-high, z-low, z-high) faces, pn the matching
 // neighbour values.  Every smoother and residual kernel goes through these
-// helpers, so all of them round identically.
+// helpers, so all of them round identically: the fused multiply-adds are
+// written out and contraction is off, since the compiler could otherwise
+// fuse either product of the first sum and choose differently per kernel.
 __device__ inline void stencil_vals(float pc, const float (&f)[6], const float (&pn)[6], float& diag,
                                     float& Ap) {
+#pragma clang fp contract(off)
     diag = f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
     float s = f[0] * (pc - pn[0]);
-    s += f[1] * (pc - pn[1]);
-    s += f[2] * (pc - pn[2]);
-    s += f[3] * (pc - pn[3]);
-    s += f[4] * (pc - pn[4]);
-    s += f[5] * (pc - pn[5]);
+    s = fmaf(f[1], pc - pn[1], s);
+    s = fmaf(f[2], pc - pn[2], s);
+    s = fmaf(f[3], pc - pn[3], s);
+    s = fmaf(f[4], pc - pn[4], s);
+    s = fmaf(f[5], pc - pn[5], s);
     Ap = s;
 }
 
@@ -756,6 +761,209 @@ __global__ __launch_bounds__(RT) void k_prolong(Dims d, Dims dc, const ST* pin, 
     if (!skip) pout[t] = from_f32<ST>(v);
 }
 
+// ================================================================ fused small levels
+// A small HBM level's whole pre-smoothing phase (sweeps from zero, colour
+// order (0,1), then the residual restricted to the coarse level) or whole
+// post-smoothing phase (input pin plus the prolongation of the coarse
+// iterate, colour order (1,0)) in one launch: on a level of a few ten
+// thousand cells each per-sweep launch is bound by launch latency, not by
+// its bytes.  A workgroup owns a TI x TJ x TK tile and holds the tile plus
+// an H-cell halo (two cells per sweep) of faces, right-hand side and
+// iterate in LDS.  Each half-sweep runs on the region the tile's result
+// still depends on (margin m of the box; it shrinks by one cell per
+// half-sweep), so the halo updates duplicate the neighbouring tiles' work
+// and no workgroup waits for another.  Every update goes through the
+// helpers of the per-sweep kernels in their operation order, so the
+// results are bitwise those of the per-sweep path.  A neighbour outside the
+// grid is the cell itself (the reference's clamped indices).
+template <int TI, int TJ, int TK, int NSW>
+struct LvBox {
+    static constexpr int NT = 1024;
+    static constexpr int H = 2 * NSW;
+    static constexpr int BI = TI + 2 * H, BJ = TJ + 2 * H, BK = TK + 2 * H;
+    // one extra layer per axis holds the high faces of the outermost cells
+    static constexpr int SJ = BK + 1, SI = (BJ + 1) * (BK + 1), N = (BI + 1) * SI;
+    static constexpr int NPT = (N + NT - 1) / NT;
+    static_assert(TI % 2 == 0 && TJ % 2 == 0 && TK % 2 == 0, "tiles restrict to whole coarse cells");
+};
+
+// One half-sweep of colour col on the cells at margin >= M of the box
+// (origin o, which is even on every axis).  ZI: the iterate is zero.
+// A neighbour outside the grid is read from its ghost entry, the box entry
+// beyond the wall, which every update of the cell sets to the cell's value
+// (the reference's clamped index is the cell itself), so the six
+// neighbour reads take no branches.
+template <typename ST, typename BX, int M, bool ZI>
+__device__ inline void lv_half_sweep(float* P, const float* B, const float* FX, const float* FY,
+                                     const float* FZ, const Dims& d, int ox, int oy, int oz, int col,
+                                     float omega) {
+    constexpr int RJ = BX::BJ - 2 * M, RK2 = (BX::BK - 2 * M) / 2;
+    constexpr int NPAIR = (BX::BI - 2 * M) * RJ * RK2;
+    constexpr int SI = BX::SI, SJ = BX::SJ;
+    for (int e = threadIdx.x; e < NPAIR; e += BX::NT) {
+        const int kp = e % RK2, y = M + (e / RK2) % RJ, x = M + e / (RK2 * RJ);
+        const int z = M + 2 * kp + ((col - (x + y + M)) & 1);
+        const int gi = ox + x, gj = oy + y, gk = oz + z;
+        if (gi < 0 || gj < 0 || gk < 0 || gi >= d.nx || gj >= d.ny || gk >= d.nz) continue;
+        const int c = x * SI + y * SJ + z;
+        const float fc[6] = {FX[c], FX[c + SI], FY[c], FY[c + SJ], FZ[c], FZ[c + 1]};
+        float v;
+        if (ZI) {
+            const float dg = fc[0] + fc[1] + fc[2] + fc[3] + fc[4] + fc[5];
+            v = dg > 0.0f ? rb_first(B[c], dg, omega) : 0.0f;
+        } else {
+            const float pc = P[c], bc = B[c];
+            const float pn[6] = {P[c - SI], P[c + SI], P[c - SJ], P[c + SJ], P[c - 1], P[c + 1]};
+            float dg, Ap;
+            stencil_vals(pc, fc, pn, dg, Ap);
+            // (formed unconditionally: every read is issued at once)
+            const float vr = rb_relax(pc, bc, Ap, dg, omega);
+            v = dg > 0.0f ? vr : pc;
+        }
+        v = round_st<ST>(v);
+        P[c] = v;
+        // the ghosts (only the cell itself reads them; a low ghost of a cell
+        // on the box's first layer lies outside the box, and that cell is
+        // not updated again)
+        if (gi == 0 && (M > 0 || x > 0)) P[c - SI] = v;
+        if (gi == d.nx - 1) P[c + SI] = v;
+        if (gj == 0 && (M > 0 || y > 0)) P[c - SJ] = v;
+        if (gj == d.ny - 1) P[c + SJ] = v;
+        if (gk == 0 && (M > 0 || z > 0)) P[c - 1] = v;
+        if (gk == d.nz - 1) P[c + 1] = v;
+    }
+}
+
+// half-sweeps h .. 2*NSW-1 at margins M0 + h, colours c1, 1 - c1, ...
+template <typename ST, typename BX, int M0, int h, bool ZFIRST>
+__device__ inline void lv_sweeps(float* P, const float* B, const float* FX, const float* FY, const float* FZ,
+                                 const Dims& d, int ox, int oy, int oz, int c1, float omega) {
+    if constexpr (h < 2 * (BX::H / 2)) {
+        lv_half_sweep<ST, BX, M0 + h, ZFIRST && h == 0>(P, B, FX, FY, FZ, d, ox, oy, oz, (c1 + h) & 1, omega);
+        __syncthreads();
+        lv_sweeps<ST, BX, M0, h + 1, ZFIRST>(P, B, FX, FY, FZ, d, ox, oy, oz, c1, omega);
+    }
+}
+
+// PRE: pin is unused (the level starts from zero), pout gets the smoothed
+// iterate and bc the restricted residual (k_resid_restrict's order).
+// !PRE: pout gets the post-smoothed iterate of pin + P pcoarse.
+template <typename ST, bool PRE, int TI, int TJ, int TK, int NSW, typename CT = float>
+__global__ __launch_bounds__(1024) void k_level(Dims d, Dims dc, const ST* __restrict__ bx,
+                                                const ST* __restrict__ by, const ST* __restrict__ bz,
+                                                const float* __restrict__ b, const ST* __restrict__ pin,
+                                                const CT* __restrict__ pcoarse, ST* __restrict__ pout,
+                                                float* __restrict__ bc, float omega,
+                                                const int* __restrict__ done) {
+    using BX = LvBox<TI, TJ, TK, NSW>;
+    __shared__ float P[BX::N], B[BX::N], FX[BX::N], FY[BX::N], FZ[BX::N];
+    const int tid = threadIdx.x;
+    const int ntk = (d.nz + TK - 1) / TK, ntj = (d.ny + TJ - 1) / TJ;
+    const int tile = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = (tile / (ntk * ntj)) * TI, j0 = ((tile / ntk) % ntj) * TJ, k0 = (tile % ntk) * TK;
+    const int ox = i0 - BX::H, oy = j0 - BX::H, oz = k0 - BX::H;
+
+    // staging: every load of the box issued before the first wait, all from
+    // coordinates clamped into their array (no branches), the entries
+    // outside it zeroed afterwards
+    // (array indices by 32-bit 24x24-bit multiply-adds: the level's face
+    // arrays hold fewer than 2^24 entries.  Left to itself the compiler
+    // forms a*n + b as a 64-bit multiply-add whose unused upper addend half
+    // may be the register of a load in flight, and each address then waits
+    // for the previous load.)
+    auto mad24 = [](unsigned a, unsigned n, unsigned b) {
+        unsigned r;
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(n), "v"(b));
+        return r;
+    };
+    auto ix = [&](unsigned a, unsigned n1, unsigned b, unsigned n2, unsigned c) {
+        return mad24(mad24(a, n1, b), n2, c);
+    };
+    ST lf[BX::NPT][3], lp[BX::NPT];
+    CT lc[BX::NPT];
+    float lb[BX::NPT];
+    unsigned m[BX::NPT];
+#pragma unroll
+    for (int q = 0; q < BX::NPT; ++q) {
+        const int e = min(tid + q * BX::NT, BX::N - 1);
+        const int z = e % BX::SJ, y = (e / BX::SJ) % (BX::BJ + 1), x = e / BX::SI;
+        const int gi = ox + x, gj = oy + y, gk = oz + z;
+        const int ci = min(max(gi, 0), d.nx - 1), cj = min(max(gj, 0), d.ny - 1), ck = min(max(gk, 0), d.nz - 1);
+        const bool ii = gi == ci, jj = gj == cj, kk = gk == ck;
+        // bit 0-2: the face of axis 0-2 exists, bit 3: the cell is in the grid
+        m[q] = (tid + q * BX::NT < BX::N)
+                   ? ((gi >= 0 && gi <= d.nx && jj && kk) | (ii && gj >= 0 && gj <= d.ny && kk) << 1 |
+                      (ii && jj && gk >= 0 && gk <= d.nz) << 2 | (ii && jj && kk) << 3)
+                   : 0u;
+        const int fi = min(max(gi, 0), d.nx), fj = min(max(gj, 0), d.ny), fk = min(max(gk, 0), d.nz);
+        lf[q][0] = bx[ix(fi, d.ny, cj, d.nz, ck)];
+        lf[q][1] = by[ix(ci, d.ny + 1, fj, d.nz, ck)];
+        lf[q][2] = bz[ix(ci, d.ny, cj, d.nz + 1, fk)];
+        const unsigned cc = ix(ci, d.ny, cj, d.nz, ck);
+        lb[q] = b[cc];
+        if constexpr (!PRE) {
+            lp[q] = pin[cc];
+            lc[q] = pcoarse[ix(ci >> 1, dc.ny, cj >> 1, dc.nz, ck >> 1)];
+        }
+    }
+    const bool skip = done && *done;
+#pragma unroll
+    for (int q = 0; q < BX::NPT; ++q) {
+        const int e = tid + q * BX::NT;
+        if (e < BX::N) {
+            const unsigned mq = m[q];
+            FX[e] = (mq & 1u) ? to_f32(lf[q][0]) : 0.0f;
+            FY[e] = (mq & 2u) ? to_f32(lf[q][1]) : 0.0f;
+            FZ[e] = (mq & 4u) ? to_f32(lf[q][2]) : 0.0f;
+            B[e] = (mq & 8u) ? lb[q] : 0.0f;
+            // (every entry from clamped coordinates: the ghosts hold their
+            // cell's value)
+            P[e] = PRE ? 0.0f : round_st<ST>(to_f32(lp[q]) + to_f32(lc[q]));
+        }
+    }
+    __syncthreads();
+    // PRE: the first half-sweep from zero runs on the whole box (it reads no
+    // neighbour), the last at margin H - 1; !PRE: margins 1 .. H
+    lv_sweeps<ST, BX, PRE ? 0 : 1, 0, PRE>(P, B, FX, FY, FZ, d, ox, oy, oz, PRE ? 0 : 1, omega);
+
+    // the tile goes out; PRE: each cell's residual (k_resid_restrict's
+    // stencil with bscale 1) into LDS, then the sums of 2x2x2 children
+    constexpr int TC = TI * TJ * TK;
+    __shared__ float Rs[PRE ? TC : 1];
+    for (int e = tid; e < TC; e += BX::NT) {
+        const int kl = e % TK, jl = (e / TK) % TJ, il = e / (TK * TJ);
+        const int gi = i0 + il, gj = j0 + jl, gk = k0 + kl;
+        if (gi >= d.nx || gj >= d.ny || gk >= d.nz) continue;
+        const int c = (il + BX::H) * BX::SI + (jl + BX::H) * BX::SJ + (kl + BX::H);
+        const float pc = P[c];
+        if (!skip) pout[cell_index(d, gi, gj, gk)] = from_f32<ST>(pc);
+        if constexpr (PRE) {
+            const float fc[6] = {FX[c], FX[c + BX::SI], FY[c], FY[c + BX::SJ], FZ[c], FZ[c + 1]};
+            const float pn[6] = {P[c - BX::SI], P[c + BX::SI], P[c - BX::SJ], P[c + BX::SJ], P[c - 1], P[c + 1]};
+            float dg, Ap;
+            stencil_vals(pc, fc, pn, dg, Ap);
+            Rs[e] = B[c] - Ap;
+        }
+    }
+    if constexpr (PRE) {
+        __syncthreads();
+        constexpr int CJ = TJ / 2, CK = TK / 2;
+        for (int e = tid; e < TC / 8; e += BX::NT) {
+            const int K = e % CK, J = (e / CK) % CJ, I = e / (CK * CJ);
+            const int gI = i0 / 2 + I, gJ = j0 / 2 + J, gK = k0 / 2 + K;
+            if (gI >= dc.nx || gJ >= dc.ny || gK >= dc.nz) continue;
+            float acc = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) acc += Rs[((2 * I + a) * TJ + 2 * J + bb) * TK + 2 * K + c];
+            if (!skip) bc[cell_index(dc, gI, gJ, gK)] = acc;
+        }
+    }
+}
+
 // ================================================================ reductions
 // Reductions are fence-free: a kernel writes one partial per block, and a
 // one-block combine kernel (k_comb) forms the scalar.  The kernel boundary
@@ -883,14 +1091,15 @@ __device__ inline void make_slot(const LdsLevel& v, int colour, int t, float ome
     S.w = dg > 0.0f ? omega / dg : 0.0f;
 }
 
-// A p at a slot's cell in the difference form (same term order as stencil_vals)
+// A p at a slot's cell in the difference form (stencil_vals' operations)
 __device__ inline float slot_Ap(const LdsLevel& v, const Slot& S, float pc) {
+#pragma clang fp contract(off)
     float s = S.f[0] * (pc - v.p[S.n[0]]);
-    s += S.f[1] * (pc - v.p[S.n[1]]);
-    s += S.f[2] * (pc - v.p[S.n[2]]);
-    s += S.f[3] * (pc - v.p[S.n[3]]);
-    s += S.f[4] * (pc - v.p[S.n[4]]);
-    s += S.f[5] * (pc - v.p[S.n[5]]);
+    s = fmaf(S.f[1], pc - v.p[S.n[1]], s);
+    s = fmaf(S.f[2], pc - v.p[S.n[2]], s);
+    s = fmaf(S.f[3], pc - v.p[S.n[3]], s);
+    s = fmaf(S.f[4], pc - v.p[S.n[4]], s);
+    s = fmaf(S.f[5], pc - v.p[S.n[5]], s);
     return s;
 }
 
@@ -1517,6 +1726,42 @@ static int launch_full_sweep(const Dims& d, const ST* fx, const ST* fy, const ST
     return PF_OK;
 }
 
+// Largest level whose pre and post phases run as one k_level launch each
+// (32^3: 256 tiles of 4 x 4 x 8 cells, one workgroup per CU).
+constexpr int64_t kLevelFusedMaxCells = 32768;
+
+// One k_level launch: PRE (the pre-smoothing phase from zero and the
+// restricted residual into bc) or the post phase (pin plus the
+// prolongation of pcoarse, dims dc, into pout); nsw sweeps, 1..3.
+template <typename ST, bool PRE, typename CT = float>
+static int launch_level(const Dims& d, const Dims& dc, int nsw, const ST* fx, const ST* fy, const ST* fz,
+                        const float* b, const ST* pin, const CT* pcoarse, ST* pout, float* bc, float omega,
+                        const int* done, hipStream_t s) {
+    constexpr int TI = 4, TJ = 4, TK = 8;
+    PF_REQUIRE(d.nx == 2 * dc.nx && d.ny == 2 * dc.ny && d.nz == 2 * dc.nz, "launch_level: fine dims must be 2x coarse");
+    // the kernel forms indices with 24-bit multiplies
+    PF_REQUIRE((int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1) < ((int64_t)1 << 24), "launch_level: level too large");
+    const unsigned grid = cdiv(d.nx, TI) * cdiv(d.ny, TJ) * cdiv(d.nz, TK);
+    switch (nsw) {
+    case 1:
+        k_level<ST, PRE, TI, TJ, TK, 1, CT><<<grid, 1024, 0, s>>>(d, dc, fx, fy, fz, b, pin, pcoarse, pout, bc,
+                                                                  omega, done);
+        break;
+    case 2:
+        k_level<ST, PRE, TI, TJ, TK, 2, CT><<<grid, 1024, 0, s>>>(d, dc, fx, fy, fz, b, pin, pcoarse, pout, bc,
+                                                                  omega, done);
+        break;
+    case 3:
+        k_level<ST, PRE, TI, TJ, TK, 3, CT><<<grid, 1024, 0, s>>>(d, dc, fx, fy, fz, b, pin, pcoarse, pout, bc,
+                                                                  omega, done);
+        break;
+    default:
+        PF_REQUIRE(false, "launch_level: 1 to 3 sweeps");
+    }
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
 // workgroups of launch_full_sweep on a level of dims d (one per tile)
 static int64_t full_sweep_blocks(const Dims& d) {
     const int64_t n = (int64_t)d.nx * d.ny * d.nz;
@@ -1535,6 +1780,14 @@ static ST* second_buffer(const pf_mg& mg, int l) {
 
 static bool fused_smoother(const pf_mg& mg, int l) {
     return !(mg.options & PF_MG_UNFUSED_SMOOTHER) && (mg.lev[l].nz % 2) == 0;
+}
+
+// level l (restricted to l + 1, below level 0) runs each smoothing phase
+// as one k_level launch
+static bool level_fused(const pf_mg& mg, int l, int G) {
+    return l >= 1 && l < G && !(mg.options & (PF_MG_UNFUSED_SMOOTHER | PF_MG_SPLIT_LEVELS)) &&
+           ncell(mg.lev[l]) <= kLevelFusedMaxCells && mg.pre >= 1 && mg.pre <= 3 && mg.post >= 1 &&
+           mg.post <= 3;
 }
 
 // One V-cycle: z0 = M^{-1} r0 (no mean removal at level 0).
@@ -1602,10 +1855,19 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
     };
 
     for (int l = 0; l < G; ++l) {
-        int rc = smooth(l, mg.pre, false, true);
-        if (rc) return rc;
         const pf_mg_level& lv = mg.lev[l];
         const pf_mg_level& cv = mg.lev[l + 1];
+        if (level_fused(mg, l, G)) {
+            cur[l] = P(l);
+            const int rc = launch_level<ST, true>(ldims(lv), ldims(cv), mg.pre, Lv<ST>::fx(lv), Lv<ST>::fy(lv),
+                                                  Lv<ST>::fz(lv), B(l), static_cast<const ST*>(nullptr),
+                                                  static_cast<const float*>(nullptr), cur[l], mg.lev[l + 1].b,
+                                                  mg.omega, done, s);
+            if (rc) return rc;
+            continue;
+        }
+        int rc = smooth(l, mg.pre, false, true);
+        if (rc) return rc;
         k_resid_restrict<ST><<<cdiv(ncell(cv), RT), RT, 0, s>>>(ldims(lv), ldims(cv), Lv<ST>::fx(lv),
                                                               Lv<ST>::fy(lv), Lv<ST>::fz(lv), cur[l], B(l),
                                                               mg.lev[l + 1].b, done, l == 0 ? bscale : nullptr);
@@ -1679,6 +1941,20 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
     for (int l = G - 1; l >= 0; --l) {
         const pf_mg_level& lv = mg.lev[l];
         const pf_mg_level& cv = mg.lev[l + 1];
+        if (level_fused(mg, l, G)) {
+            ST* out = other(l, cur[l]);
+            const int rc =
+                (l + 1 == sm)
+                    ? launch_level<ST, false, float>(ldims(lv), ldims(cv), mg.post, Lv<ST>::fx(lv), Lv<ST>::fy(lv),
+                                                     Lv<ST>::fz(lv), B(l), cur[l], cv.p, out, nullptr, mg.omega,
+                                                     done, s)
+                    : launch_level<ST, false, ST>(ldims(lv), ldims(cv), mg.post, Lv<ST>::fx(lv), Lv<ST>::fy(lv),
+                                                  Lv<ST>::fz(lv), B(l), cur[l], cur[l + 1], out, nullptr,
+                                                  mg.omega, done, s);
+            if (rc) return rc;
+            cur[l] = out;
+            continue;
+        }
         // the first post-sweep forms the prolongation itself when the last
         // post-sweep then lands where it must (level 0: in z0)
         if (fused_smoother(mg, l) && mg.post > 0 &&

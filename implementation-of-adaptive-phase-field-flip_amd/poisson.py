@@ -21,6 +21,10 @@ Differences from the reference that do not change the algorithm:
   buffer and writes it to a second one (``levels[l].r``, and ``levels[0].p``
   in FP32); ``fused_smoother=False`` runs the two half-sweeps in place, with
   the same result bit for bit;
+* on the small levels below level 0 (up to 32^3 cells) the pre-sweeps with
+  the restriction, and the prolongation with the post-sweeps, are one kernel
+  each over LDS-resident tiles with redundant halos; ``split_levels=True``
+  runs them one kernel per sweep (the same result bit for bit);
 * the CG's inner products after the preconditioner (sum z, sum r, sum Ap,
   <r, z>, <Ap, z>) are formed by the V-cycle's last level-0 sweep from the
   values it holds, one FP64 partial per tile; ``fold_dots=False`` forms
@@ -79,7 +83,8 @@ class MGPCG:
     """Multigrid-preconditioned flexible CG (pfflip/poisson.py:240-409)."""
 
     def __init__(self, res, device, *, omega=6.0 / 7.0, pre=2, post=2, coarsest=60, min_dim=4,
-                 precision="fp32", fused_smoother=True, fold_dots=True):
+                 precision="fp32", fused_smoother=True, fold_dots=True,
+                 split_levels=False):
         self.device = _require_cuda(device)
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
@@ -104,7 +109,8 @@ class MGPCG:
         # one fused kernel per red-black sweep (default) or one per half-sweep;
         # both give the same bits
         mg.options = (0 if fused_smoother else _lib.MG_UNFUSED_SMOOTHER) | \
-            (0 if fold_dots else _lib.MG_SEPARATE_DOTS)
+            (0 if fold_dots else _lib.MG_SEPARATE_DOTS) | \
+            (_lib.MG_SPLIT_LEVELS if split_levels else 0)
         for i, L in enumerate(self.levels):
             mg.lev[i].nx, mg.lev[i].ny, mg.lev[i].nz = L.res
         _lib.check(lib.pf_mg_setup(ctypes.byref(mg)), "pf_mg_setup")

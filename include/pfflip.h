@@ -228,6 +228,12 @@ typedef struct pf_mg {
  * them from the values it holds (the same sums in another order: results
  * agree to rounding; for testing and timing). */
 #define PF_MG_SEPARATE_DOTS 2
+/* Run the small levels below level 0 (up to 32^3 cells) with one kernel per
+ * sweep like the larger ones; by default each of their smoothing phases
+ * (pre-sweeps with the restriction, prolongation with the post-sweeps) is
+ * one kernel over LDS-resident tiles (same results bit for bit; for testing
+ * and timing). */
+#define PF_MG_SPLIT_LEVELS 4
 
 size_t pf_mg_scalar_bytes(void);
 /* Validate the level table and choose small_from. */

@@ -499,13 +499,19 @@ def test_mgpcg_matches_oracle_iterations(pf):
 @pytest.mark.parametrize("res,pre,post", [((32, 32, 32), 3, 3), ((48, 40, 24), 3, 3),
                                           ((64, 64, 64), 2, 1), ((128, 64, 32), 1, 2),
                                           ((128, 128, 128), 3, 3), ((68, 136, 120), 2, 1),
-                                          ((104, 118, 190), 3, 2)])
+                                          ((104, 118, 190), 3, 2), ((68, 60, 44), 3, 3),
+                                          ((112, 112, 20), 2, 3)])
 def test_fused_smoother_is_bitwise_split(pf, res, pre, post, precision):
     """The fused full-sweep smoother (LDS tile, halo recomputation, ping-pong
-    buffers) gives exactly the bits of two in-place half-sweeps, including
-    partial tiles and odd sweep counts; 68 x 136 x 120 takes the large-level
-    kernel (from 2^20 cells) with partial tiles along i and k, and
-    104 x 118 x 190 (585 tiles) with partial tiles along j and k."""
+    buffers) and the one-launch phases of the small levels give exactly the
+    bits of two in-place half-sweeps per sweep, including partial tiles and
+    odd sweep counts; 68 x 136 x 120 takes the large-level kernel (from 2^20
+    cells) with partial tiles along i and k, and 104 x 118 x 190 (585 tiles)
+    with partial tiles along j and k.  One-launch phases: level 1 of 64^3
+    (32^3, pre 2 / post 1) and of 128 x 64 x 32 (pre 1 / post 2), level 2 of
+    128^3, level 1 of 68 x 60 x 44 (34 x 30 x 22: partial tiles on every
+    axis), and level 1 of 112 x 112 x 20 (56 x 56 x 10, whose coarse level
+    28 x 28 x 5 is the coarsest and runs out of HBM)."""
     say(f"fused smoother {res} pre={pre} post={post} {precision}")
     rng = np.random.default_rng(11)
     bx, by, bz = random_beta(res, rng, 1000.0)
@@ -541,6 +547,31 @@ def test_fused_smoother_solve_is_bitwise_split(pf):
         it, rel = solver.solve(g(b), p, tol=1e-6, max_iter=200)
         out.append((it, rel, np_(p)))
         del solver
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_fused_levels_solve_is_bitwise_split(pf, precision):
+    """Whole FPCG solves at 128^3 with the 32^3 level's smoothing phases as
+    one launch each and as one launch per sweep: same iterations, residual
+    and pressure bit for bit."""
+    say(f"fused levels solve 128^3 {precision}")
+    res = (128, 128, 128)
+    rng = np.random.default_rng(15)
+    bx, by, bz = random_beta(res, rng, 1000.0)
+    b = rng.standard_normal(res).astype(np.float32)
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
+    out = []
+    for split in (False, True):
+        solver = pf.MGPCG(res, "cuda:0", pre=3, post=3, precision=precision, split_levels=split)
+        solver.build_hierarchy(g(bx), g(by), g(bz))
+        p = torch.zeros(res, device="cuda")
+        it, rel = solver.solve(g(b), p, tol=1e-6, max_iter=200)
+        out.append((it, rel, np_(p)))
+        del solver
+    say(f"  {out[0][0]} it rel {out[0][1]:.3e}")
+    assert out[0][1] < 1e-6
     assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
     assert np.array_equal(out[0][2], out[1][2])
 
