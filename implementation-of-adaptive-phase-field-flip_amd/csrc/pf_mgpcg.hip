@@ -781,8 +781,11 @@ struct LvBox {
     static constexpr int NT = 1024;
     static constexpr int H = 2 * NSW;
     static constexpr int BI = TI + 2 * H, BJ = TJ + 2 * H, BK = TK + 2 * H;
-    // one extra layer per axis holds the high faces of the outermost cells
-    static constexpr int SJ = BK + 1, SI = (BJ + 1) * (BK + 1), N = (BI + 1) * SI;
+    // one extra layer per axis holds the high faces of the outermost cells;
+    // rows of BK + 2 (even) entries: with odd strides every cell of one
+    // colour would sit at an address of one parity, so a half-sweep's
+    // accesses would use half the LDS banks
+    static constexpr int SJ = BK + 2, SI = (BJ + 1) * SJ, N = (BI + 1) * SI;
     static constexpr int NPT = (N + NT - 1) / NT;
     static_assert(TI % 2 == 0 && TJ % 2 == 0 && TK % 2 == 0, "tiles restrict to whole coarse cells");
 };
