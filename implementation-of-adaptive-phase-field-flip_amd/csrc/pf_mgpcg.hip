@@ -730,9 +730,10 @@ __global__ __launch_bounds__(RT) void k_resid_restrict(Dims d, Dims dc, const ST
                                                        const float* __restrict__ bscale) {
     const bool skip = done && *done;
     const float bs = bscale ? *bscale : 1.0f;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)dc.nx * dc.ny * dc.nz) return;
-    const int K = (int)(t % dc.nz), J = (int)((t / dc.nz) % dc.ny), I = (int)(t / ((int64_t)dc.nz * dc.ny));
+    // 32-bit index arithmetic (levels hold fewer than 2^31 cells)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= dc.nx * dc.ny * dc.nz) return;
+    const int K = t % dc.nz, J = (t / dc.nz) % dc.ny, I = t / (dc.nz * dc.ny);
     float acc = 0.0f;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -1463,20 +1464,39 @@ __global__ __launch_bounds__(WT) void k_cg_apply(CGK<ZT> a) {
         const float zz = zval(a, zs, c) + cm;
         return zz + be * a.q_old[c];
     };
-    for (int64_t c = (int64_t)blockIdx.x * WT + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * WT) {
-        const int k = (int)(c % d.nz), j = (int)((c / d.nz) % d.ny), i = (int)(c / ((int64_t)d.nz * d.ny));
-        const float f[6] = {a.bx[face_index<0>(d, i, j, k)], a.bx[face_index<0>(d, i + 1, j, k)],
-                            a.by[face_index<1>(d, i, j, k)], a.by[face_index<1>(d, i, j + 1, k)],
-                            a.bz[face_index<2>(d, i, j, k)], a.bz[face_index<2>(d, i, j, k + 1)]};
-        const float pc = qn((int)c);
-        const float pn[6] = {qn(cell_index(d, max(i - 1, 0), j, k)), qn(cell_index(d, min(i + 1, d.nx - 1), j, k)),
-                             qn(cell_index(d, i, max(j - 1, 0), k)), qn(cell_index(d, i, min(j + 1, d.ny - 1), k)),
-                             qn(cell_index(d, i, j, max(k - 1, 0))), qn(cell_index(d, i, j, min(k + 1, d.nz - 1)))};
+    // 32-bit indices (the level's face arrays hold fewer than 2^31
+    // entries): the thread's first cell and the grid stride are decoded into
+    // (i, j, k) once, and the loop adds the stride with carries; the faces
+    // and the clamped neighbours are offsets from the cell index
+    // (face_index and cell_index written out).  64-bit division per cell
+    // made this loop several times longer.
+    const int n = (int)a.n, sjk = d.ny * d.nz;
+    const int c0 = blockIdx.x * WT + threadIdx.x, S = gridDim.x * WT;
+    int k = c0 % d.nz, j = (c0 / d.nz) % d.ny, i = c0 / sjk;
+    const int sk = S % d.nz, sj = (S / d.nz) % d.ny, si = S / sjk;
+    for (int c = c0; c < n; c += S) {
+        const int fy = c + i * d.nz, fz = c + i * d.ny + j;
+        const float f[6] = {a.bx[c], a.bx[c + sjk], a.by[fy], a.by[fy + d.nz], a.bz[fz], a.bz[fz + 1]};
+        const float pc = qn(c);
+        const float pn[6] = {qn(i > 0 ? c - sjk : c),  qn(i < d.nx - 1 ? c + sjk : c),
+                             qn(j > 0 ? c - d.nz : c), qn(j < d.ny - 1 ? c + d.nz : c),
+                             qn(k > 0 ? c - 1 : c),    qn(k < d.nz - 1 ? c + 1 : c)};
         float dg, Ap;
         stencil_vals(pc, f, pn, dg, Ap);
         a.q[c] = pc;
         a.Ap[c] = Ap;
         v[0] += (double)pc * (double)Ap;
+        k += sk;
+        if (k >= d.nz) {
+            k -= d.nz;
+            ++j;
+        }
+        j += sj;
+        if (j >= d.ny) {
+            j -= d.ny;
+            ++i;
+        }
+        i += si;
     }
     block_sum<1>(v, sh);
     if (threadIdx.x == 0) st->part[5 * MAXB + blockIdx.x] = v[0];
@@ -2385,6 +2405,8 @@ static int check_mg(const pf_mg* mg) {
     for (int l = 0; l < mg->nlev; ++l) {
         const pf_mg_level& v = mg->lev[l];
         PF_REQUIRE(v.nx > 0 && v.ny > 0 && v.nz > 0, "pf_mg: level %d has bad dims", l);
+        // the kernels index cells and faces with 32-bit ints
+        PF_REQUIRE((int64_t)(v.nx + 1) * (v.ny + 1) * (v.nz + 1) < ((int64_t)1 << 31), "pf_mg: level %d too large", l);
         PF_REQUIRE(v.bx && v.by && v.bz && v.b, "pf_mg: level %d missing arrays", l);
         if (l > 0) {
             const pf_mg_level& f = mg->lev[l - 1];
