@@ -1007,9 +1007,10 @@ struct SmallArgs {
     int off[PF_MAX_LEVELS][5];  // LDS float offsets of bx, by, bz, p, b
     int scratch;                // LDS float offset of the residual scratch (n(s) floats)
     // global -> LDS copies at entry: segment q copies seg_len[q] floats from
-    // seg_src[q] to LDS offset seg_dst[q] (the face arrays of every level,
-    // then the top level's b)
+    // seg_src[q] to LDS offset seg_dst[q] (the top level's faces and b, then
+    // the coarser levels' faces and the coarsest level's matrix)
     int nseg;
+    int nseg_first;  // segments [0, nseg_first): the top level's; the rest land during its descent
     const float* seg_src[3 * PF_MAX_LEVELS + 2];
     int seg_dst[3 * PF_MAX_LEVELS + 2];
     int seg_len[3 * PF_MAX_LEVELS + 2];
@@ -1131,6 +1132,11 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroup barrier over LDS only: unlike __syncthreads it does not wait
+// for loads in flight, so the coarser levels' staging overlaps the top
+// level's descent.
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // The slots of both colours of one level for this thread (colour c0 first).
 struct LevelSlots {
     Slot s0[kSlots], s1[kSlots];
@@ -1165,12 +1171,12 @@ __device__ inline void small_smooth(const LdsLevel& v, const LevelSlots& L, int 
     } else {
         for (int sw = 0; sw < sweeps; ++sw) {
             slots_sweep(v, L.s0);
-            __syncthreads();
+            lds_barrier();
             slots_sweep(v, L.s1);
-            __syncthreads();
+            lds_barrier();
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // The whole V-cycle below `s` in one workgroup, every level in LDS.
@@ -1179,31 +1185,50 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
     extern __shared__ float lds[];
     const int tid = threadIdx.x, nt = kSmallThreads;
     const int dn = a.done ? *a.done : 0;
-    // faces of every level and the top b straight into LDS (lane l of a wave
-    // writes its dword to the wave's LDS base + 4 l)
-    for (int q = 0; q < a.nseg; ++q) {
-        const float* src = a.seg_src[q];
-        const int len = a.seg_len[q];
-        float* dst = lds + a.seg_dst[q];
-        for (int e0 = 0; e0 < len; e0 += nt)
-            if (e0 + tid < len)
-                __builtin_amdgcn_global_load_lds((const void*)(src + e0 + tid),
-                                                 (__attribute__((address_space(3))) void*)(dst + e0 + (tid & ~63)),
-                                                 4, 0, 0);
-    }
+    // segments straight from memory into LDS (lane l of a wave writes its
+    // dword to the wave's LDS base + 4 l)
+    auto stage = [&](int q0, int q1) {
+        for (int q = q0; q < q1; ++q) {
+            const float* src = a.seg_src[q];
+            const int len = a.seg_len[q];
+            float* dst = lds + a.seg_dst[q];
+            for (int e0 = 0; e0 < len; e0 += nt)
+                if (e0 + tid < len)
+                    __builtin_amdgcn_global_load_lds(
+                        (const void*)(src + e0 + tid), (__attribute__((address_space(3))) void*)(dst + e0 + (tid & ~63)),
+                        4, 0, 0);
+        }
+    };
+    // the top level's faces and b first; the coarser levels' faces and the
+    // coarsest matrix are issued after that wait and land while the top
+    // level is smoothed (its barriers are LDS-only), waited for before
+    // their first use
+    stage(0, a.nseg_first);
     __builtin_amdgcn_s_waitcnt(0);
     if (dn) return;  // uniform: no workgroup barrier has been reached yet
-    __syncthreads();
+    stage(a.nseg_first, a.nseg);
+    bool staged = a.nseg_first >= a.nseg;
+    auto wait_staged = [&]() {
+        if (!staged) {
+            __builtin_amdgcn_s_waitcnt(0);
+            staged = true;
+        }
+    };
+    lds_barrier();
     float* scr = lds + a.scratch;
     LevelSlots S;
     // descend: smooth from zero, restrict the residual (poisson.py:323-336)
     for (int l = a.s; l < a.L - 1; ++l) {
+        if (l > a.s) {
+            wait_staged();
+            lds_barrier();
+        }
         const LdsLevel v = lds_level(a, lds, l);
         const LdsLevel c = lds_level(a, lds, l + 1);
         const int n = v.d.nx * v.d.ny * v.d.nz;
         for (int e = tid; e < n; e += nt) v.p[e] = 0.0f;
         make_level_slots(v, 0, a.omega, &c.d, S);
-        __syncthreads();
+        lds_barrier();
         small_smooth(v, S, a.pre);
         // residual of every fine cell into the scratch, then R = P^T
 #pragma unroll
@@ -1217,7 +1242,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
                 scr[S.s1[q].c] = S.s1[q].b - slot_Ap(v, S.s1[q], pc);
             }
         }
-        __syncthreads();
+        lds_barrier();
         const int nc = c.d.nx * c.d.ny * c.d.nz;
         const float inz = 1.0f / (float)c.d.nz, iny = 1.0f / (float)c.d.ny;
         for (int t = tid; t < nc; t += nt) {
@@ -1232,23 +1257,45 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_vcycle(SmallArgs a) {
                     for (int z = 0; z < 2; ++z) acc += scr[cell_index(v.d, 2 * I + x, 2 * J + y, 2 * K + z)];
             c.b[t] = acc;
         }
-        __syncthreads();
+        lds_barrier();
     }
+    wait_staged();
+    lds_barrier();
     // coarsest: sweeps, then remove the mean (poisson.py:325-328)
     if (a.cop) {
-        // the same map, precomputed: p = M b (column j of M contiguous)
+        // the same map, precomputed: p = M b (column j of M contiguous).
+        // Each output's sum is split over G groups of n threads (a
+        // contiguous range of j each; a wave reads 64 consecutive entries
+        // of one column), and the G partials are added in group order
+        // (the FP64 partials go through the scratch, 8-byte aligned).
         const LdsLevel v = lds_level(a, lds, a.L - 1);
         const int n = v.d.nx * v.d.ny * v.d.nz;
+        const int G = min(nt / n, 16), J = (n + G - 1) / G;
+        double* part = reinterpret_cast<double*>(scr);
+        // (one instance per memory kind, so the loads are LDS or global
+        // loads, not generic ones)
+        auto partial = [&](const auto* M) {
+            if (tid < n * G) {
+                const int t = tid % n, g = tid / n;
+                const int j1 = min(n, (g + 1) * J);
+                double acc = 0.0;
+#pragma unroll 4
+                for (int j = g * J; j < j1; ++j) acc += M[(int64_t)j * n + t] * (double)v.b[j];
+                part[g * n + t] = acc;
+            }
+        };
         // the matrix staged in LDS with the faces, else read from memory
-        const double* M = a.cop_lds >= 0 ? reinterpret_cast<const double*>(lds + a.cop_lds) : a.cop;
+        if (a.cop_lds >= 0)
+            partial(reinterpret_cast<const double*>(lds + a.cop_lds));
+        else
+            partial(a.cop);
+        lds_barrier();
         for (int t = tid; t < n; t += nt) {
-            double acc = 0.0;
-            // 16 loads in flight per wait (the sum keeps its order)
-#pragma unroll 16
-            for (int j = 0; j < n; ++j) acc += M[(int64_t)j * n + t] * (double)v.b[j];
+            double acc = part[t];
+            for (int g = 1; g < G; ++g) acc += part[g * n + t];
             v.p[t] = (float)acc;
         }
-        __syncthreads();
+        lds_barrier();
     } else {
         const LdsLevel v = lds_level(a, lds, a.L - 1);
         const int n = v.d.nx * v.d.ny * v.d.nz;
@@ -1665,8 +1712,11 @@ static int64_t small_lds_floats(const pf_mg& mg, int s, int (*off)[5], int* scra
         if (off) off[l][4] = (int)o;
         o += nc;
     }
+    // the scratch: the top level's residual, and the FP64 partials of the
+    // coarsest level's matrix product (8-byte aligned, at most kSmallThreads)
+    o = (o + 1) & ~(int64_t)1;
     if (scratch) *scratch = (int)o;
-    return o + ncell(mg.lev[s]);
+    return o + std::max<int64_t>(ncell(mg.lev[s]), 2 * kSmallThreads);
 }
 
 constexpr int64_t kSmallMaxCells = 8192;
@@ -1914,13 +1964,17 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
             a.seg_dst[a.nseg] = dst;
             a.seg_len[a.nseg++] = (int)n;
         };
+        // the top level's faces and b, then the coarser levels' faces
         for (int l = sm; l < L; ++l) {
             const Dims d = a.d[l];
             seg(mg.lev[l].bx, a.off[l][0], face_count(d, 0));
             seg(mg.lev[l].by, a.off[l][1], face_count(d, 1));
             seg(mg.lev[l].bz, a.off[l][2], face_count(d, 2));
+            if (l == sm) {
+                seg(B(sm), a.off[sm][4], ncell(mg.lev[sm]));
+                a.nseg_first = a.nseg;
+            }
         }
-        seg(B(sm), a.off[sm][4], ncell(mg.lev[sm]));
         a.done = done;
         a.cop = use_coarse_op(mg) ? static_cast<CGState*>(mg.scalars)->coarse_op : nullptr;
         a.cop_lds = -1;
@@ -2109,17 +2163,21 @@ static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int*
 // HIP graphs of the prologue and of one iteration (two instances that differ
 // only in which search-direction buffer they write), cached per solver layout.  A solve issues
 // one graph launch per iteration instead of ~50 kernel launches.
+// pair: an odd iteration and the even one after it in one graph; the GPU
+// spends ~8 us between two graphs, none between two kernels of one graph
 struct SolveGraphs {
     std::vector<uint64_t> key;
     hipGraphExec_t pro = nullptr;
     hipGraphExec_t it[2] = {nullptr, nullptr};
+    hipGraphExec_t pair = nullptr;
 };
 
 static void destroy_graphs(SolveGraphs& g) {
     if (g.pro) (void)hipGraphExecDestroy(g.pro);
     for (auto& e : g.it)
         if (e) (void)hipGraphExecDestroy(e);
-    g.pro = g.it[0] = g.it[1] = nullptr;
+    if (g.pair) (void)hipGraphExecDestroy(g.pair);
+    g.pro = g.it[0] = g.it[1] = g.pair = nullptr;
 }
 
 template <typename F>
@@ -2198,6 +2256,16 @@ static int get_graphs(SolverState& st, const pf_mg& mg, const CGK<ST>& a, ST* z,
         const CGK<ST> aq = slot_args(a, mg, q);
         rc = capture(cs, [&](hipStream_t t) { return enqueue_iteration<ST>(mg, aq, z, done, st.flag_d, nb, t); },
                      &g.it[q]);
+    }
+    if (!rc) {
+        const CGK<ST> a1 = slot_args(a, mg, 1), a0 = slot_args(a, mg, 0);
+        rc = capture(
+            cs,
+            [&](hipStream_t t) {
+                const int r1 = enqueue_iteration<ST>(mg, a1, z, done, st.flag_d, nb, t);
+                return r1 ? r1 : enqueue_iteration<ST>(mg, a0, z, done, st.flag_d, nb, t);
+            },
+            &g.pair);
     }
     (void)hipStreamDestroy(cs);
     if (rc) {
@@ -2342,23 +2410,34 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
         rc = enqueue_prologue<ST>(mg, a, z, done, nb, s);
         if (rc) return rc;
     }
-    for (int it = 1; it <= max_iter; ++it) {
+    for (int it = 1; it <= max_iter;) {
         const int slot = it & 1;
+        // graphs: iterations in (odd, even) pairs, one graph each; then wait
+        // for the pair's first residual test (it opens the pair's first
+        // kernel), so one and a half iterations stay queued.  Single
+        // iterations (no graphs, or an odd max_iter's last): wait for the
+        // previous iteration's test.  Either way the GPU always has queued
+        // work, and the host stops once a published test ended the solve.
+        int poll = it - 1, step = 1;
         if (gr) {
-            PF_CHECK_HIP(hipGraphLaunch(gr->it[slot], s));
+            if (slot == 1 && it < max_iter) {
+                PF_CHECK_HIP(hipGraphLaunch(gr->pair, s));
+                poll = it;
+                step = 2;
+            } else {
+                PF_CHECK_HIP(hipGraphLaunch(gr->it[slot], s));
+            }
         } else {
             rc = enqueue_iteration<ST>(mg, slot_args(a, mg, slot), z, done, hp->flag_d, nb, s);
             if (rc) return rc;
         }
-        // poll one iteration behind, so the GPU always has queued work: wait
-        // for iteration it-1's residual test to be published, then stop if
-        // it (or an earlier one) ended the solve
-        if (it >= 2) {
+        if (poll >= 1) {
             bool stop = false;
-            rc = wait_published(hp->flag, it - 1, s, &stop);
+            rc = wait_published(hp->flag, poll, s, &stop);
             if (rc) return rc;
             if (stop) break;
         }
+        it += step;
     }
     // _remove_mean(x) (poisson.py:408)
     k_part_sum<float><<<nb, RT, 0, s>>>(x, n, st->part);
