@@ -573,6 +573,10 @@ __global__ __launch_bounds__(NTH, 4) void k_rbgs_full(Dims d, const ST* __restri
     } else if (bscale) {
         bs = *bscale;
     }
+    // a finished solve's queued iterations stop here: after the loads (the
+    // flag is not waited for before issuing them) but before any arithmetic
+    // (every load has landed; skip is the same in the whole workgroup)
+    if (skip) return;
     if (bscale) {
 #pragma unroll
         for (int q = 0; q < T::PPT; ++q) {
@@ -926,6 +930,7 @@ __global__ __launch_bounds__(1024) void k_level(Dims d, Dims dc, const ST* __res
         }
     }
     __syncthreads();
+    if (skip) return;  // a finished solve's queued iteration: no sweeps (uniform)
     // PRE: the first half-sweep from zero runs on the whole box (it reads no
     // neighbour), the last at margin H - 1; !PRE: margins 1 .. H
     lv_sweeps<ST, BX, PRE ? 0 : 1, 0, PRE>(P, B, FX, FY, FZ, d, ox, oy, oz, PRE ? 0 : 1, omega);
