@@ -9,7 +9,7 @@
 // from `small_from` down is solved inside one workgroup with all its data
 // in LDS.  The FPCG scalars live on the
 // device; dot products are deterministic FP64 block reductions, and the
-// host only polls a "done" flag one iteration behind the GPU.
+// host polls a "done" flag while the GPU still has queued work.
 #include <algorithm>
 #include <atomic>
 #include <cstdlib>
@@ -2165,9 +2165,10 @@ static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int*
     return PF_OK;
 }
 
-// HIP graphs of the prologue and of one iteration (two instances that differ
-// only in which search-direction buffer they write), cached per solver layout.  A solve issues
-// one graph launch per iteration instead of ~50 kernel launches.
+// HIP graphs of the prologue, of one iteration (two instances that differ
+// only in which search-direction buffer they write) and of an odd and an
+// even iteration together, cached per solver layout.  A solve issues one
+// graph launch per two iterations instead of ~20 kernel launches each.
 // pair: an odd iteration and the even one after it in one graph; the GPU
 // spends ~8 us between two graphs, none between two kernels of one graph
 struct SolveGraphs {
@@ -2417,17 +2418,17 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
     }
     for (int it = 1; it <= max_iter;) {
         const int slot = it & 1;
-        // graphs: iterations in (odd, even) pairs, one graph each; then wait
-        // for the pair's first residual test (it opens the pair's first
-        // kernel), so one and a half iterations stay queued.  Single
-        // iterations (no graphs, or an odd max_iter's last): wait for the
-        // previous iteration's test.  Either way the GPU always has queued
-        // work, and the host stops once a published test ended the solve.
-        int poll = it - 1, step = 1;
+        // graphs: iterations in (odd, even) pairs, one graph each (single
+        // iterations without graphs and for an odd max_iter's last).  Then
+        // wait for the last launched iteration's residual test: it opens
+        // that iteration's first kernel, so the rest of the iteration (about
+        // 200 us at 128^3) is still queued while the host launches the next
+        // one, and after convergence at most the iteration that found it and
+        // one more run (as early exits).
+        int step = 1;
         if (gr) {
             if (slot == 1 && it < max_iter) {
                 PF_CHECK_HIP(hipGraphLaunch(gr->pair, s));
-                poll = it;
                 step = 2;
             } else {
                 PF_CHECK_HIP(hipGraphLaunch(gr->it[slot], s));
@@ -2436,12 +2437,10 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
             rc = enqueue_iteration<ST>(mg, slot_args(a, mg, slot), z, done, hp->flag_d, nb, s);
             if (rc) return rc;
         }
-        if (poll >= 1) {
-            bool stop = false;
-            rc = wait_published(hp->flag, poll, s, &stop);
-            if (rc) return rc;
-            if (stop) break;
-        }
+        bool stop = false;
+        rc = wait_published(hp->flag, it + step - 1, s, &stop);
+        if (rc) return rc;
+        if (stop) break;
         it += step;
     }
     // _remove_mean(x) (poisson.py:408)

@@ -8,7 +8,8 @@ the work is done by ``pf_mg_*`` / ``pf_mgpcg_solve`` in libpfflip_hip.so.
 Differences from the reference that do not change the algorithm:
 
 * the FPCG scalars stay on the GPU; ``solve`` synchronises with the host
-  about once per iteration (one iteration behind) instead of six times; the
+  about once per two iterations (while the GPU has queued work) instead of
+  six times per iteration; the
   solver owns its host state (pinned flags, captured HIP graphs) through
   ``pf_mgpcg_state_create`` and frees it with the solver;
 * dot products and means are deterministic FP64 reductions instead of FP32
