@@ -624,6 +624,39 @@ def test_mgpcg_zero_rhs_and_max_iter(pf):
     assert it == 3 and 0.0 < rel < 1.0
 
 
+def test_graph_launches_equal_kernel_launches(pf):
+    """The solver state's captured graphs (iterations launched in odd/even
+    pairs, a single one for an odd max_iter's last; the host waits for the
+    last launched iteration's residual test) against a solve without solver
+    state (every kernel launched one by one):
+    same iterations, residual and pressure bit for bit, for iteration caps
+    that end on a pair, on a single iteration and before convergence, and
+    for a converging solve (whose queued iterations run as early exits)."""
+    say("graph pairs vs kernel launches 48^3")
+    res = (48, 48, 48)
+    rng = np.random.default_rng(16)
+    bx, by, bz = random_beta(res, rng, 1000.0)
+    b = rng.standard_normal(res).astype(np.float32)
+    g = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
+    solver = pf.MGPCG(res, "cuda:0", pre=3, post=3)
+    solver.build_hierarchy(g(bx), g(by), g(bz))
+    out = {}
+    for graphs in (True, False):
+        solver._mg.host_state = solver._state if graphs else None
+        for cap, tol in ((1, 1e-30), (2, 1e-30), (5, 1e-30), (8, 1e-30), (300, 1e-6)):
+            p = torch.zeros(res, device="cuda")
+            it, rel = solver.solve(g(b), p, tol=tol, max_iter=cap)
+            out[(graphs, cap)] = (it, rel, np_(p))
+    solver._mg.host_state = solver._state
+    for cap in (1, 2, 5, 8, 300):
+        (i0, r0, p0), (i1, r1, p1) = out[(True, cap)], out[(False, cap)]
+        assert i0 == i1 and r0 == r1, (cap, i0, i1, r0, r1)
+        assert np.array_equal(p0, p1), cap
+        if cap < 300:
+            assert i0 == cap
+    assert out[(True, 300)][1] < 1e-6
+
+
 # ------------------------------------------------------------------ projection / extrapolation
 def test_project_and_divergence_match_oracle(pf):
     say("project 16^3")
