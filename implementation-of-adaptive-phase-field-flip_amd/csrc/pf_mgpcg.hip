@@ -89,6 +89,29 @@ __device__ inline void residual_test_record(CGState* st, float m, int* host_flag
         __hip_atomic_store(host_flag, st->seq * 2 + (st->done != 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Pinned host words of a solver: the residual tests' flag, then the
+// solve's result (its number, the iteration count, the residual as the bits
+// of a double).
+// HW_WANT: the number the next result is to carry, written by the host
+// before the launch (a captured graph cannot take it as an argument).
+enum HostWord { HW_FLAG = 0, HW_RESULT_NO = 1, HW_IT = 2, HW_WANT = 3, HW_REL = 4, HW_WORDS = 8 };
+
+// The solve's result for the host: the iteration count and the residual,
+// then the solve's number, into pinned host memory with system-scope stores
+// (the number last, with release order), so the host spins on one word
+// instead of copying the scalars back and synchronising the stream.  The
+// kernels before it on the stream have completed when it runs.
+__global__ void k_publish_result(const CGState* st, int* words) {
+    if (threadIdx.x != 0) return;
+    const int number = __hip_atomic_load(words + HW_WANT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const int it = st->it;
+    const unsigned long long rel = (unsigned long long)__double_as_longlong(st->rel);
+    __hip_atomic_store(words + HW_IT, it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(words + HW_REL), rel, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(words + HW_RESULT_NO, number, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // The residual test folded into the V-cycle's first kernel (the zero-start
 // level-0 sweep): every block takes the max of k_cg_update's partials itself
 // (max is exact, so all blocks agree) and so knows whether to store and the
@@ -2082,8 +2105,9 @@ static int vcycle(const pf_mg& mg, const float* r0, ST* z0, const int* done, con
 struct SolveGraphs;
 struct SolverState {
     int device = -1;
-    int* flag = nullptr;    // pinned int the residual tests publish to (host address)
+    int* flag = nullptr;    // HW_WORDS pinned ints (host address)
     int* flag_d = nullptr;  // the same memory as the device addresses it
+    int solves = 0;         // results published so far
     std::vector<SolveGraphs>* graphs = nullptr;
 };
 
@@ -2092,9 +2116,9 @@ static void release_state(SolverState& st);
 static int init_state(SolverState& st) {
     PF_CHECK_HIP(hipGetDevice(&st.device));
     // coherent: the device's stores reach host memory without cache flushes
-    PF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.flag), 2 * sizeof(int),
+    PF_CHECK_HIP(hipHostMalloc(reinterpret_cast<void**>(&st.flag), HW_WORDS * sizeof(int),
                                hipHostMallocMapped | hipHostMallocCoherent));
-    st.flag[0] = st.flag[1] = 0;
+    for (int w = 0; w < HW_WORDS; ++w) st.flag[w] = 0;
     PF_CHECK_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&st.flag_d), st.flag, 0));
     return PF_OK;
 }
@@ -2165,6 +2189,19 @@ static int enqueue_prologue(const pf_mg& mg, const CGK<ST>& a, ST* z, const int*
     return PF_OK;
 }
 
+// _remove_mean(x) (poisson.py:408), then the result for the host
+static int enqueue_epilogue(CGState* st, float* x, int64_t n, unsigned nb, int* words, hipStream_t s) {
+    k_part_sum<float><<<nb, RT, 0, s>>>(x, n, st->part);
+    PF_CHECK_LAUNCH();
+    k_comb<<<1, RT, 0, s>>>(st, C_MEAN, (int)nb, n, &st->mu_x, 0.0);
+    PF_CHECK_LAUNCH();
+    k_shift<float, float><<<nb, RT, 0, s>>>(x, x, n, &st->mu_x);
+    PF_CHECK_LAUNCH();
+    k_publish_result<<<1, 64, 0, s>>>(st, words);
+    PF_CHECK_LAUNCH();
+    return PF_OK;
+}
+
 // HIP graphs of the prologue, of one iteration (two instances that differ
 // only in which search-direction buffer they write) and of an odd and an
 // even iteration together, cached per solver layout.  A solve issues one
@@ -2176,6 +2213,7 @@ struct SolveGraphs {
     hipGraphExec_t pro = nullptr;
     hipGraphExec_t it[2] = {nullptr, nullptr};
     hipGraphExec_t pair = nullptr;
+    hipGraphExec_t epi = nullptr;  // mean removal and the result for the host
 };
 
 static void destroy_graphs(SolveGraphs& g) {
@@ -2183,7 +2221,8 @@ static void destroy_graphs(SolveGraphs& g) {
     for (auto& e : g.it)
         if (e) (void)hipGraphExecDestroy(e);
     if (g.pair) (void)hipGraphExecDestroy(g.pair);
-    g.pro = g.it[0] = g.it[1] = g.pair = nullptr;
+    if (g.epi) (void)hipGraphExecDestroy(g.epi);
+    g.pro = g.it[0] = g.it[1] = g.pair = g.epi = nullptr;
 }
 
 template <typename F>
@@ -2273,6 +2312,9 @@ static int get_graphs(SolverState& st, const pf_mg& mg, const CGK<ST>& a, ST* z,
             },
             &g.pair);
     }
+    if (!rc)
+        rc = capture(
+            cs, [&](hipStream_t t) { return enqueue_epilogue(a.st, a.x, a.n, nb, st.flag_d, t); }, &g.epi);
     (void)hipStreamDestroy(cs);
     if (rc) {
         destroy_graphs(g);
@@ -2334,6 +2376,27 @@ static int wait_published(const int* flag, int seq, hipStream_t s, bool* stop) {
     }
 }
 
+// Spin until k_publish_result stored `number` (the stream running dry
+// without it is an error, so the loop cannot spin forever); the result
+// words are read after it (x86 keeps loads in order).
+static int wait_result(const int* words, int number, hipStream_t s) {
+    const volatile int* f = words + HW_RESULT_NO;
+    for (unsigned spin = 0;; ++spin) {
+        if (*f == number) break;
+        if ((spin & 255) == 255) {
+            const hipError_t q = hipStreamQuery(s);
+            if (q == hipSuccess) {
+                if (*f == number) break;
+                return set_error(PF_ERR_INVALID, "pf_mgpcg_solve: the result was never published");
+            }
+            if (q != hipErrorNotReady) return set_error((int)q, "pf_mgpcg_solve: %s", hipGetErrorString(q));
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return PF_OK;
+}
+
 template <typename ST>
 static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int max_iter,
                       int* it_out, double* rel_out, hipStream_t s) {
@@ -2373,16 +2436,12 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
     PF_CHECK_LAUNCH();
     k_cg_init<ST><<<nb, RT, 0, s>>>(a);
     PF_CHECK_LAUNCH();
+    // b == 0 after mean removal (poisson.py:366-368): C_INIT sets done = 2
+    // and rel = 0, x stays k_cg_init's zeros, every later kernel exits on the
+    // flag and the first residual test stops the host loop, so the result
+    // is (0, 0) without a synchronisation here
     k_comb<<<1, RT, 0, s>>>(st, C_INIT, (int)nb, n, nullptr, tol);
     PF_CHECK_LAUNCH();
-    int init_done = 0;
-    PF_CHECK_HIP(hipMemcpyAsync(&init_done, done, sizeof(int), hipMemcpyDeviceToHost, s));
-    PF_CHECK_HIP(hipStreamSynchronize(s));
-    if (init_done == 2) {  // b == 0 after mean removal: x = 0 (poisson.py:366-368)
-        *it_out = 0;
-        *rel_out = 0.0;
-        return PF_OK;
-    }
     // the caller's solver state (graphs, flags), or a temporary one for this
     // call (no graphs)
     SolverState local;
@@ -2443,22 +2502,21 @@ static int solve_impl(const pf_mg& mg, const float* b, float* x, double tol, int
         if (stop) break;
         it += step;
     }
-    // _remove_mean(x) (poisson.py:408)
-    k_part_sum<float><<<nb, RT, 0, s>>>(x, n, st->part);
-    PF_CHECK_LAUNCH();
-    k_comb<<<1, RT, 0, s>>>(st, C_MEAN, (int)nb, n, &st->mu_x, 0.0);
-    PF_CHECK_LAUNCH();
-    k_shift<float, float><<<nb, RT, 0, s>>>(x, x, n, &st->mu_x);
-    PF_CHECK_LAUNCH();
-    struct {
-        int it;
-        double rel;
-    } res;
-    PF_CHECK_HIP(hipMemcpyAsync(&res.it, &st->it, sizeof(int), hipMemcpyDeviceToHost, s));
-    PF_CHECK_HIP(hipMemcpyAsync(&res.rel, &st->rel, sizeof(double), hipMemcpyDeviceToHost, s));
-    PF_CHECK_HIP(hipStreamSynchronize(s));
-    *it_out = res.it;
-    *rel_out = res.rel;
+    // _remove_mean(x) (poisson.py:408) and the result, published under the
+    // solve's number
+    const int number = ++hp->solves;
+    static_cast<volatile int*>(hp->flag)[HW_WANT] = number;
+    if (gr) {
+        PF_CHECK_HIP(hipGraphLaunch(gr->epi, s));
+    } else {
+        rc = enqueue_epilogue(st, x, n, nb, hp->flag_d, s);
+        if (rc) return rc;
+    }
+    rc = wait_result(hp->flag, number, s);
+    if (rc) return rc;
+    const volatile int* w = hp->flag;
+    *it_out = w[HW_IT];
+    *rel_out = __builtin_bit_cast(double, *reinterpret_cast<const volatile unsigned long long*>(w + HW_REL));
     return PF_OK;
 }
 

@@ -260,7 +260,8 @@ int pf_mgpcg_state_destroy(void* state);
  * Writes x (mean removed); *it_out / *rel_out are host pointers.  Blocks:
  * every residual test publishes (tests so far, done) to a pinned host int,
  * which the host polls, while the GPU still has queued work, to stop
- * enqueueing iterations (no events).  The library keeps no state of its own between
+ * enqueueing iterations, and the result is published the same way (no
+ * events, no copies, no stream synchronisation).  The library keeps no state of its own between
  * calls: graphs and flags live in mg->host_state. */
 int pf_mgpcg_solve(const pf_mg* mg, const float* b, float* x, double tol, int32_t max_iter,
                    int32_t* it_out, double* rel_out, pf_stream_t stream);
