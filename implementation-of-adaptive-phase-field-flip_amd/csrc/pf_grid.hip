@@ -141,10 +141,13 @@ __device__ inline float process_face(const PhaseK& a, int i, int j, int k) {
 // its three low faces; the last layer of each axis only has faces.
 __global__ __launch_bounds__(256) void k_phase(PhaseK a) {
     const Dims d = a.d;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ez = d.nz + 1, ey = d.ny + 1;
-    if (t >= (int64_t)(d.nx + 1) * ey * ez) return;
-    const int k = (int)(t % ez), j = (int)((t / ez) % ey), i = (int)(t / ((int64_t)ez * ey));
+    // 32-bit decode (the entry points check that the lattice has fewer
+    // than 2^31 points)
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned ez = d.nz + 1, ey = d.ny + 1;
+    if (t >= (unsigned)(d.nx + 1) * ey * ez) return;
+    const unsigned row = t / ez;
+    const int k = (int)(t - row * ez), i = (int)(row / ey), j = (int)(row - (unsigned)i * ey);
     const bool ci = i < d.nx, cj = j < d.ny, ck = k < d.nz;
     float ulo = 0.f, vlo = 0.f, wlo = 0.f;
     if (cj && ck) ulo = process_face<0>(a, i, j, k);
@@ -206,10 +209,13 @@ __device__ inline void project_face(const ProjK& a, int i, int j, int k) {
 
 __global__ __launch_bounds__(256) void k_project(ProjK a) {
     const Dims d = a.d;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ez = d.nz + 1, ey = d.ny + 1;
-    if (t >= (int64_t)(d.nx + 1) * ey * ez) return;
-    const int k = (int)(t % ez), j = (int)((t / ez) % ey), i = (int)(t / ((int64_t)ez * ey));
+    // 32-bit decode (the entry points check that the lattice has fewer
+    // than 2^31 points)
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned ez = d.nz + 1, ey = d.ny + 1;
+    if (t >= (unsigned)(d.nx + 1) * ey * ez) return;
+    const unsigned row = t / ez;
+    const int k = (int)(t - row * ez), i = (int)(row / ey), j = (int)(row - (unsigned)i * ey);
     const bool ci = i < d.nx, cj = j < d.ny, ck = k < d.nz;
     if (cj && ck) project_face<0>(a, i, j, k);
     if (ci && ck) project_face<1>(a, i, j, k);
@@ -222,10 +228,10 @@ __global__ __launch_bounds__(256) void k_max_div(Dims d, const float* __restrict
                                                  const float* __restrict__ w, float dx, float* out) {
     __shared__ float sh[4];
     float m = 0.0f;
-    const int64_t nc = (int64_t)d.nx * d.ny * d.nz;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        const int k = (int)(c % d.nz), j = (int)((c / d.nz) % d.ny), i = (int)(c / ((int64_t)d.nz * d.ny));
+    const unsigned nc = (unsigned)d.nx * d.ny * d.nz;
+    for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const unsigned row = c / d.nz;
+        const int k = (int)(c - row * d.nz), i = (int)(row / d.ny), j = (int)(row - (unsigned)i * d.ny);
         const float dv = (u[face_index<0>(d, i + 1, j, k)] - u[face_index<0>(d, i, j, k)]) +
                          (v[face_index<1>(d, i, j + 1, k)] - v[face_index<1>(d, i, j, k)]) +
                          (w[face_index<2>(d, i, j, k + 1)] - w[face_index<2>(d, i, j, k)]);
@@ -238,6 +244,7 @@ __global__ __launch_bounds__(256) void k_max_div(Dims d, const float* __restrict
 // ============================================================== extrapolation
 struct ExtrapK {
     Dims d;
+    int nb0, nb1;  // workgroups of the x and y face arrays (decode_face)
     const float* vin[3];
     const uint8_t* kin[3];
     float* vout[3];
@@ -245,74 +252,97 @@ struct ExtrapK {
     const int* is_liq;
 };
 
-// decode a flat index over the three face arrays
-__device__ inline bool decode_face(const Dims& d, int64_t t, int& A, int& i, int& j, int& k, int& f) {
-    const int64_t f0 = face_count(d, 0), f1 = face_count(d, 1), f2 = face_count(d, 2);
-    if (t < f0) {
-        A = 0;
-    } else if (t < f0 + f1) {
-        A = 1;
-        t -= f0;
-    } else if (t < f0 + f1 + f2) {
-        A = 2;
-        t -= f0 + f1;
-    } else {
-        return false;
-    }
-    const int sy = d.ny + (A == 1), sz = d.nz + (A == 2);
-    f = (int)t;
-    k = (int)(t % sz);
-    j = (int)((t / sz) % sy);
-    i = (int)(t / ((int64_t)sz * sy));
+// The face this thread owns: the workgroups cover the three face arrays one
+// after another (nb0 workgroups for the x faces, nb1 for the y faces, the
+// rest for the z faces), so the array is the same in a whole workgroup and
+// its pointers stay scalar.  32-bit indices: the caller checks that the
+// arrays hold fewer than 2^31 faces.
+__device__ inline bool decode_face(const Dims& d, int nb0, int nb1, int& A, int& i, int& j, int& k, int& f) {
+    const int b = blockIdx.x;
+    A = b < nb0 ? 0 : (b < nb0 + nb1 ? 1 : 2);
+    const int t = (b - (A == 0 ? 0 : (A == 1 ? nb0 : nb0 + nb1))) * blockDim.x + threadIdx.x;
+    const unsigned sx = d.nx + (A == 0), sy = d.ny + (A == 1), sz = d.nz + (A == 2);
+    if ((unsigned)t >= sx * sy * sz) return false;
+    const unsigned row = (unsigned)t / sz;
+    f = t;
+    k = (int)((unsigned)t - row * sz);
+    i = (int)(row / sy);
+    j = (int)(row - (unsigned)i * sy);
     return true;
 }
 
 // seed_liquid_velocity, kernels.py:406-425
 __global__ __launch_bounds__(256) void k_extrap_seed(ExtrapK a) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int A, i, j, k, f;
-    if (!decode_face(a.d, t, A, i, j, k, f)) return;
+    if (!decode_face(a.d, a.nb0, a.nb1, A, i, j, k, f)) return;
     const Dims d = a.d;
-    int liq = 0;
-    if (i < d.nx && j < d.ny && k < d.nz) liq = max(liq, a.is_liq[cell_index(d, i, j, k)]);
+    // the two cells' tags and the face value loaded at once (clamped
+    // indices, masked), not the value after the tags
     const int im = i - (A == 0), jm = j - (A == 1), km = k - (A == 2);
-    if (im >= 0 && jm >= 0 && km >= 0) liq = max(liq, a.is_liq[cell_index(d, im, jm, km)]);
+    const bool hi = i < d.nx && j < d.ny && k < d.nz, lo = im >= 0 && jm >= 0 && km >= 0;
+    const int th = a.is_liq[cell_index(d, min(i, d.nx - 1), min(j, d.ny - 1), min(k, d.nz - 1))];
+    const int tl = a.is_liq[cell_index(d, max(im, 0), max(jm, 0), max(km, 0))];
+    const float v = a.vin[A][f];
+    asm volatile("" ::"v"(th), "v"(tl), "v"(v));
+    int liq = 0;
+    if (hi) liq = max(liq, th);
+    if (lo) liq = max(liq, tl);
     a.kout[A][f] = (uint8_t)liq;
-    a.vout[A][f] = liq == 1 ? a.vin[A][f] : 0.0f;
+    a.vout[A][f] = liq == 1 ? v : 0.0f;
 }
 
-// extrapolate_sweep, kernels.py:428-462
+// extrapolate_sweep, kernels.py:428-462.  Every load is issued at once (the
+// face's own value and flag, and its six neighbours' from indices clamped
+// into the array, masked as unknown where the neighbour does not exist):
+// loads that waited for the face's flag and then for the neighbours' flags
+// made three dependent round trips per face.  The known neighbours are
+// summed in the reference's order.
 __global__ __launch_bounds__(256) void k_extrap_sweep(ExtrapK a) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int A, i, j, k, f;
-    if (!decode_face(a.d, t, A, i, j, k, f)) return;
+    if (!decode_face(a.d, a.nb0, a.nb1, A, i, j, k, f)) return;
     const float* vin = a.vin[A];
     const uint8_t* kin = a.kin[A];
-    if (kin[f] == 1) {
-        a.vout[A][f] = vin[f];
-        a.kout[A][f] = 1;
-        return;
-    }
     const Dims d = a.d;
     const int sx = d.nx + (A == 0), sy = d.ny + (A == 1), sz = d.nz + (A == 2);
-    float acc = 0.0f;
-    int cnt = 0;
+    const uint8_t kc = kin[f];
+    const float vc = vin[f];
+    uint8_t nk[6];
+    float nv[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int ax = q / 2, s = (q % 2 == 0) ? -1 : 1;
         const int ii = i + (ax == 0 ? s : 0), jj = j + (ax == 1 ? s : 0), kk = k + (ax == 2 ? s : 0);
-        if (ii < 0 || jj < 0 || kk < 0 || ii >= sx || jj >= sy || kk >= sz) continue;
-        const int g = (ii * sy + jj) * sz + kk;
-        if (kin[g] == 1) {
-            acc += vin[g];
+        const bool in = ii >= 0 && jj >= 0 && kk >= 0 && ii < sx && jj < sy && kk < sz;
+        // (clamped: a missing neighbour reads the face itself, masked out)
+        const int g = (min(max(ii, 0), sx - 1) * sy + min(max(jj, 0), sy - 1)) * sz + min(max(kk, 0), sz - 1);
+        nk[q] = kin[g];
+        nv[q] = vin[g];
+        if (!in) nk[q] = 0;
+    }
+    // an empty asm that takes every loaded value keeps the compiler from
+    // sinking the neighbours' loads into the branches that use them (which
+    // costs one round trip per branch): all 14 loads are in flight together
+#pragma unroll
+    for (int q = 0; q < 6; ++q) asm volatile("" ::"v"((int)nk[q]), "v"(nv[q]));
+    asm volatile("" ::"v"((int)kc), "v"(vc));
+    if (kc == 1) {
+        a.vout[A][f] = vc;
+        a.kout[A][f] = 1;
+        return;
+    }
+    float acc = 0.0f;
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        if (nk[q] == 1) {
+            acc += nv[q];
             cnt += 1;
         }
-    }
     if (cnt > 0) {
         a.vout[A][f] = acc / (float)cnt;
         a.kout[A][f] = 1;
     } else {
-        a.vout[A][f] = vin[f];
+        a.vout[A][f] = vc;
         a.kout[A][f] = 0;
     }
 }
@@ -410,6 +440,7 @@ PF_API int pf_project(const float* p, float* const beta[3], float* const vel[3],
     PF_REQUIRE(vel && vel[0] && vel[1] && vel[2], "pf_project: need vel");
     PF_REQUIRE(dt == 0.0f || (p && beta && beta[0] && beta[1] && beta[2]), "pf_project: need p, beta");
     PF_REQUIRE(dx > 0.f, "pf_project: dx must be positive");
+    PF_REQUIRE((int64_t)(nx + 1) * (ny + 1) * (nz + 1) < (int64_t)1 << 31, "pf_project: grid too large");
     hipStream_t s = S(stream);
     const Dims d{nx, ny, nz};
     ProjK a;
@@ -449,6 +480,7 @@ PF_API int pf_extrapolate(float* const vel[3], const int32_t* is_liquid, float* 
     hipStream_t s = S(stream);
     const Dims d{nx, ny, nz};
     const int64_t n = face_count(d, 0) + face_count(d, 1) + face_count(d, 2);
+    PF_REQUIRE(n < ((int64_t)1 << 31), "pf_extrapolate: grid too large");
     // ping-pong so that the last sweep lands in vel_liq
     float* const* bufs[2] = {vel_liq, tmp};
     uint8_t* const* kb[2] = {known, known2};
@@ -462,7 +494,10 @@ PF_API int pf_extrapolate(float* const vel[3], const int32_t* is_liquid, float* 
         a.vout[q] = bufs[cur][q];
         a.kout[q] = kb[cur][q];
     }
-    k_extrap_seed<<<cdiv(n, 256), 256, 0, s>>>(a);
+    a.nb0 = (int)cdiv(face_count(d, 0), 256);
+    a.nb1 = (int)cdiv(face_count(d, 1), 256);
+    const unsigned nbs = (unsigned)(a.nb0 + a.nb1) + cdiv(face_count(d, 2), 256);
+    k_extrap_seed<<<nbs, 256, 0, s>>>(a);
     PF_CHECK_LAUNCH();
     for (int sw = 0; sw < sweeps; ++sw) {
         const int nxt = 1 - cur;
@@ -472,7 +507,7 @@ PF_API int pf_extrapolate(float* const vel[3], const int32_t* is_liquid, float* 
             a.vout[q] = bufs[nxt][q];
             a.kout[q] = kb[nxt][q];
         }
-        k_extrap_sweep<<<cdiv(n, 256), 256, 0, s>>>(a);
+        k_extrap_sweep<<<nbs, 256, 0, s>>>(a);
         PF_CHECK_LAUNCH();
         cur = nxt;
     }
