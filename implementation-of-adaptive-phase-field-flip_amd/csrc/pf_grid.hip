@@ -222,11 +222,14 @@ __global__ __launch_bounds__(256) void k_project(ProjK a) {
     if (ci && cj) project_face<2>(a, i, j, k);
 }
 
-// max |div u / dx| (kernels.py:390-400); out must start at 0
-__global__ __launch_bounds__(256) void k_max_div(Dims d, const float* __restrict__ u,
-                                                 const float* __restrict__ v,
-                                                 const float* __restrict__ w, float dx, float* out) {
-    __shared__ float sh[4];
+// max |div u / dx| (kernels.py:390-400); out must start at 0.  Few wide
+// workgroups, one atomic each: at 128^3, 2048 same-address atomics from
+// 256-thread workgroups took most of its 27 us.
+constexpr int kMaxDivThreads = 1024;
+__global__ __launch_bounds__(kMaxDivThreads) void k_max_div(Dims d, const float* __restrict__ u,
+                                                            const float* __restrict__ v,
+                                                            const float* __restrict__ w, float dx, float* out) {
+    __shared__ float sh[kMaxDivThreads / 64];
     float m = 0.0f;
     const unsigned nc = (unsigned)d.nx * d.ny * d.nz;
     for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
@@ -464,9 +467,9 @@ PF_API int pf_project(const float* p, float* const beta[3], float* const vel[3],
     if (max_div) {
         int rc = fill_u32(max_div, 1, 0u, s);
         if (rc) return rc;
-        const int64_t nb = cdiv((int64_t)nx * ny * nz, 256 * 4);
-        k_max_div<<<(unsigned)(nb < 4096 ? nb : 4096), 256, 0, s>>>(d, vel[0], vel[1], vel[2], dx,
-                                                                     max_div);
+        const int64_t nb = cdiv((int64_t)nx * ny * nz, kMaxDivThreads * 4);
+        k_max_div<<<(unsigned)(nb < 512 ? nb : 512), kMaxDivThreads, 0, s>>>(d, vel[0], vel[1], vel[2], dx,
+                                                                             max_div);
         PF_CHECK_LAUNCH();
     }
     return PF_OK;
