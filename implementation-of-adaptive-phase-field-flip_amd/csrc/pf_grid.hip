@@ -35,79 +35,69 @@ __device__ inline float phase_field(float rho, const PhaseK& a) {
     return fminf(sqrtf(fmaxf(rho - a.rho_min, 0.0f) / (a.alpha_phi * a.rho_0l)), 1.0f);
 }
 
-__device__ inline int cell_tag(const PhaseK& a, int c) {
-    if (a.flags & PF_PH_TAGS_IN) return a.is_liq[c];
-    return phase_field(a.cmass[c], a) >= 0.5f ? 1 : 0;
+// Liquid tag of a cell: eq. (7) phase >= 1/2 from its mass, or the is_liquid
+// entry with PF_PH_TAGS_IN.
+__device__ inline int cell_tag(const PhaseK& a, int fl, float cmass, int tag) {
+    if (fl & PF_PH_TAGS_IN) return tag;
+    return phase_field(cmass, a) >= 0.5f ? 1 : 0;
 }
 
-// Coordinates of the low / high cell of face (i,j,k) of axis A.
-template <int A>
-__device__ inline void face_cells(const Dims& d, int i, int j, int k, bool& lo_out, bool& hi_out,
-                                  int& clo, int& chi) {
-    const int ia = A == 0 ? i : (A == 1 ? j : k);
-    const int na = A == 0 ? d.nx : (A == 1 ? d.ny : d.nz);
-    lo_out = ia == 0;
-    hi_out = ia == na;
-    clo = lo_out ? -1 : cell_index(d, i - (A == 0), j - (A == 1), k - (A == 2));
-    chi = hi_out ? -1 : cell_index(d, i, j, k);
+// The inputs of one face: mass (with PF_PH_VEL_FROM_MOM, or with
+// PF_PH_PHASE unless VEL_ONLY) and momentum (PF_PH_VEL_FROM_MOM) or velocity.
+struct FaceIn {
+    float m, u;
+};
+
+template <int A, bool VEL_ONLY>
+__device__ inline FaceIn load_face(const PhaseK& a, int fl, int f) {
+    FaceIn r{0.0f, 0.0f};
+    const int need_m = VEL_ONLY ? PF_PH_VEL_FROM_MOM : (PF_PH_VEL_FROM_MOM | PF_PH_PHASE);
+    if (fl & need_m) r.m = a.mass[A][f];
+    r.u = (fl & PF_PH_VEL_FROM_MOM) ? a.mom[A][f] : a.vel[A][f];
+    return r;
 }
 
-// walls (kernels.py:314-326) and solid-touching faces (kernels.py:601-627)
-template <int A>
-__device__ inline bool face_blocked(const PhaseK& a, bool lo_out, bool hi_out, int clo, int chi) {
-    if (lo_out || hi_out) return true;
-    if (a.solid && (a.solid[clo] == 1 || a.solid[chi] == 1)) return true;
-    return false;
-}
-
-// Face velocity after every velocity stage selected by flags, without
-// writing anything (used for the high faces a cell's RHS needs).
-template <int A>
-__device__ inline float final_face_vel(const PhaseK& a, int i, int j, int k) {
-    const int f = face_index<A>(a.d, i, j, k);
-    float v;
-    if (a.flags & PF_PH_VEL_FROM_MOM) {
-        const float m = a.mass[A][f];
-        v = m > a.mass_eps ? a.mom[A][f] / m : 0.0f;
-    } else {
-        v = a.vel[A][f];
+// face_velocity_from_momentum (kernels.py:212-228), or the stored velocity
+__device__ inline float face_vel(const PhaseK& a, int fl, const FaceIn& in, bool& ok) {
+    if (fl & PF_PH_VEL_FROM_MOM) {
+        ok = in.m > a.mass_eps;
+        return ok ? in.u / in.m : 0.0f;
     }
-    if (a.flags & PF_PH_ZERO) {
-        bool lo, hi;
-        int clo, chi;
-        face_cells<A>(a.d, i, j, k, lo, hi, clo, chi);
-        if (face_blocked<A>(a, lo, hi, clo, chi)) return 0.0f;
-    }
-    if ((a.flags & PF_PH_GRAVITY) && a.g[A] != 0.0f) v += a.gdt[A];
+    ok = true;
+    return in.u;
+}
+
+// Velocity of a cell's high face after every velocity stage selected by
+// flags, without writing anything (the cell's RHS needs it; the point that
+// owns the face writes it).  The face's low cell is the cell itself (solid
+// flag s_lo); hi_out: the face is the domain's high wall; sol: solids given.
+template <int A>
+__device__ inline float high_face_vel(const PhaseK& a, int fl, bool sol, const FaceIn& in, bool hi_out,
+                                      int s_lo, int s_hi) {
+    bool ok;
+    float v = face_vel(a, fl, in, ok);
+    // walls (kernels.py:314-326) and solid-touching faces (kernels.py:601-627)
+    if ((fl & PF_PH_ZERO) && (hi_out || (sol && (s_lo == 1 || s_hi == 1)))) return 0.0f;
+    if ((fl & PF_PH_GRAVITY) && a.g[A] != 0.0f) v += a.gdt[A];
     return v;
 }
 
-// Process face (i,j,k) of axis A: all selected stages, with writes.
-// Returns the final velocity (for the owning cell's RHS).
+// Low face f of axis A: all selected stages, with writes; returns the final
+// velocity (for the owning cell's RHS).  lo / hi: the face is the domain's
+// low / high wall; t_lo, t_hi: liquid tags of its two cells (used only when
+// the cell exists); blocked: zeroed by PF_PH_ZERO.
 template <int A>
-__device__ inline float process_face(const PhaseK& a, int i, int j, int k) {
-    const int f = face_index<A>(a.d, i, j, k);
-    const int fl = a.flags;
-    bool lo, hi;
-    int clo, chi;
-    face_cells<A>(a.d, i, j, k, lo, hi, clo, chi);
-    const bool blocked = (fl & PF_PH_ZERO) ? face_blocked<A>(a, lo, hi, clo, chi) : false;
-
-    float v;
-    if (fl & PF_PH_VEL_FROM_MOM) {
-        const float m = a.mass[A][f];
-        const bool ok = m > a.mass_eps;
-        v = ok ? a.mom[A][f] / m : 0.0f;
-        if (a.valid[A]) a.valid[A][f] = ok ? 1 : 0;
-    } else {
-        v = a.vel[A][f];
-    }
+__device__ inline float low_face(const PhaseK& a, int fl, int f, bool lo, bool hi, const FaceIn& in,
+                                 int t_lo, int t_hi, bool blocked) {
+    bool ok;
+    float v = face_vel(a, fl, in, ok);
+    if ((fl & PF_PH_VEL_FROM_MOM) && a.valid[A]) a.valid[A][f] = ok ? 1 : 0;
     bool write_beta = false;
     float beta = 0.0f;
     if (fl & PF_PH_PHASE) {
         // face_beta, kernels.py:278-311
-        int ta = lo ? 0 : cell_tag(a, clo);
-        int tb = hi ? 0 : cell_tag(a, chi);
+        int ta = lo ? 0 : t_lo;
+        int tb = hi ? 0 : t_hi;
         if (lo) ta = tb;
         if (hi) tb = ta;
         float ph;
@@ -116,7 +106,7 @@ __device__ inline float process_face(const PhaseK& a, int i, int j, int k) {
         else if (ta == 0 && tb == 0)
             ph = 0.0f;
         else
-            ph = phase_field(a.mass[A][f], a);
+            ph = phase_field(in.m, a);
         a.phi_face[A][f] = ph;
         const float rho = (1.0f - ph) * a.rho_g + ph * a.rho_l;
         beta = 1.0f / rho;
@@ -138,7 +128,15 @@ __device__ inline float process_face(const PhaseK& a, int i, int j, int k) {
 }
 
 // One thread per point of the (nx+1)(ny+1)(nz+1) lattice: cell (i,j,k) and
-// its three low faces; the last layer of each axis only has faces.
+// its three low faces; the last layer of each axis only has faces.  Every
+// input of a point is loaded first, from clamped indices (each addresses an
+// existing entry and is the true index wherever its value is used), and the
+// values are pinned before the first branch on them: the point's loads are
+// in flight together instead of one round trip per dependent branch.
+// FL >= 0: the flags (and kPhaseSolid for a solid array) fixed at compile
+// time, for the step's combinations; FL < 0: a.flags at run time.
+constexpr int kPhaseSolid = 1 << 8;
+template <int FL>
 __global__ __launch_bounds__(256) void k_phase(PhaseK a) {
     const Dims d = a.d;
     // 32-bit decode (the entry points check that the lattice has fewer
@@ -149,27 +147,98 @@ __global__ __launch_bounds__(256) void k_phase(PhaseK a) {
     const unsigned row = t / ez;
     const int k = (int)(t - row * ez), i = (int)(row / ey), j = (int)(row - (unsigned)i * ey);
     const bool ci = i < d.nx, cj = j < d.ny, ck = k < d.nz;
-    float ulo = 0.f, vlo = 0.f, wlo = 0.f;
-    if (cj && ck) ulo = process_face<0>(a, i, j, k);
-    if (ci && ck) vlo = process_face<1>(a, i, j, k);
-    if (ci && cj) wlo = process_face<2>(a, i, j, k);
-    if (!(ci && cj && ck)) return;
-    const int c = cell_index(d, i, j, k);
-    if (a.flags & PF_PH_PHASE) {
-        // cell_phase, kernels.py:242-255
-        const float ph = phase_field(a.cmass[c], a);
-        a.phi_cell[c] = ph;
-        if (!(a.flags & PF_PH_TAGS_IN)) a.is_liq[c] = ph >= 0.5f ? 1 : 0;
+    const int fl = FL >= 0 ? FL : a.flags;
+    const bool sol = FL >= 0 ? (FL & kPhaseSolid) != 0 : a.solid != nullptr;
+    const bool rhs = fl & PF_PH_RHS;
+    const int ic = min(i, d.nx - 1), jc = min(j, d.ny - 1), kc = min(k, d.nz - 1);
+    const int c = cell_index(d, ic, jc, kc);
+    // low neighbours (the low cells of the low faces), high neighbours (the
+    // high cells of the high faces)
+    const int cn0 = cell_index(d, max(i - 1, 0), jc, kc), cn1 = cell_index(d, ic, max(j - 1, 0), kc),
+              cn2 = cell_index(d, ic, jc, max(k - 1, 0));
+    const int cp0 = cell_index(d, min(i + 1, d.nx - 1), jc, kc),
+              cp1 = cell_index(d, ic, min(j + 1, d.ny - 1), kc),
+              cp2 = cell_index(d, ic, jc, min(k + 1, d.nz - 1));
+    const int f0 = face_index<0>(d, i, jc, kc), f1 = face_index<1>(d, ic, j, kc),
+              f2 = face_index<2>(d, ic, jc, k);
+
+    const FaceIn l0 = load_face<0, false>(a, fl, f0), l1 = load_face<1, false>(a, fl, f1),
+                 l2 = load_face<2, false>(a, fl, f2);
+    float cm = 0.f, cm0 = 0.f, cm1 = 0.f, cm2 = 0.f;
+    int tg = 0, tg0 = 0, tg1 = 0, tg2 = 0;
+    if (fl & PF_PH_PHASE) {
+        cm = a.cmass[c];
+        if (fl & PF_PH_TAGS_IN) {
+            tg = a.is_liq[c];
+            tg0 = a.is_liq[cn0];
+            tg1 = a.is_liq[cn1];
+            tg2 = a.is_liq[cn2];
+        } else {
+            cm0 = a.cmass[cn0];
+            cm1 = a.cmass[cn1];
+            cm2 = a.cmass[cn2];
+        }
     }
-    if (a.flags & PF_PH_RHS) {
+    int sc = 0, sn0 = 0, sn1 = 0, sn2 = 0, sp0 = 0, sp1 = 0, sp2 = 0;
+    if ((fl & PF_PH_ZERO) && sol) {
+        sc = a.solid[c];
+        sn0 = a.solid[cn0];
+        sn1 = a.solid[cn1];
+        sn2 = a.solid[cn2];
+        if (rhs) {
+            sp0 = a.solid[cp0];
+            sp1 = a.solid[cp1];
+            sp2 = a.solid[cp2];
+        }
+    }
+    FaceIn h0{0.f, 0.f}, h1{0.f, 0.f}, h2{0.f, 0.f};
+    float ws = 0.f;
+    if (rhs) {
+        h0 = load_face<0, true>(a, fl, face_index<0>(d, ic + 1, jc, kc));
+        h1 = load_face<1, true>(a, fl, face_index<1>(d, ic, jc + 1, kc));
+        h2 = load_face<2, true>(a, fl, face_index<2>(d, ic, jc, kc + 1));
+        if (a.k_div > 0.0f) ws = a.cwsum[c];
+    }
+    // one statement: the compiler may move loads across separate ones
+    asm volatile("" ::"v"(l0.m), "v"(l0.u), "v"(l1.m), "v"(l1.u), "v"(l2.m), "v"(l2.u), "v"(cm), "v"(cm0),
+                 "v"(cm1), "v"(cm2), "v"(tg), "v"(tg0), "v"(tg1), "v"(tg2), "v"(sc), "v"(sn0), "v"(sn1),
+                 "v"(sn2), "v"(sp0), "v"(sp1), "v"(sp2), "v"(h0.m), "v"(h0.u), "v"(h1.m), "v"(h1.u),
+                 "v"(h2.m), "v"(h2.u), "v"(ws));
+
+    const int t_c = cell_tag(a, fl, cm, tg);
+    const bool zero = fl & PF_PH_ZERO;
+    float ulo = 0.f, vlo = 0.f, wlo = 0.f;
+    if (cj && ck) {
+        const bool lo = i == 0, hi = i == d.nx;
+        ulo = low_face<0>(a, fl, f0, lo, hi, l0, cell_tag(a, fl, cm0, tg0), t_c,
+                          zero && (lo || hi || (sol && (sn0 == 1 || sc == 1))));
+    }
+    if (ci && ck) {
+        const bool lo = j == 0, hi = j == d.ny;
+        vlo = low_face<1>(a, fl, f1, lo, hi, l1, cell_tag(a, fl, cm1, tg1), t_c,
+                          zero && (lo || hi || (sol && (sn1 == 1 || sc == 1))));
+    }
+    if (ci && cj) {
+        const bool lo = k == 0, hi = k == d.nz;
+        wlo = low_face<2>(a, fl, f2, lo, hi, l2, cell_tag(a, fl, cm2, tg2), t_c,
+                          zero && (lo || hi || (sol && (sn2 == 1 || sc == 1))));
+    }
+    if (!(ci && cj && ck)) return;
+    if (fl & PF_PH_PHASE) {
+        // cell_phase, kernels.py:242-255
+        const float ph = phase_field(cm, a);
+        a.phi_cell[c] = ph;
+        if (!(fl & PF_PH_TAGS_IN)) a.is_liq[c] = ph >= 0.5f ? 1 : 0;
+    }
+    if (rhs) {
         // build_rhs, kernels.py:338-363
-        const float uhi = final_face_vel<0>(a, i + 1, j, k);
-        const float vhi = final_face_vel<1>(a, i, j + 1, k);
-        const float whi = final_face_vel<2>(a, i, j, k + 1);
+        const float uhi = high_face_vel<0>(a, fl, sol, h0, i + 1 == d.nx, sc, sp0);
+        const float vhi = high_face_vel<1>(a, fl, sol, h1, j + 1 == d.ny, sc, sp1);
+        const float whi = high_face_vel<2>(a, fl, sol, h2, k + 1 == d.nz, sc, sp2);
         const float du = (uhi - ulo) + (vhi - vlo) + (whi - wlo);
         float r = -(a.dx / a.dt) * du;
         if (a.k_div > 0.0f) {
-            const float excess = (a.cwsum[c] - a.rho_t0) / a.rho_t0;
+            const float excess = (ws - a.rho_t0) / a.rho_t0;
             r -= a.dx * a.dx * a.k_div * fmaxf(excess, 0.0f) / a.dt;
         }
         a.rhs[c] = r;
@@ -354,6 +423,22 @@ __global__ __launch_bounds__(256) void k_extrap_sweep(ExtrapK a) {
 
 using namespace pf;
 
+// The step's flag combinations get kernels with the flags fixed at compile
+// time (no branches between a point's loads); the others read them.
+static void launch_phase(const PhaseK& a, unsigned nb, hipStream_t s) {
+    constexpr int kStep = PF_PH_VEL_FROM_MOM | PF_PH_PHASE | PF_PH_ZERO | PF_PH_STAR | PF_PH_GRAVITY | PF_PH_RHS;
+    const int key = a.flags | (a.solid ? kPhaseSolid : 0);
+    switch (key) {
+        case kStep: k_phase<kStep><<<nb, 256, 0, s>>>(a); break;
+        case kStep | kPhaseSolid: k_phase<kStep | kPhaseSolid><<<nb, 256, 0, s>>>(a); break;
+        case kStep | PF_PH_TAGS_IN: k_phase<kStep | PF_PH_TAGS_IN><<<nb, 256, 0, s>>>(a); break;
+        case kStep | PF_PH_TAGS_IN | kPhaseSolid:
+            k_phase<kStep | PF_PH_TAGS_IN | kPhaseSolid><<<nb, 256, 0, s>>>(a);
+            break;
+        default: k_phase<-1><<<nb, 256, 0, s>>>(a);
+    }
+}
+
 PF_API int pf_phase_beta_forces(const pf_phase_args* args, pf_stream_t stream) {
     PF_REQUIRE(args, "pf_phase_beta_forces: null args");
     const pf_phase_args& in = *args;
@@ -364,6 +449,11 @@ PF_API int pf_phase_beta_forces(const pf_phase_args* args, pf_stream_t stream) {
     PF_REQUIRE(fl != 0 && (fl & ~127) == 0, "pf_phase_beta_forces: bad flags %d", fl);
     PF_REQUIRE(!(fl & PF_PH_TAGS_IN) || ((fl & PF_PH_PHASE) && in.is_liquid),
                "pf_phase_beta_forces: PF_PH_TAGS_IN goes with PF_PH_PHASE and an is_liquid input");
+    // a cell's RHS reads its high faces, which the neighbouring points write
+    // in the same pass when vel is an input that ZERO or GRAVITY modifies
+    PF_REQUIRE(!((fl & PF_PH_RHS) && !(fl & PF_PH_VEL_FROM_MOM) && (fl & (PF_PH_ZERO | PF_PH_GRAVITY))),
+               "pf_phase_beta_forces: PF_PH_RHS with PF_PH_ZERO or PF_PH_GRAVITY needs "
+               "PF_PH_VEL_FROM_MOM (or a separate PF_PH_RHS pass)");
     for (int q = 0; q < 3; ++q) {
         if (fl & PF_PH_VEL_FROM_MOM)
             PF_REQUIRE(in.mass[q] && in.mom[q], "pf_phase_beta_forces: need mass/mom");
@@ -411,7 +501,7 @@ PF_API int pf_phase_beta_forces(const pf_phase_args* args, pf_stream_t stream) {
     a.rho_t0 = in.rho_tilde_0;
     a.k_div = in.k_div;
     const int64_t n = (int64_t)(in.nx + 1) * (in.ny + 1) * (in.nz + 1);
-    k_phase<<<cdiv(n, 256), 256, 0, S(stream)>>>(a);
+    launch_phase(a, cdiv(n, 256), S(stream));
     PF_CHECK_LAUNCH();
     return PF_OK;
 }

@@ -135,7 +135,10 @@ int pf_p2g(const float* px, const float* pv, const float* pmass,
  *                       replayed from another run, e.g. to difference a
  *                       simulation along the branch its adjoint follows);
  *                       phi_cell is still written
- * `solid` may be NULL (no obstacles). */
+ * `solid` may be NULL (no obstacles).  PF_PH_RHS together with PF_PH_ZERO or
+ * PF_PH_GRAVITY needs PF_PH_VEL_FROM_MOM: a cell's RHS reads its high faces,
+ * which the same pass rewrites when vel is the input (PF_ERR_INVALID
+ * otherwise). */
 #define PF_PH_VEL_FROM_MOM 1
 #define PF_PH_PHASE 2
 #define PF_PH_ZERO 4

@@ -57,6 +57,26 @@ def test_error_path_without_gpu():
         setattr(g, f, 16)
     rc = lib.pf_msbg_apply(ctypes.byref(g), None, ctypes.c_void_p(16), ctypes.c_void_p(16), 1, None)
     assert rc == -1 and b"exceed the limit" in lib.pf_last_error()
+    # a RHS read from vel that the same pass modifies (gravity, walls) would
+    # race with the neighbouring points' writes: refused.  The stages alone
+    # pass that check and stop at a later one (a missing rhs / beta), so no
+    # call below reaches a launch.
+    def phase(flags, rhs=16, beta=16):
+        a = L.PhaseArgs()
+        a.nx = a.ny = a.nz = 4
+        a.flags = flags
+        for q in range(3):
+            a.vel[q] = 16
+            a.beta[q] = beta
+        a.rhs = rhs
+        a.dt = 0.01
+        return lib.pf_phase_beta_forces(ctypes.byref(a), None)
+    for bad in (L.PH_RHS | L.PH_GRAVITY, L.PH_RHS | L.PH_ZERO, L.PH_RHS | L.PH_ZERO | L.PH_GRAVITY):
+        assert phase(bad) == -1 and b"needs PF_PH_VEL_FROM_MOM" in lib.pf_last_error()
+    assert phase(L.PH_RHS, rhs=None) == -1 and b"need rhs" in lib.pf_last_error()
+    assert phase(L.PH_GRAVITY | L.PH_ZERO, beta=None) == -1 and b"need beta" in lib.pf_last_error()
+    assert phase(L.PH_VEL_FROM_MOM | L.PH_GRAVITY | L.PH_RHS) == -1
+    assert b"need mass/mom" in lib.pf_last_error()
 
 
 C_LAYOUT = r"""
