@@ -256,26 +256,26 @@ struct ProjK {
     float dx, dt;
 };
 
+// Face f of axis A with its loaded inputs: v (vel), b (beta), ph / pl
+// (pressure of the high / low cell, used only inside the domain), vs
+// (vel_star, with dvel); ia / na: the face's index and count along A.
 template <int A>
-__device__ inline void project_face(const ProjK& a, int i, int j, int k) {
-    const Dims d = a.d;
-    const int f = face_index<A>(d, i, j, k);
-    float v = a.vel[A][f];
-    const int ia = A == 0 ? i : (A == 1 ? j : k);
-    const int na = A == 0 ? d.nx : (A == 1 ? d.ny : d.nz);
+__device__ inline void project_face(const ProjK& a, int f, int ia, int na, float v, float b, float ph, float pl,
+                                    float vs) {
     if (a.dt != 0.0f) {
-        const float b = a.beta[A][f];
         // kernels.py:376-387: skip beta == 0 and the two boundary planes
         if (b != 0.0f && ia > 0 && ia < na) {
-            const float ph = a.p[cell_index(d, i, j, k)];
-            const float pl = a.p[cell_index(d, i - (A == 0), j - (A == 1), k - (A == 2))];
             v -= a.dt * b * (ph - pl) / a.dx;
             a.vel[A][f] = v;
         }
     }
-    if (a.dvel[A]) a.dvel[A][f] = v - a.vel_star[A][f];
+    if (a.dvel[A]) a.dvel[A][f] = v - vs;
 }
 
+// One thread per lattice point: its three low faces.  Every input is loaded
+// first from clamped indices (valid addresses, the true index wherever the
+// value is used) and pinned before the branches, so a point's loads travel
+// together.
 __global__ __launch_bounds__(256) void k_project(ProjK a) {
     const Dims d = a.d;
     // 32-bit decode (the entry points check that the lattice has fewer
@@ -286,9 +286,29 @@ __global__ __launch_bounds__(256) void k_project(ProjK a) {
     const unsigned row = t / ez;
     const int k = (int)(t - row * ez), i = (int)(row / ey), j = (int)(row - (unsigned)i * ey);
     const bool ci = i < d.nx, cj = j < d.ny, ck = k < d.nz;
-    if (cj && ck) project_face<0>(a, i, j, k);
-    if (ci && ck) project_face<1>(a, i, j, k);
-    if (ci && cj) project_face<2>(a, i, j, k);
+    const int ic = min(i, d.nx - 1), jc = min(j, d.ny - 1), kc = min(k, d.nz - 1);
+    const int f0 = face_index<0>(d, i, jc, kc), f1 = face_index<1>(d, ic, j, kc),
+              f2 = face_index<2>(d, ic, jc, k);
+    const float v0 = a.vel[0][f0], v1 = a.vel[1][f1], v2 = a.vel[2][f2];
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f, pc = 0.f, p0 = 0.f, p1 = 0.f, p2 = 0.f;
+    if (a.dt != 0.0f) {
+        b0 = a.beta[0][f0];
+        b1 = a.beta[1][f1];
+        b2 = a.beta[2][f2];
+        pc = a.p[cell_index(d, ic, jc, kc)];
+        p0 = a.p[cell_index(d, max(i - 1, 0), jc, kc)];
+        p1 = a.p[cell_index(d, ic, max(j - 1, 0), kc)];
+        p2 = a.p[cell_index(d, ic, jc, max(k - 1, 0))];
+    }
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    if (a.dvel[0]) s0 = a.vel_star[0][f0];
+    if (a.dvel[1]) s1 = a.vel_star[1][f1];
+    if (a.dvel[2]) s2 = a.vel_star[2][f2];
+    asm volatile("" ::"v"(v0), "v"(v1), "v"(v2), "v"(b0), "v"(b1), "v"(b2), "v"(pc), "v"(p0), "v"(p1), "v"(p2),
+                 "v"(s0), "v"(s1), "v"(s2));
+    if (cj && ck) project_face<0>(a, f0, i, d.nx, v0, b0, pc, p0, s0);
+    if (ci && ck) project_face<1>(a, f1, j, d.ny, v1, b1, pc, p1, s1);
+    if (ci && cj) project_face<2>(a, f2, k, d.nz, v2, b2, pc, p2, s2);
 }
 
 // max |div u / dx| (kernels.py:390-400); out must start at 0.  Few wide

@@ -508,60 +508,94 @@ __global__ __launch_bounds__(256) void k_max_count(const int* __restrict__ start
     if (threadIdx.x == 0) atomicMax(cmax, max(max(sh[0], sh[1]), max(sh[2], sh[3])));
 }
 
-// Tiles whose region contains index i of an axis: normal (face-normal) axes
-// span [tT, tT+T], tangential / cell axes [tT-1, tT+T].  Returns the count
-// (<= 2) in ascending tile order with the region-local index of i.
-__device__ inline int tiles_normal(int i, int nt, int (&t)[2], int (&l)[2]) {
-    int c = 0;
-    for (int tt = i / T - 1; tt <= i / T; ++tt) {
-        const int li = i - tt * T;
-        if (tt >= 0 && tt < nt && li >= 0 && li <= T) {
-            t[c] = tt;
-            l[c] = li;
-            ++c;
-        }
-    }
-    return c;
-}
-__device__ inline int tiles_tangential(int i, int nt, int (&t)[2], int (&l)[2]) {
-    int c = 0;
-    const int a0 = (i + 1) / T;
-    for (int tt = a0 - 1; tt <= a0; ++tt) {
-        const int li = i - tt * T + 1;
-        if (tt >= 0 && tt < nt && li >= 0 && li <= T + 1) {
-            t[c] = tt;
-            l[c] = li;
-            ++c;
-        }
-    }
-    return c;
+// Tiles whose region holds index i of an axis.  A normal (face-normal) axis
+// of tile t spans [tT, tT+T], a tangential / cell axis [tT-1, tT+T], so an
+// index lies in one or two tiles.  Candidate 0 always exists; candidate 1
+// exists when `two`, and otherwise repeats candidate 0 (an address that is
+// valid to read; its value is masked out of the sums).
+struct AxisTiles {
+    int t0, l0, t1, l1;  // tile index and region-local index
+    bool two;
+};
+
+__device__ inline AxisTiles axis_normal(int i, int nt) {
+    const int q = i / T, r = i - q * T;
+    AxisTiles a;
+    a.t0 = q < nt ? q : q - 1;  // q == nt only for the last face, r == 0
+    a.l0 = q < nt ? r : r + T;
+    a.two = r == 0 && q >= 1 && q < nt;
+    a.t1 = a.two ? q - 1 : a.t0;
+    a.l1 = a.two ? T : a.l0;
+    return a;
 }
 
-template <int A>
-__device__ inline void gather_face(const TileGeom& g, const unsigned long long* __restrict__ stage,
-                                   int R, const FixScale& fs, int i, int j, int k, float* mass,
-                                   float* mom) {
-    int tX[2], lX[2], tY[2], lY[2], tZ[2], lZ[2];
-    const int cX = A == 0 ? tiles_normal(i, g.ntx, tX, lX) : tiles_tangential(i, g.ntx, tX, lX);
-    const int cY = A == 1 ? tiles_normal(j, g.nty, tY, lY) : tiles_tangential(j, g.nty, tY, lY);
-    const int cZ = A == 2 ? tiles_normal(k, g.ntz, tZ, lZ) : tiles_tangential(k, g.ntz, tZ, lZ);
-    constexpr int SX = A == 0 ? E * E : (A == 1 ? F * E : E * F);
-    constexpr int SY = A == 2 ? F : E;
-    constexpr int OM = A == 0 ? O_XM : (A == 1 ? O_YM : O_ZM);
-    constexpr int OU = A == 0 ? O_XU : (A == 1 ? O_YU : O_ZU);
-    unsigned long long am = 0ull, au = 0ull;
-    for (int a = 0; a < cX; ++a)
-        for (int b = 0; b < cY; ++b)
-            for (int c = 0; c < cZ; ++c) {
-                const int64_t tile = ((int64_t)tX[a] * g.nty + tY[b]) * g.ntz + tZ[c];
-                const int64_t off = tile * R + lX[a] * SX + lY[b] * SY + lZ[c];
-                am += stage[off + OM];
-                au += stage[off + OU];
+__device__ inline AxisTiles axis_tangential(int i, int nt) {
+    const int s = i + 1, q = s / T, r = s - q * T;
+    AxisTiles a;
+    a.t0 = q < nt ? q : q - 1;  // q == nt only for the last cell, r == 0
+    a.l0 = q < nt ? r : r + T;
+    a.two = r <= 1 && q >= 1 && q < nt;
+    a.t1 = a.two ? q - 1 : a.t0;
+    a.l1 = a.two ? r + T : a.l0;
+    return a;
+}
+
+// Sum of one quantity pair (offsets OM, OU in the region; U only if has_u)
+// over the <= 8 tile regions that hold one output, as exact 64-bit integers
+// (the order of the adds does not matter).  Every load is issued before the
+// first add: the pair along Z is read unconditionally (candidate 1 repeats
+// candidate 0 when there is one tile), the extra X and Y tiles behind
+// branches that are mostly uniform over a wave, and the values are pinned
+// before the sums so that no load waits for another.
+template <int SX, int SY, int OM, int OU>
+__device__ inline void gather8(const unsigned long long* __restrict__ stage, unsigned R, const TileGeom& g,
+                               const AxisTiles& X, const AxisTiles& Y, const AxisTiles& Z, bool has_u,
+                               unsigned long long& am, unsigned long long& au) {
+    auto off = [&](int tx, int lx, int ty, int ly) {
+        return ((unsigned)(tx * g.nty + ty) * (unsigned)g.ntz) * R + (unsigned)(lx * SX + ly * SY);
+    };
+    const unsigned z0 = (unsigned)Z.t0 * R + (unsigned)Z.l0, z1 = (unsigned)Z.t1 * R + (unsigned)Z.l1;
+    unsigned long long m[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+    unsigned long long u[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+    const unsigned b00 = off(X.t0, X.l0, Y.t0, Y.l0);
+    m[0] = stage[b00 + z0 + OM];
+    m[1] = stage[b00 + z1 + OM];
+    if (has_u) {
+        u[0] = stage[b00 + z0 + OU];
+        u[1] = stage[b00 + z1 + OU];
+    }
+    if (Y.two) {
+        const unsigned b = off(X.t0, X.l0, Y.t1, Y.l1);
+        m[2] = stage[b + z0 + OM];
+        m[3] = stage[b + z1 + OM];
+        if (has_u) {
+            u[2] = stage[b + z0 + OU];
+            u[3] = stage[b + z1 + OU];
+        }
+    }
+    if (X.two) {
+        const unsigned b = off(X.t1, X.l1, Y.t0, Y.l0);
+        m[4] = stage[b + z0 + OM];
+        m[5] = stage[b + z1 + OM];
+        if (has_u) {
+            u[4] = stage[b + z0 + OU];
+            u[5] = stage[b + z1 + OU];
+        }
+        if (Y.two) {
+            const unsigned b2 = off(X.t1, X.l1, Y.t1, Y.l1);
+            m[6] = stage[b2 + z0 + OM];
+            m[7] = stage[b2 + z1 + OM];
+            if (has_u) {
+                u[6] = stage[b2 + z0 + OU];
+                u[7] = stage[b2 + z1 + OU];
             }
-    const Dims d{g.nx, g.ny, g.nz};
-    const int f = face_index<A>(d, i, j, k);
-    mass[f] = (float)((double)(long long)am * fs.im);
-    mom[f] = (float)((double)(long long)au * fs.iu);
+        }
+    }
+    asm volatile("" ::"v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]),
+                 "v"(u[0]), "v"(u[1]), "v"(u[2]), "v"(u[3]), "v"(u[4]), "v"(u[5]), "v"(u[6]), "v"(u[7]));
+    const unsigned long long zm = Z.two ? ~0ull : 0ull;
+    am = m[0] + (m[1] & zm) + m[2] + (m[3] & zm) + m[4] + (m[5] & zm) + m[6] + (m[7] & zm);
+    au = u[0] + (u[1] & zm) + u[2] + (u[3] & zm) + u[4] + (u[5] & zm) + u[6] + (u[7] & zm);
 }
 
 struct GatherArgs {
@@ -576,34 +610,49 @@ struct GatherArgs {
     float* cwsum;
 };
 
+// One thread per point of the (nx+1)(ny+1)(nz+1) lattice: the three low
+// faces and the cell of the point, each the sum of the tile regions that
+// hold it.  32-bit index arithmetic: the entry point checks that the lattice
+// and the staging buffer have fewer than 2^31 entries.
 __global__ __launch_bounds__(256) void k_p2g_gather(GatherArgs a) {
     const TileGeom g = a.g;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int ez = g.nz + 1, ey = g.ny + 1;
-    if (t >= (int64_t)(g.nx + 1) * ey * ez) return;
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned ez = g.nz + 1, ey = g.ny + 1;
+    if (t >= (unsigned)(g.nx + 1) * ey * ez) return;
     const FixScale fs = fix_scale(a.bounds, a.cmax);
-    const int k = (int)(t % ez), j = (int)((t / ez) % ey), i = (int)(t / ((int64_t)ez * ey));
+    const unsigned row = t / ez;
+    const int k = (int)(t - row * ez), i = (int)(row / ey), j = (int)(row - (unsigned)i * ey);
     const bool ci = i < g.nx, cj = j < g.ny, ck = k < g.nz;
-    if (cj && ck) gather_face<0>(g, a.stage, a.R, fs, i, j, k, a.mass[0], a.mom[0]);
-    if (ci && ck) gather_face<1>(g, a.stage, a.R, fs, i, j, k, a.mass[1], a.mom[1]);
-    if (ci && cj) gather_face<2>(g, a.stage, a.R, fs, i, j, k, a.mass[2], a.mom[2]);
+    const unsigned R = (unsigned)a.R;
+    // the tangential lists of an index past the last cell are never used
+    const AxisTiles nX = axis_normal(i, g.ntx), nY = axis_normal(j, g.nty), nZ = axis_normal(k, g.ntz);
+    const AxisTiles tX = axis_tangential(min(i, g.nx - 1), g.ntx), tY = axis_tangential(min(j, g.ny - 1), g.nty),
+                    tZ = axis_tangential(min(k, g.nz - 1), g.ntz);
+    const Dims d{g.nx, g.ny, g.nz};
+    unsigned long long am, au;
+    if (cj && ck) {
+        gather8<E * E, E, O_XM, O_XU>(a.stage, R, g, nX, tY, tZ, true, am, au);
+        const int f = face_index<0>(d, i, j, k);
+        a.mass[0][f] = (float)((double)(long long)am * fs.im);
+        a.mom[0][f] = (float)((double)(long long)au * fs.iu);
+    }
+    if (ci && ck) {
+        gather8<F * E, E, O_YM, O_YU>(a.stage, R, g, tX, nY, tZ, true, am, au);
+        const int f = face_index<1>(d, i, j, k);
+        a.mass[1][f] = (float)((double)(long long)am * fs.im);
+        a.mom[1][f] = (float)((double)(long long)au * fs.iu);
+    }
+    if (ci && cj) {
+        gather8<E * F, F, O_ZM, O_ZU>(a.stage, R, g, tX, tY, nZ, true, am, au);
+        const int f = face_index<2>(d, i, j, k);
+        a.mass[2][f] = (float)((double)(long long)am * fs.im);
+        a.mom[2][f] = (float)((double)(long long)au * fs.iu);
+    }
     if (ci && cj && ck) {
-        int tX[2], lX[2], tY[2], lY[2], tZ[2], lZ[2];
-        const int cX = tiles_tangential(i, g.ntx, tX, lX);
-        const int cY = tiles_tangential(j, g.nty, tY, lY);
-        const int cZ = tiles_tangential(k, g.ntz, tZ, lZ);
-        unsigned long long am = 0ull, aw = 0ull;
-        for (int x = 0; x < cX; ++x)
-            for (int y = 0; y < cY; ++y)
-                for (int z = 0; z < cZ; ++z) {
-                    const int64_t tile = ((int64_t)tX[x] * g.nty + tY[y]) * g.ntz + tZ[z];
-                    const int64_t off = tile * a.R + (lX[x] * E + lY[y]) * E + lZ[z];
-                    am += a.stage[off + O_CM];
-                    if (a.cwsum) aw += a.stage[off + O_CW];
-                }
-        const int c = (i * g.ny + j) * g.nz + k;
+        gather8<E * E, E, O_CM, O_CW>(a.stage, R, g, tX, tY, tZ, a.cwsum != nullptr, am, au);
+        const int c = cell_index(d, i, j, k);
         a.cmass[c] = (float)((double)(long long)am * fs.im);
-        if (a.cwsum) a.cwsum[c] = (float)((double)(long long)aw * fs.iw);
+        if (a.cwsum) a.cwsum[c] = (float)((double)(long long)au * fs.iw);
     }
 }
 
@@ -1331,6 +1380,8 @@ PF_API int pf_p2g(const float* px, const float* pv, const float* pmass, const ui
     PF_REQUIRE(stage_bytes >= pf_p2g_stage_bytes(nx, ny, nz, cell_wsum != nullptr),
                "pf_p2g: stage buffer too small (%zu < %zu bytes)", stage_bytes,
                pf_p2g_stage_bytes(nx, ny, nz, cell_wsum != nullptr));
+    if (n_tiles(tile_geom(nx, ny, nz)) * REG_W >= (int64_t)1 << 31)
+        return set_error(PF_ERR_SIZE, "pf_p2g: grid too large (staging offsets must fit in 31 bits)");
     hipStream_t s = S(stream);
     P2GArgs a;
     a.px = px;
