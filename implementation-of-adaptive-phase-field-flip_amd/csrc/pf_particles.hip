@@ -57,9 +57,14 @@ __global__ __launch_bounds__(256) void k_bin(const float* __restrict__ px,
     bool active = false;
     if (valid) {
         const uint32_t p = order ? order[idx] : (uint32_t)idx;
-        if (pstate[p] == 0) {
+        // the state and the position in one round trip (the position is read
+        // whatever the state)
+        const int st = pstate[p];
+        const float x = px[3 * (int64_t)p], y = px[3 * (int64_t)p + 1], z = px[3 * (int64_t)p + 2];
+        asm volatile("" ::"v"(st), "v"(x), "v"(y), "v"(z));
+        if (st == 0) {
             active = true;
-            bucket = cell_key(px[3 * (int64_t)p], px[3 * (int64_t)p + 1], px[3 * (int64_t)p + 2], dx, g);
+            bucket = cell_key(x, y, z, dx, g);
         } else {
             bucket = nkeys;
         }
