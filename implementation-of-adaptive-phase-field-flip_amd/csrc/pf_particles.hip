@@ -1035,23 +1035,41 @@ __device__ inline void sample3(const float* fb, const float* const* fg, float x,
     }
 }
 
+// One face array's part of the box: the (clamped) grid index of box entry
+// e, e in [0, GB3).
 template <int AX>
-__device__ inline void stage_face(float* __restrict__ dst, const float* __restrict__ src, const Dims& d, int ox, int oy,
-                                  int oz) {
+__device__ inline int64_t box_src_index(int e, const Dims& d, int ox, int oy, int oz) {
     const int n0 = d.nx + (AX == 0), n1 = d.ny + (AX == 1), n2 = d.nz + (AX == 2);
-    for (int e = threadIdx.x; e < GB3; e += kG2PThreads) {
-        const int li = e / GB2, lj = (e / GB) % GB, lk = e % GB;
-        const int gi = min(max(ox + li, 0), n0 - 1);
-        const int gj = min(max(oy + lj, 0), n1 - 1);
-        const int gk = min(max(oz + lk, 0), n2 - 1);
-        dst[e] = src[((int64_t)gi * n1 + gj) * n2 + gk];
-    }
+    const int li = e / GB2, lj = (e / GB) % GB, lk = e % GB;
+    const int gi = min(max(ox + li, 0), n0 - 1);
+    const int gj = min(max(oy + lj, 0), n1 - 1);
+    const int gk = min(max(oz + lk, 0), n2 - 1);
+    return ((int64_t)gi * n1 + gj) * n2 + gk;
 }
 
+// Stage the three face arrays of one field into the box: every load of a
+// thread is issued before the first LDS store, so the staging costs one
+// round trip to memory instead of one per element.
+constexpr int kStageIters = (GB3 + kG2PThreads - 1) / kG2PThreads;
 __device__ inline void stage_field(float* dst, const float* const* src, const Dims& d, int ox, int oy, int oz) {
-    stage_face<0>(dst, src[0], d, ox, oy, oz);
-    stage_face<1>(dst + GB3, src[1], d, ox, oy, oz);
-    stage_face<2>(dst + 2 * GB3, src[2], d, ox, oy, oz);
+    float v[3][kStageIters];
+#pragma unroll
+    for (int q = 0; q < kStageIters; ++q) {
+        // the last pass's idle threads read entry GB3-1 again (not stored)
+        const int e = min((int)threadIdx.x + q * kG2PThreads, GB3 - 1);
+        v[0][q] = src[0][box_src_index<0>(e, d, ox, oy, oz)];
+        v[1][q] = src[1][box_src_index<1>(e, d, ox, oy, oz)];
+        v[2][q] = src[2][box_src_index<2>(e, d, ox, oy, oz)];
+    }
+#pragma unroll
+    for (int q = 0; q < kStageIters; ++q) {
+        const int e = (int)threadIdx.x + q * kG2PThreads;
+        if (e < GB3) {
+            dst[e] = v[0][q];
+            dst[GB3 + e] = v[1][q];
+            dst[2 * GB3 + e] = v[2][q];
+        }
+    }
 }
 
 // bounds (optional, with pmass): the CFL speed and P2G bounds of the
