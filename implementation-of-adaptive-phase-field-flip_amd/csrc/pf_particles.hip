@@ -1047,27 +1047,44 @@ __device__ inline int64_t box_src_index(int e, const Dims& d, int ox, int oy, in
     return ((int64_t)gi * n1 + gj) * n2 + gk;
 }
 
-// Stage the three face arrays of one field into the box: every load of a
-// thread is issued before the first LDS store, so the staging costs one
-// round trip to memory instead of one per element.
+// Stage the fields of the box (vel; dvel with g2p; vel_liq with adv): every
+// load of a thread is issued before the first LDS store, so the staging
+// costs one round trip to memory instead of one per element.
 constexpr int kStageIters = (GB3 + kG2PThreads - 1) / kG2PThreads;
-__device__ inline void stage_field(float* dst, const float* const* src, const Dims& d, int ox, int oy, int oz) {
-    float v[3][kStageIters];
+__device__ inline void stage_fields(float* box, const G2PArgs& a, bool g2p, bool adv, int ox, int oy, int oz) {
+    float v[9][kStageIters];
 #pragma unroll
     for (int q = 0; q < kStageIters; ++q) {
         // the last pass's idle threads read entry GB3-1 again (not stored)
         const int e = min((int)threadIdx.x + q * kG2PThreads, GB3 - 1);
-        v[0][q] = src[0][box_src_index<0>(e, d, ox, oy, oz)];
-        v[1][q] = src[1][box_src_index<1>(e, d, ox, oy, oz)];
-        v[2][q] = src[2][box_src_index<2>(e, d, ox, oy, oz)];
+        const int64_t i0 = box_src_index<0>(e, a.d, ox, oy, oz), i1 = box_src_index<1>(e, a.d, ox, oy, oz),
+                      i2 = box_src_index<2>(e, a.d, ox, oy, oz);
+        v[0][q] = a.vel[0][i0];
+        v[1][q] = a.vel[1][i1];
+        v[2][q] = a.vel[2][i2];
+        if (g2p) {
+            v[3][q] = a.dvel[0][i0];
+            v[4][q] = a.dvel[1][i1];
+            v[5][q] = a.dvel[2][i2];
+        }
+        if (adv) {
+            v[6][q] = a.vel_liq[0][i0];
+            v[7][q] = a.vel_liq[1][i1];
+            v[8][q] = a.vel_liq[2][i2];
+        }
     }
 #pragma unroll
     for (int q = 0; q < kStageIters; ++q) {
         const int e = (int)threadIdx.x + q * kG2PThreads;
         if (e < GB3) {
-            dst[e] = v[0][q];
-            dst[GB3 + e] = v[1][q];
-            dst[2 * GB3 + e] = v[2][q];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) box[f * GB3 + e] = v[f][q];
+            if (g2p)
+#pragma unroll
+                for (int f = 3; f < 6; ++f) box[f * GB3 + e] = v[f][q];
+            if (adv)
+#pragma unroll
+                for (int f = 6; f < 9; ++f) box[f * GB3 + e] = v[f][q];
         }
     }
 }
@@ -1088,9 +1105,7 @@ __global__ __launch_bounds__(kG2PThreads) void k_g2p_tiled(G2PArgs a, const int*
     if (s0 == s1) return;
     const bool g2p = a.flags & PF_GA_G2P, adv = a.flags & PF_GA_ADVECT;
     SpeedBounds sb;
-    stage_field(box, a.vel, a.d, ox, oy, oz);
-    if (g2p) stage_field(box + 3 * GB3, a.dvel, a.d, ox, oy, oz);
-    if (adv) stage_field(box + 6 * GB3, a.vel_liq, a.d, ox, oy, oz);
+    stage_fields(box, a, g2p, adv, ox, oy, oz);
     __syncthreads();
 
     // the next particle's record is loaded before the current one is
