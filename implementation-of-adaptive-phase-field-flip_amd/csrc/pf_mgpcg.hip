@@ -1504,6 +1504,41 @@ __global__ __launch_bounds__(WT) void k_cg_apply(CGK<ZT> a) {
     __shared__ double sh[5 * 16];
     __shared__ float s_cm, s_be;
     CGState* st = a.st;
+    const Dims d = a.d;
+    // 32-bit indices (the level's face arrays hold fewer than 2^31
+    // entries): the thread's first cell and the grid stride are decoded into
+    // (i, j, k) once, and the loop adds the stride with carries; the faces
+    // and the clamped neighbours are offsets from the cell index
+    // (face_index and cell_index written out).  64-bit division per cell
+    // made this loop several times longer.
+    const int n = (int)a.n, sjk = d.ny * d.nz;
+    const int c0 = blockIdx.x * WT + threadIdx.x, S = gridDim.x * WT;
+    int k = c0 % d.nz, j = (c0 / d.nz) % d.ny, i = c0 / sjk;
+    const int sk = S % d.nz, sj = (S / d.nz) % d.ny, si = S / sjk;
+    // a cell's inputs: faces, then z (as FP32, unscaled) and q_old at the
+    // cell and its six clamped neighbours (-x, +x, -y, +y, -z, +z)
+    float f[6], zr[7], qo[7];
+    auto load_cell = [&](int c) {
+        const int fy = c + i * d.nz, fz = c + i * d.ny + j;
+        f[0] = a.bx[c];
+        f[1] = a.bx[c + sjk];
+        f[2] = a.by[fy];
+        f[3] = a.by[fy + d.nz];
+        f[4] = a.bz[fz];
+        f[5] = a.bz[fz + 1];
+        const int nb[7] = {c, i > 0 ? c - sjk : c, i < d.nx - 1 ? c + sjk : c, j > 0 ? c - d.nz : c,
+                           j < d.ny - 1 ? c + d.nz : c, k > 0 ? c - 1 : c, k < d.nz - 1 ? c + 1 : c};
+#pragma unroll
+        for (int t = 0; t < 7; ++t) {
+            zr[t] = to_f32(a.z[nb[t]]);
+            qo[t] = a.q_old[nb[t]];
+        }
+    };
+    // the first cell's loads are issued before the flag test and the
+    // combine below, so their round trip overlaps that work (a finished
+    // solve's queued launch returns with them in flight)
+    if (c0 < n) load_cell(c0);
+    const float zs = a.zscale ? *a.zscale : 1.0f;
     if (st->done) return;
     if (st->beta_parts) {
         // z' = z - mean(z); rz_new = <r, z'>; num = rz_new - <r_prev, z'> =
@@ -1532,30 +1567,16 @@ __global__ __launch_bounds__(WT) void k_cg_apply(CGK<ZT> a) {
     __syncthreads();
     const float cm = s_cm;
     const float be = s_be;
-    const float zs = a.zscale ? *a.zscale : 1.0f;
     double v[1] = {0.0};
-    const Dims d = a.d;
-    auto qn = [&](int c) {
-        const float zz = zval(a, zs, c) + cm;
-        return zz + be * a.q_old[c];
+    // q at input point t: (z - mu) + beta q_old
+    auto qv = [&](int t) {
+        const float zz = zr[t] * zs + cm;
+        return zz + be * qo[t];
     };
-    // 32-bit indices (the level's face arrays hold fewer than 2^31
-    // entries): the thread's first cell and the grid stride are decoded into
-    // (i, j, k) once, and the loop adds the stride with carries; the faces
-    // and the clamped neighbours are offsets from the cell index
-    // (face_index and cell_index written out).  64-bit division per cell
-    // made this loop several times longer.
-    const int n = (int)a.n, sjk = d.ny * d.nz;
-    const int c0 = blockIdx.x * WT + threadIdx.x, S = gridDim.x * WT;
-    int k = c0 % d.nz, j = (c0 / d.nz) % d.ny, i = c0 / sjk;
-    const int sk = S % d.nz, sj = (S / d.nz) % d.ny, si = S / sjk;
     for (int c = c0; c < n; c += S) {
-        const int fy = c + i * d.nz, fz = c + i * d.ny + j;
-        const float f[6] = {a.bx[c], a.bx[c + sjk], a.by[fy], a.by[fy + d.nz], a.bz[fz], a.bz[fz + 1]};
-        const float pc = qn(c);
-        const float pn[6] = {qn(i > 0 ? c - sjk : c),  qn(i < d.nx - 1 ? c + sjk : c),
-                             qn(j > 0 ? c - d.nz : c), qn(j < d.ny - 1 ? c + d.nz : c),
-                             qn(k > 0 ? c - 1 : c),    qn(k < d.nz - 1 ? c + 1 : c)};
+        if (c != c0) load_cell(c);
+        const float pc = qv(0);
+        const float pn[6] = {qv(1), qv(2), qv(3), qv(4), qv(5), qv(6)};
         float dg, Ap;
         stencil_vals(pc, f, pn, dg, Ap);
         a.q[c] = pc;
@@ -1589,6 +1610,16 @@ __global__ __launch_bounds__(RT) void k_cg_update(CGK<ZT> a) {
     __shared__ float s_al;
     __shared__ int s_ok;
     CGState* st = a.st;
+    // the first element's inputs are loaded before the flag test and the
+    // combine, so their round trip overlaps that work
+    const int64_t i0 = (int64_t)blockIdx.x * RT + threadIdx.x, stride = (int64_t)gridDim.x * RT;
+    float x0 = 0.f, q0 = 0.f, r0 = 0.f, ap0 = 0.f;
+    if (i0 < a.n) {
+        x0 = a.x[i0];
+        q0 = a.q[i0];
+        r0 = a.r[i0];
+        ap0 = a.Ap[i0];
+    }
     if (st->done) return;
     double t[1];
     combine_sums<1>(st->part + 5 * MAXB, a.np_apply, t, shd);
@@ -1611,9 +1642,15 @@ __global__ __launch_bounds__(RT) void k_cg_update(CGK<ZT> a) {
     if (!s_ok) return;
     const float al = s_al, nal = -s_al;
     float m = 0.0f;
-    for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * RT) {
-        a.x[i] = a.x[i] + al * a.q[i];
-        const float rn = a.r[i] + nal * a.Ap[i];
+    for (int64_t i = i0; i < a.n; i += stride) {
+        if (i != i0) {
+            x0 = a.x[i];
+            q0 = a.q[i];
+            r0 = a.r[i];
+            ap0 = a.Ap[i];
+        }
+        a.x[i] = x0 + al * q0;
+        const float rn = r0 + nal * ap0;
         a.r[i] = rn;
         m = fmaxf(m, fabsf(rn));
     }
