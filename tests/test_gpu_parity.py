@@ -256,10 +256,18 @@ def test_phase_beta_forces_match_oracle(pf):
     assert rel_err(np_(sim.rhs), rhs) < 1e-5
 
 
-def test_fused_phase_equals_stages(pf):
-    """The fused step pass equals the stage-by-stage sequence bit for bit."""
-    say("fused phase == stages")
+@pytest.mark.parametrize("solids", [False, True])
+def test_fused_phase_equals_stages(pf, solids):
+    """The fused step pass equals the stage-by-stage sequence bit for bit,
+    with and without a solid mask (each combination runs its own kernel
+    instance, the stages the run-time-flag one), and with the liquid tags
+    handed in (PF_PH_TAGS_IN) it equals the pass that derived them."""
+    say(f"fused phase == stages solids={solids}")
     cfg, sim = make(pf, 16, vel_noise=0.3, gravity=(0.0, 0.0, -9.81))
+    if solids:
+        m = np.zeros((16, 16, 16), bool)
+        m[6:10, 5:11, 2:9] = True
+        sim.set_solids(m)
     sim._p2g()
     sim._phase_field_and_coefficients()
     for ax in range(3):
@@ -268,15 +276,24 @@ def test_fused_phase_equals_stages(pf):
     sim._add_forces(dt)
     sim._phase(32, dt)
     ref = {k: [np_(t).copy() for t in getattr(sim, k)] for k in ("vel", "vel_star", "beta", "phi_face")}
-    ref_rhs = np_(sim.rhs).copy()
-    for t in sim.vel + sim.vel_star + sim.beta + sim.phi_face:
-        t.fill_(7.0)
-    sim._phase(1 | 2 | 4 | 8 | 16 | 32, dt)
-    torch.cuda.synchronize()
-    for k, arrs in ref.items():
-        for ax in range(3):
-            assert np.array_equal(np_(getattr(sim, k)[ax]), arrs[ax]), (k, ax)
-    assert np.array_equal(np_(sim.rhs), ref_rhs)
+    ref_rhs, ref_phi, ref_tags = np_(sim.rhs).copy(), np_(sim.phi_cell).copy(), np_(sim.is_liquid).copy()
+    if solids:
+        assert (ref["beta"][0][6:11, 5:11, 2:9] == 0.0).all()
+    for tags_in in (0, 64):
+        for t in sim.vel + sim.vel_star + sim.beta + sim.phi_face:
+            t.fill_(7.0)
+        sim.phi_cell.fill_(7.0)
+        sim.rhs.fill_(7.0)
+        if tags_in:
+            sim.is_liquid.copy_(torch.from_numpy(ref_tags))
+        sim._phase(1 | 2 | 4 | 8 | 16 | 32 | tags_in, dt)
+        torch.cuda.synchronize()
+        for k, arrs in ref.items():
+            for ax in range(3):
+                assert np.array_equal(np_(getattr(sim, k)[ax]), arrs[ax]), (k, ax, tags_in)
+        assert np.array_equal(np_(sim.rhs), ref_rhs), tags_in
+        assert np.array_equal(np_(sim.phi_cell), ref_phi), tags_in
+        assert np.array_equal(np_(sim.is_liquid), ref_tags), tags_in
 
 
 # ------------------------------------------------------------------ operator / solver
