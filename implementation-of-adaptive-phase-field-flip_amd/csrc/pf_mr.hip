@@ -524,6 +524,77 @@ __device__ inline float sample_face(const MGrid& g, const float* __restrict__ f,
     return wsum > 0.0f ? acc / wsum : 0.0f;
 }
 
+// G2P samples two fields at one point through the same taps: the taps of
+// the three staggered components (weight and cell address; -1: not summed,
+// a zero weight or no cell) are resolved once and the 48 values loaded
+// together (973 -> 830 us at 9.1M particles).  The advection's three
+// samples at three points keep sample_face: with the taps batched per point
+// it took 1.38 ms instead of 1.20 (more registers, the out-of-block lookups
+// still serial).
+struct Taps3 {
+    float w[3][8];
+    int addr[3][8];
+};
+
+__device__ inline void sample_taps(const MGrid& g, float x, float y, float z, const SamplePoint& sp, Taps3& t) {
+    const MCell& c = sp.c;
+    const BlockRef& br = sp.br;
+    const int ext = 1 << c.level;
+    const float h = cell_size(g, c.level);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const float sx = x / h - (ax == 0 ? 0.0f : 0.5f);
+        const float sy = y / h - (ax == 1 ? 0.0f : 0.5f);
+        const float sz = z / h - (ax == 2 ? 0.0f : 0.5f);
+        const float fi = floorf(sx), fj = floorf(sy), fk = floorf(sz);
+        const int i0 = (int)fi, j0 = (int)fj, k0 = (int)fk;
+        const float tx = sx - fi, ty = sy - fj, tz = sz - fk;
+#pragma unroll
+        for (int di = 0; di < 2; ++di) {
+            const float wi = di == 0 ? 1.0f - tx : tx;
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj) {
+                const float wj = dj == 0 ? 1.0f - ty : ty;
+#pragma unroll
+                for (int dk = 0; dk < 2; ++dk) {
+                    const float wk = dk == 0 ? 1.0f - tz : tz;
+                    const int q = di * 4 + dj * 2 + dk;
+                    const float w = wi * wj * wk;
+                    t.w[ax][q] = w;
+                    t.addr[ax][q] = w > 0.0f
+                                        ? locate_near(g, br, (i0 + di) * ext, (j0 + dj) * ext, (k0 + dk) * ext).addr
+                                        : -1;
+                }
+            }
+        }
+    }
+}
+
+// The tap values of field f (address -1 reads entry 0, masked in the sum):
+// all 24 loads are issued before the first is used.
+__device__ inline void tap_values(const float* __restrict__ f, const Taps3& t, float (&v)[3][8]) {
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[ax][q] = f[max(t.addr[ax][q], 0) * 3 + ax];
+}
+
+// trilinear face samples from the taps, normalised by the weight of the taps
+// that exist, summed in tap order (mr_kernels.py:359-405)
+__device__ inline void tap_sum(const SamplePoint& sp, const Taps3& t, const float (&v)[3][8], float (&out)[3]) {
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        float acc = 0.0f, wsum = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (t.addr[ax][q] >= 0) {
+                acc += t.w[ax][q] * v[ax][q];
+                wsum += t.w[ax][q];
+            }
+        out[ax] = sp.c.addr < 0 ? 0.0f : (wsum > 0.0f ? acc / wsum : 0.0f);
+    }
+}
+
 // alpha_FLIP from the local grid spacing (particles.py:251-276), then
 // u_p = alpha (u_p + I(du)) + (1 - alpha) I(u), eq. (12) (mr_kernels.py:413-430)
 __global__ __launch_bounds__(256) void k_mr_g2p(MGrid g, const float* __restrict__ x, float* __restrict__ v,
@@ -538,12 +609,19 @@ __global__ __launch_bounds__(256) void k_mr_g2p(MGrid g, const float* __restrict
     const float dx = cell_size(g, lv);
     const float nu = type[p] == 1 ? c.nu_l : c.nu_g;
     const float alpha = fminf(fmaxf(1.0f - 6.0f * nu * c.dt / (dx * dx), 0.0f), c.alpha_cap);
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        const float delta = sample_face(g, fdvel, px, py, pz, ax, sp);
-        const float pic = sample_face(g, fvel, px, py, pz, ax, sp);
-        v[3 * p + ax] = alpha * (v[3 * p + ax] + delta) + (1.0f - alpha) * pic;
+    // both fields through the same taps, all 48 values loaded at once
+    float delta[3] = {0.0f, 0.0f, 0.0f}, pic[3] = {0.0f, 0.0f, 0.0f};
+    if (cl.addr >= 0) {
+        Taps3 t;
+        sample_taps(g, px, py, pz, sp, t);
+        float vd[3][8], vu[3][8];
+        tap_values(fdvel, t, vd);
+        tap_values(fvel, t, vu);
+        tap_sum(sp, t, vd, delta);
+        tap_sum(sp, t, vu, pic);
     }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) v[3 * p + ax] = alpha * (v[3 * p + ax] + delta[ax]) + (1.0f - alpha) * pic[ax];
 }
 
 // Ralston RK3 through the MSBG velocity, clamped to [lo, hi]
